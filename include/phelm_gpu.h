@@ -1,0 +1,167 @@
+/*
+ * phelm_gpu.h - C ABI of the MI355X (gfx950) sphere-FEM solve path.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes
+ * (0 = ok, < 0 = error, text in phg_last_error()).  Vector arguments named
+ * d_* are device pointers owned by the caller (the reference passes device
+ * arrays under its GPU build, examples/schafe_impl.h:134-149); calls are
+ * ordered on the stream given (NULL = the default stream).
+ *
+ * Each entry point names the reference interface it replaces.  The
+ * reference's solver library (Solver<T>, vec_* helpers) is external and not
+ * in its tree; its call sites are cited instead.
+ */
+#ifndef PHELM_GPU_H
+#define PHELM_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct phg_mesh_s *phg_mesh_t;
+typedef struct phg_matrix_s *phg_matrix_t;
+typedef void *phg_stream_t; /* a hipStream_t */
+
+enum {
+    PHG_OK = 0,
+    PHG_ERR_ARG = -1,      /* bad argument                               */
+    PHG_ERR_FORMAT = -2,   /* mesh text rejected (Mesh::load "bad file") */
+    PHG_ERR_HIP = -3,      /* HIP runtime error                          */
+    PHG_ERR_NOMEM = -4,    /* allocation failed / over admission budget  */
+    PHG_ERR_NOCONV = 1     /* solver hit max_iters (not an error: result valid) */
+};
+
+/* value precisions of a matrix / solve */
+enum { PHG_F64 = 0, PHG_F32 = 1, PHG_BF16 = 2 };
+/* element kinds (callbacks of src/slaplace.cpp:171-197 and
+ * examples/barvortex_impl.h:57-80) */
+enum { PHG_MASS = 0, PHG_LAPLACE = 1, PHG_HELMHOLTZ = 2 };
+/* orthogonalisation of the Arnoldi step */
+enum { PHG_MGS = 0, PHG_CGS2 = 1 };
+
+/* ---- library ---------------------------------------------------------- */
+/* Version string and last error of the calling thread. */
+const char *phg_version(void);
+const char *phg_last_error(void);
+/* Selects the HIP device for later calls of this thread (linal_init). */
+int phg_init(int device);
+
+/* ---- mesh (src/mesh.cpp:323-519 Mesh::load; utils/icosahedron.cpp:481-624) */
+/* Text of `mke_mesh_sphere --type full --coord local --iter level`; the
+ * buffer is malloc'ed by the library, free it with phg_free(). */
+int phg_mesh_sphere_text(int level, char **text, size_t *len);
+void phg_free(void *p);
+/* Mesh::load of a P1 mesh text (src/mesh.cpp:323-519). */
+int phg_mesh_load(const char *text, size_t len, phg_mesh_t *out);
+/* phg_mesh_sphere_text + phg_mesh_load without the caller seeing the text. */
+int phg_mesh_sphere(int level, phg_mesh_t *out);
+/* Sizes: m.ps.size(), m.tr.size(), inner_size, outer_size, zone count. */
+int phg_mesh_info(phg_mesh_t m, int *n_points, int *n_triangles,
+                  int *n_inner, int *n_outer, int *n_zones);
+/* Host copies of the loaded arrays (any pointer may be NULL):
+ * pr[V*3], coords[V*n_zones*2] (chart x = latitude, y = longitude),
+ * tri[T*3] (0-based), tri_zone[T] (0-based), adj_ptr[V+1], adj[3T]
+ * (triangle-id order), is_boundary[V], inner[n_inner], outer[n_outer],
+ * p2io[V] (mesh.h:354-427). */
+int phg_mesh_arrays(phg_mesh_t m, double *pr, double *coords, int *tri,
+                    int *tri_zone, int *adj_ptr, int *adj, int *is_boundary,
+                    int *inner, int *outer, int *p2io);
+void phg_mesh_destroy(phg_mesh_t m);
+/* CSR pattern of the inner x inner operator that phg_assemble fills: row i
+ * holds p2io of every inner vertex of every triangle around inner[i],
+ * columns ascending (the Solver::add pattern of src/generators.cpp:71-110).
+ * row_ptr has n_inner + 1 entries; pass col_idx = NULL to get *nnz first.
+ * Host only. */
+int phg_mesh_csr_structure(phg_mesh_t m, int *row_ptr, int *col_idx, long *nnz);
+
+/* ---- matrices (Solver<T> of the external solver library) --------------- */
+/* Solver<T>(n) + add(i, j, a) + first use: an n_rows x n_cols matrix built
+ * from host triplets; duplicates accumulate in the order given, like
+ * Solver::add (src/generators.cpp:49-55, src/impl/generators_impl.h:38-72). */
+int phg_matrix_from_triplets(int n_rows, int n_cols, long n, const int *rows,
+                             const int *cols, const double *vals,
+                             phg_matrix_t *out);
+/* Same from a host CSR (columns ascending in each row). */
+int phg_matrix_from_csr(int n_rows, int n_cols, const int *row_ptr,
+                        const int *col_idx, const double *vals,
+                        phg_matrix_t *out);
+/* generate_matrix(A, m, cb) (src/generators.cpp:116-139) on the GPU for the
+ * inner x inner operator.  kind selects cb: PHG_MASS = id_cb
+ * (src/slaplace.cpp:185-197), PHG_LAPLACE = slaplace_integrate_cb
+ * (src/slaplace.cpp:171-183), PHG_HELMHOLTZ = c_mass * mass + c_lap *
+ * stiffness per triangle (examples/barvortex_impl.h:57-80 with c_mass =
+ * 1/tau + sigma*theta, c_lap = -theta*mu; examples/schafe.cpp:67-89 with
+ * theta = 1/2).  CSR structure is built on the host, values on the device,
+ * bit-identical to the reference's row-wise order. */
+int phg_assemble(phg_mesh_t m, int kind, double c_mass, double c_lap,
+                 phg_stream_t stream, phg_matrix_t *out);
+/* generate_boundary_matrix(A, m, cb) (src/generators.cpp:188-212): the
+ * inner x outer matrix, same kinds. */
+int phg_assemble_boundary(phg_mesh_t m, int kind, double c_mass, double c_lap,
+                          phg_matrix_t *out);
+/* Shape and host copies of the CSR (pointers may be NULL). */
+int phg_matrix_info(phg_matrix_t A, int *n_rows, int *n_cols, long *nnz);
+int phg_matrix_csr(phg_matrix_t A, int *row_ptr, int *col_idx, double *vals);
+void phg_matrix_destroy(phg_matrix_t A);
+
+/* ---- products (Solver::mult_vector, 68 call sites, e.g.
+ *      src/impl/slaplace_impl.h:109,168-169) ------------------------------- */
+/* y = A x, FP64, device vectors of n_cols / n_rows entries. */
+int phg_spmv(phg_matrix_t A, const double *d_x, double *d_y, phg_stream_t s);
+/* y = A x with FP32 vectors and values stored as prec (PHG_F32 / PHG_BF16). */
+int phg_spmv_f32(phg_matrix_t A, int prec, const float *d_x, float *d_y,
+                 phg_stream_t s);
+/* Y = A X for nrhs right-hand sides, row-major n x nrhs (ld = nrhs), FP32. */
+int phg_spmm_f32(phg_matrix_t A, int prec, int nrhs, const float *d_X,
+                 float *d_Y, phg_stream_t s);
+
+/* ---- BLAS-1 (vec_scalar2, vec_sum1, vec_mult_scalar, vec_diff: e.g.
+ *      src/mesh.h:550-591, examples/barvortex_impl.h:234-270) ------------- */
+int phg_dot(int n, const double *d_x, const double *d_y, double *result,
+            phg_stream_t s);   /* result: host pointer, synchronises */
+int phg_nrm2(int n, const double *d_x, double *result, phg_stream_t s);
+/* r = k1 * a + k2 * b  (vec_sum1) */
+int phg_axpby(int n, double k1, const double *d_a, double k2,
+              const double *d_b, double *d_r, phg_stream_t s);
+
+/* ---- gather / scatter (src/proj.cpp:49-160, src/gpu/phelm.cu:45-157) ----- */
+/* p[i] = u[inner[i]]  (u2p) */
+int phg_u2p(phg_mesh_t m, const double *d_u, double *d_p, phg_stream_t s);
+/* u[i] = boundary ? (bnd ? bnd[p2io[i]] : 0) : p[p2io[i]]  (p2u) */
+int phg_p2u(phg_mesh_t m, const double *d_p, const double *d_bnd, double *d_u,
+            phg_stream_t s);
+
+/* ---- GMRES (Solver::solve: src/mesh.h:740-746, src/impl/slaplace_impl.h:171) */
+typedef struct {
+    int restart;     /* m of GMRES(m), default 30                          */
+    int max_iters;   /* Arnoldi steps in total (inner steps for "mixed")   */
+    double rtol;     /* stop at ||b - A x||_2 / ||b||_2 <= rtol, FP64       */
+    int ortho;       /* PHG_MGS (reference form) or PHG_CGS2               */
+    int precision;   /* PHG_F64: plain form.  PHG_F32 / PHG_BF16: inner
+                        Krylov in FP32 (values stored as precision) under
+                        FP64 iterative refinement                          */
+    double inner_rtol; /* relative reduction per refinement step (mixed)   */
+    int use_x0;      /* 0: start from x = 0; 1: start from the given x      */
+} phg_gmres_opts;
+
+typedef struct {
+    int iterations;       /* Arnoldi steps executed                       */
+    int restarts;         /* completed restart cycles                     */
+    int refinements;      /* outer refinement steps (mixed forms)         */
+    double rel_residual;  /* true FP64 ||b - A x|| / ||b|| at exit          */
+    int converged;
+} phg_gmres_info;
+
+void phg_gmres_default_opts(phg_gmres_opts *o);
+/* Solves A x = b; d_b and d_x are device vectors of n_rows entries. */
+int phg_gmres(phg_matrix_t A, const double *d_b, double *d_x,
+              const phg_gmres_opts *opts, phg_gmres_info *info,
+              phg_stream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHELM_GPU_H */

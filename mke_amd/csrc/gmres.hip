@@ -1,0 +1,562 @@
+// Restarted GMRES(m) on gfx950, device resident.
+//
+// Replaces Solver<T>::solve of the reference's external solver library
+// (call sites src/mesh.h:740-746, src/impl/slaplace_impl.h:171).  One restart
+// cycle = m Arnoldi steps; each cycle begins from the true FP64 residual
+// r = b - A x, so the same driver gives
+//   * the plain FP64 form (vectors, matrix and Krylov basis in FP64), and
+//   * the mixed form: Krylov basis, work vector and Arnoldi SpMV in FP32
+//     (matrix values stored as FP32 or BF16), with FP64 kept for the
+//     residual r = b - A x, every dot/norm accumulation, the Hessenberg
+//     matrix and its Givens rotations, and the update x += V y.  Restarting
+//     from the FP64 residual makes each cycle one step of iterative
+//     refinement.
+//
+// Orthogonalisation: modified Gram-Schmidt (one kernel per basis vector, the
+// reference form) or classical Gram-Schmidt with one re-orthogonalisation
+// pass (CGS2: the j+1 dots of a pass are computed together).
+//
+// Reductions are deterministic: every block writes one partial per dot, and
+// the consumer adds the partials in block order (redundantly in every block
+// of the next kernel, or in a one-block kernel).  Results therefore depend
+// only on the inputs and the launch shape, never on timing.
+//
+// Scalars (H, Givens rotations, g, convergence flags) live in device memory;
+// the host reads 40 bytes once per restart cycle.
+#include <cmath>
+#include <cstring>
+
+#include "phg_internal.h"
+
+namespace phg {
+
+constexpr int GM_MAXM = 64;
+constexpr int GM_BS = 256;
+constexpr int GM_RPT = 4;     // max rows per thread
+constexpr int GM_MAXG = 1024; // max blocks
+
+struct GmresState {
+    double H[(GM_MAXM + 1) * GM_MAXM];
+    double cs[GM_MAXM], sn[GM_MAXM], g[GM_MAXM + 1], y[GM_MAXM];
+    double hc[GM_MAXM + 1];        // CGS2 coefficients of the current pass
+    double beta, bnorm, rel, rtol, inner_rtol, vscale;
+    int m, active, k, iters, max_iters, converged, cycles, pad;
+};
+
+struct GmresWork {
+    int n = 0, m = 0, G = 0, rpt = 1, prec = 0;
+    size_t ldv = 0;
+    DevBuf<char> V, W;     // (m+1) x ldv and 2 x ldv vectors of the Krylov type
+    DevBuf<double> part;   // (m+2) x G partial sums
+    DevBuf<GmresState> st;
+    GmresState* host = nullptr;  // pinned mirror
+    ~GmresWork()
+    {
+        if (host) (void)hipHostFree(host);
+    }
+};
+
+void destroy_gmres_work(GmresWork* w) { delete w; }
+
+namespace {
+
+// ---------------------------------------------------------------- helpers --
+__device__ __forceinline__ double wave_sum(double v)
+{
+    // xor butterfly: every lane ends with the same, order-fixed sum
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum of a block's per-thread values, identical in every thread.
+__device__ __forceinline__ double block_sum(double v, double* sh)
+{
+    v = wave_sum(v);
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    for (int w = 0; w < GM_BS / 64; ++w) r += sh[w];
+    __syncthreads();
+    return r;
+}
+
+// Sum of G block partials in a fixed order, computed by each wavefront.
+__device__ __forceinline__ double sum_partials(const double* __restrict__ p, int G)
+{
+    const int lane = threadIdx.x & 63;
+    double s = 0.0;
+    for (int b = lane; b < G; b += 64) s += p[b];
+    return wave_sum(s);
+}
+
+template <typename T>
+__device__ __forceinline__ double to_d(T v) { return (double)v; }
+template <typename V>
+__device__ __forceinline__ float mval(V v);
+template <>
+__device__ __forceinline__ float mval<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ float mval<uint16_t>(uint16_t v) { return __uint_as_float(((uint32_t)v) << 16); }
+
+// y_row of the Arnoldi SpMV with the source vector scaled on the fly.
+template <typename VT, typename MT>
+struct RowDot;
+template <>
+struct RowDot<double, double> {
+    __device__ __forceinline__ static double run(int n, int width, int row, const int* col,
+                                                 const double* val, const double* x,
+                                                 double inv)
+    {
+        double s = 0.0;
+        for (int k = 0; k < width; ++k) {
+            const size_t o = (size_t)k * n + row;
+            s += val[o] * (x[col[o]] * inv);
+        }
+        return s;
+    }
+};
+template <typename MT>
+struct RowDot<float, MT> {
+    __device__ __forceinline__ static float run(int n, int width, int row, const int* col,
+                                                const MT* val, const float* x, float inv)
+    {
+        float s = 0.0f;
+        for (int k = 0; k < width; ++k) {
+            const size_t o = (size_t)k * n + row;
+            s = fmaf(mval(val[o]), x[col[o]] * inv, s);
+        }
+        return s;
+    }
+};
+
+// ---------------------------------------------------------------- kernels --
+// r = b - A x in FP64 (the refinement residual); W0 = r; partial ||r||^2.
+template <typename VT>
+__global__ void __launch_bounds__(GM_BS)
+k_residual(int n, int rpt, int width, const int* __restrict__ col,
+           const double* __restrict__ val, const double* __restrict__ b,
+           const double* __restrict__ x, VT* __restrict__ W0, double* __restrict__ part,
+           const GmresState* __restrict__ st)
+{
+    __shared__ double sh[GM_BS / 64];
+    if (st->converged) return;
+    double acc = 0.0;
+    const int base = blockIdx.x * GM_BS * rpt + threadIdx.x;
+    for (int r = 0; r < rpt; ++r) {
+        const int row = base + r * GM_BS;
+        if (row >= n) break;
+        double s = 0.0;
+        for (int k = 0; k < width; ++k) {
+            const size_t o = (size_t)k * n + row;
+            s += val[o] * x[col[o]];
+        }
+        const double rr = b[row] - s;
+        W0[row] = (VT)rr;
+        acc += rr * rr;
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ void __launch_bounds__(GM_BS)
+k_sumsq(int n, int rpt, const double* __restrict__ b, double* __restrict__ part)
+{
+    __shared__ double sh[GM_BS / 64];
+    double acc = 0.0;
+    const int base = blockIdx.x * GM_BS * rpt + threadIdx.x;
+    for (int r = 0; r < rpt; ++r) {
+        const int row = base + r * GM_BS;
+        if (row >= n) break;
+        acc += b[row] * b[row];
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ void k_set_bnorm(GmresState* st, const double* __restrict__ part, int G)
+{
+    const double s = sum_partials(part, G);
+    if (threadIdx.x == 0) {
+        double bn = sqrt(s);
+        st->bnorm = bn > 0.0 ? bn : 1.0;
+        st->iters = 0;
+        st->cycles = 0;
+        st->converged = 0;
+        st->active = 0;
+        st->k = 0;
+    }
+}
+
+// Start of a restart cycle from the residual partials.
+__global__ void k_cycle_start(GmresState* st, const double* __restrict__ part, int G)
+{
+    if (st->converged) return;
+    const double s = sum_partials(part, G);
+    if (threadIdx.x == 0) {
+        const double beta = sqrt(s);
+        st->beta = beta;
+        st->rel = beta / st->bnorm;
+        st->k = 0;
+        if (st->rel <= st->rtol || beta == 0.0) {
+            st->converged = 1;
+            st->active = 0;
+        } else if (st->iters >= st->max_iters) {
+            st->active = 0;
+        } else {
+            st->active = 1;
+            st->vscale = beta;
+            st->g[0] = beta;
+            for (int i = 1; i <= st->m; ++i) st->g[i] = 0.0;
+            st->cycles += 1;
+        }
+    }
+}
+
+// Arnoldi step j: v_j = W_src / vscale (stored), W_dst = A v_j, and the
+// partial of <W_dst, v_0> (MGS) when ndots = 1, or of <W_dst, v_i> for all
+// i <= j (CGS2 first pass) when ndots = j + 1.
+template <typename VT, typename MT>
+__global__ void __launch_bounds__(GM_BS)
+k_step_spmv(int n, int rpt, int width, const int* __restrict__ col, const MT* __restrict__ val,
+            const VT* __restrict__ Wsrc, VT* __restrict__ V, size_t ldv, VT* __restrict__ Wdst,
+            int j, int ndots, double* __restrict__ part, int G, const GmresState* __restrict__ st)
+{
+    __shared__ double sh[GM_BS / 64];
+    if (!st->active) return;
+    const VT inv = (VT)(1.0 / st->vscale);
+    VT* Vj = V + (size_t)j * ldv;
+    VT w[GM_RPT];
+    const int base = blockIdx.x * GM_BS * rpt + threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < GM_RPT; ++r) {
+        const int row = base + r * GM_BS;
+        w[r] = VT(0);
+        if (r < rpt && row < n) {
+            Vj[row] = Wsrc[row] * inv;
+            w[r] = RowDot<VT, MT>::run(n, width, row, col, val, Wsrc, inv);
+            Wdst[row] = w[r];
+        }
+    }
+    for (int i = 0; i < ndots; ++i) {
+        const VT* Vi = V + (size_t)i * ldv;
+        double acc = 0.0;
+#pragma unroll
+        for (int r = 0; r < GM_RPT; ++r) {
+            const int row = base + r * GM_BS;
+            if (r < rpt && row < n) {
+                const VT vi = (i == j) ? Wsrc[row] * inv : Vi[row];
+                acc += to_d(w[r]) * to_d(vi);
+            }
+        }
+        acc = block_sum(acc, sh);
+        if (threadIdx.x == 0) part[(size_t)i * G + blockIdx.x] = acc;
+    }
+}
+
+// MGS pass i (1 <= i <= j+1): h_{i-1,j} from the partials, w -= h v_{i-1},
+// then the partial of <w, v_i> (i <= j) or of ||w||^2 (i = j + 1).
+template <typename VT>
+__global__ void __launch_bounds__(GM_BS)
+k_mgs(int n, int rpt, VT* __restrict__ w, const VT* __restrict__ V, size_t ldv, int i, int j,
+      double* __restrict__ part, int G, GmresState* __restrict__ st)
+{
+    __shared__ double sh[GM_BS / 64];
+    if (!st->active) return;
+    const double h = sum_partials(part + (size_t)(i - 1) * G, G);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->H[(i - 1) * st->m + j] = h;
+    const VT hv = (VT)h;
+    const VT* Vp = V + (size_t)(i - 1) * ldv;
+    const VT* Vi = V + (size_t)i * ldv;
+    const bool last = (i == j + 1);
+    double acc = 0.0;
+    const int base = blockIdx.x * GM_BS * rpt + threadIdx.x;
+    for (int r = 0; r < rpt; ++r) {
+        const int row = base + r * GM_BS;
+        if (row >= n) break;
+        const VT wr = w[row] - hv * Vp[row];
+        w[row] = wr;
+        const double o = last ? to_d(wr) : to_d(Vi[row]);
+        acc += to_d(wr) * o;
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[(size_t)i * G + blockIdx.x] = acc;
+}
+
+// CGS2: one block adds the partials of the j+1 dots of a pass (thread i
+// handles dot i); pass 0 stores them, pass 1 adds the correction into H.
+__global__ void k_cgs_reduce(int j, int pass, const double* __restrict__ part, int G,
+                             GmresState* __restrict__ st)
+{
+    if (!st->active) return;
+    const int i = threadIdx.x;
+    if (i > j) return;
+    double s = 0.0;
+    for (int b = 0; b < G; ++b) s += part[(size_t)i * G + b];
+    st->hc[i] = s;
+    if (pass == 0) st->H[i * st->m + j] = s;
+    else st->H[i * st->m + j] += s;
+}
+
+// CGS2 update: w -= sum_i hc_i v_i (i = 0..j); then partial dots with all
+// v_i (second pass) or the partial of ||w||^2 (final).
+template <typename VT>
+__global__ void __launch_bounds__(GM_BS)
+k_cgs_update(int n, int rpt, VT* __restrict__ w, const VT* __restrict__ V, size_t ldv, int j,
+             int final_pass, double* __restrict__ part, int G, const GmresState* __restrict__ st)
+{
+    __shared__ double sh[GM_BS / 64];
+    __shared__ double hs[GM_MAXM + 1];
+    if (!st->active) return;
+    if ((int)threadIdx.x <= j) hs[threadIdx.x] = st->hc[threadIdx.x];
+    __syncthreads();
+    VT wv[GM_RPT];
+    const int base = blockIdx.x * GM_BS * rpt + threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < GM_RPT; ++r) {
+        const int row = base + r * GM_BS;
+        wv[r] = VT(0);
+        if (r < rpt && row < n) {
+            VT a = w[row];
+            for (int i = 0; i <= j; ++i) a = a - (VT)hs[i] * V[(size_t)i * ldv + row];
+            wv[r] = a;
+            w[row] = a;
+        }
+    }
+    const int ndots = final_pass ? 1 : j + 1;
+    for (int i = 0; i < ndots; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int r = 0; r < GM_RPT; ++r) {
+            const int row = base + r * GM_BS;
+            if (r < rpt && row < n) {
+                const double o = final_pass ? to_d(wv[r]) : to_d(V[(size_t)i * ldv + row]);
+                acc += to_d(wv[r]) * o;
+            }
+        }
+        acc = block_sum(acc, sh);
+        if (threadIdx.x == 0) part[(size_t)(final_pass ? j + 1 : i) * G + blockIdx.x] = acc;
+    }
+}
+
+// Norm, Givens rotations and the residual estimate of step j (one wave).
+__global__ void k_givens(int j, const double* __restrict__ part, int G,
+                         GmresState* __restrict__ st)
+{
+    if (!st->active) return;
+    const double s = sum_partials(part + (size_t)(j + 1) * G, G);
+    if (threadIdx.x != 0) return;
+    const int m = st->m;
+    double* H = st->H;
+    const double hn = sqrt(s);
+    H[(j + 1) * m + j] = hn;
+    st->vscale = hn;
+    for (int i = 0; i < j; ++i) {
+        const double t = st->cs[i] * H[i * m + j] + st->sn[i] * H[(i + 1) * m + j];
+        H[(i + 1) * m + j] = -st->sn[i] * H[i * m + j] + st->cs[i] * H[(i + 1) * m + j];
+        H[i * m + j] = t;
+    }
+    const double a = H[j * m + j], c = H[(j + 1) * m + j];
+    const double r = hypot(a, c);
+    st->cs[j] = a / r;
+    st->sn[j] = c / r;
+    H[j * m + j] = r;
+    H[(j + 1) * m + j] = 0.0;
+    st->g[j + 1] = -st->sn[j] * st->g[j];
+    st->g[j] = st->cs[j] * st->g[j];
+    st->k = j + 1;
+    st->iters += 1;
+    // end the cycle: target reached (or the refinement step's share), Krylov
+    // space exhausted (hn = 0), step budget spent, or cycle full
+    const double est = fabs(st->g[j + 1]);
+    const double stop = fmax(st->rtol * st->bnorm, st->inner_rtol * st->beta);
+    if (est <= stop || !(hn > 0.0) || st->iters >= st->max_iters || j + 1 == m) st->active = 0;
+}
+
+// y = H^{-1} g (k x k upper triangular), one thread.
+__global__ void k_solve_y(GmresState* __restrict__ st)
+{
+    if (st->converged || threadIdx.x != 0) return;
+    const int k = st->k, m = st->m;
+    for (int i = k - 1; i >= 0; --i) {
+        double s = st->g[i];
+        for (int l = i + 1; l < k; ++l) s -= st->H[i * m + l] * st->y[l];
+        st->y[i] = s / st->H[i * m + i];
+    }
+}
+
+// x += sum_i y_i v_i, accumulated in FP64.
+template <typename VT>
+__global__ void __launch_bounds__(GM_BS)
+k_xupdate(int n, int rpt, const VT* __restrict__ V, size_t ldv, double* __restrict__ x,
+          const GmresState* __restrict__ st)
+{
+    __shared__ double ys[GM_MAXM];
+    if (st->converged) return;
+    const int k = st->k;
+    if (k == 0) return;
+    if ((int)threadIdx.x < k) ys[threadIdx.x] = st->y[threadIdx.x];
+    __syncthreads();
+    const int base = blockIdx.x * GM_BS * rpt + threadIdx.x;
+    for (int r = 0; r < rpt; ++r) {
+        const int row = base + r * GM_BS;
+        if (row >= n) break;
+        double s = 0.0;
+        for (int i = 0; i < k; ++i) s += to_d(V[(size_t)i * ldv + row]) * ys[i];
+        x[row] += s;
+    }
+}
+
+__global__ void k_fill0(int n, double* __restrict__ x)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.0;
+}
+
+// ---------------------------------------------------------------- driver --
+int prepare_work(phg_matrix_s* A, int prec, int m, GmresWork** out)
+{
+    auto& slot = A->work[prec];
+    const int n = A->n_rows;
+    if (!slot || slot->m < m) {
+        std::unique_ptr<GmresWork> w(new GmresWork());
+        w->n = n;
+        w->m = m;
+        w->prec = prec;
+        int G = std::max(1, std::min((n + GM_BS - 1) / GM_BS, 512));
+        int rpt = (n + G * GM_BS - 1) / (G * GM_BS);
+        if (rpt > GM_RPT) {
+            rpt = GM_RPT;
+            G = (n + GM_BS * GM_RPT - 1) / (GM_BS * GM_RPT);
+        }
+        if (G > GM_MAXG) {
+            set_error("gmres: matrix too large for one device workspace");
+            return PHG_ERR_ARG;
+        }
+        w->G = G;
+        w->rpt = std::max(rpt, 1);
+        w->ldv = ((size_t)n + 63) / 64 * 64;
+        const size_t es = prec == PHG_F64 ? 8 : 4;
+        PHG_HIP(w->V.alloc((size_t)(m + 1) * w->ldv * es));
+        PHG_HIP(w->W.alloc(2 * w->ldv * es));
+        PHG_HIP(w->part.alloc((size_t)(m + 2) * G));
+        PHG_HIP(w->st.alloc(1));
+        PHG_HIP(hipHostMalloc((void**)&w->host, sizeof(GmresState), hipHostMallocDefault));
+        slot.reset(w.release());
+    }
+    *out = slot.get();
+    return PHG_OK;
+}
+
+template <typename VT, typename MT>
+int gmres_run(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double* b, double* x,
+              const phg_gmres_opts& o, phg_gmres_info* info, hipStream_t s)
+{
+    const int n = w->n, m = o.restart, G = w->G, rpt = w->rpt;
+    const size_t ldv = w->ldv;
+    VT* V = (VT*)w->V.p;
+    VT* W0 = (VT*)w->W.p;
+    VT* W1 = W0 + ldv;
+    double* part = w->part.p;
+    GmresState* st = w->st.p;
+    const dim3 grid(G), block(GM_BS);
+
+    // parameters of the solve
+    GmresState init;
+    memset(&init, 0, sizeof init);
+    init.m = m;
+    init.rtol = o.rtol;
+    init.inner_rtol = (o.precision == PHG_F64) ? 0.0 : o.inner_rtol;
+    init.max_iters = o.max_iters;
+    PHG_HIP(hipMemcpyAsync(st, &init, sizeof init, hipMemcpyHostToDevice, s));
+    if (!o.use_x0) {
+        hipLaunchKernelGGL(k_fill0, dim3((n + 255) / 256), dim3(256), 0, s, n, x);
+    }
+    hipLaunchKernelGGL(k_sumsq, grid, block, 0, s, n, rpt, b, part);
+    hipLaunchKernelGGL(k_set_bnorm, dim3(1), dim3(64), 0, s, st, part, G);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_residual<VT>), grid, block, 0, s, n, rpt,
+                       A->ell64.width, A->ell64.col.p, A->ell64.val.p, b, x, W0, part, st);
+    hipLaunchKernelGGL(k_cycle_start, dim3(1), dim3(64), 0, s, st, part, G);
+    PHG_HIP(hipGetLastError());
+
+    const bool cgs = o.ortho == PHG_CGS2;
+    for (;;) {
+        PHG_HIP(hipMemcpyAsync(w->host, st, sizeof(GmresState), hipMemcpyDeviceToHost, s));
+        PHG_HIP(hipStreamSynchronize(s));
+        const GmresState& h = *w->host;
+        if (h.converged || !h.active) break;
+        for (int j = 0; j < m; ++j) {
+            VT* src = (j & 1) ? W1 : W0;
+            VT* dst = (j & 1) ? W0 : W1;
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_step_spmv<VT, MT>), grid, block, 0, s, n, rpt,
+                               E.width, E.col.p, E.val.p, src, V, ldv, dst, j,
+                               cgs ? j + 1 : 1, part, G, st);
+            if (!cgs) {
+                for (int i = 1; i <= j + 1; ++i)
+                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mgs<VT>), grid, block, 0, s, n, rpt, dst,
+                                       V, ldv, i, j, part, G, st);
+            } else {
+                hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(64), 0, s, j, 0, part, G, st);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cgs_update<VT>), grid, block, 0, s, n, rpt,
+                                   dst, V, ldv, j, 0, part, G, st);
+                hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(64), 0, s, j, 1, part, G, st);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cgs_update<VT>), grid, block, 0, s, n, rpt,
+                                   dst, V, ldv, j, 1, part, G, st);
+            }
+            hipLaunchKernelGGL(k_givens, dim3(1), dim3(64), 0, s, j, part, G, st);
+        }
+        PHG_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_solve_y, dim3(1), dim3(64), 0, s, st);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_xupdate<VT>), grid, block, 0, s, n, rpt, V, ldv, x,
+                           st);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_residual<VT>), grid, block, 0, s, n, rpt,
+                           A->ell64.width, A->ell64.col.p, A->ell64.val.p, b, x, W0, part, st);
+        hipLaunchKernelGGL(k_cycle_start, dim3(1), dim3(64), 0, s, st, part, G);
+        PHG_HIP(hipGetLastError());
+    }
+    const GmresState& h = *w->host;
+    if (info) {
+        info->iterations = h.iters;
+        info->restarts = h.cycles;
+        info->refinements = o.precision == PHG_F64 ? 0 : h.cycles;
+        info->rel_residual = h.rel;
+        info->converged = h.converged;
+    }
+    return h.converged ? PHG_OK : PHG_ERR_NOCONV;
+}
+
+}  // namespace
+
+int gmres_solve(phg_matrix_s* A, const double* b, double* x, const phg_gmres_opts* opts,
+                phg_gmres_info* info, hipStream_t s)
+{
+    phg_gmres_opts o = *opts;
+    if (o.restart < 1 || o.restart > GM_MAXM) {
+        set_error("gmres: restart must be in 1..64");
+        return PHG_ERR_ARG;
+    }
+    if (A->n_rows != A->n_cols) {
+        set_error("gmres: matrix is not square");
+        return PHG_ERR_ARG;
+    }
+    if (A->n_rows == 0) {
+        if (info) memset(info, 0, sizeof *info), info->converged = 1;
+        return PHG_OK;
+    }
+    int rc = ensure_ell64(A, s);
+    if (rc) return rc;
+    GmresWork* w = nullptr;
+    rc = prepare_work(A, o.precision, o.restart, &w);
+    if (rc) return rc;
+    if (o.precision == PHG_F64) return gmres_run<double, double>(A, A->ell64, w, b, x, o, info, s);
+    if (o.precision == PHG_F32) {
+        rc = ensure_ell32(A, s);
+        if (rc) return rc;
+        return gmres_run<float, float>(A, A->ell32, w, b, x, o, info, s);
+    }
+    rc = ensure_ell16(A, s);
+    if (rc) return rc;
+    return gmres_run<float, uint16_t>(A, A->ell16, w, b, x, o, info, s);
+}
+
+}  // namespace phg

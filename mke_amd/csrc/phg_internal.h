@@ -1,0 +1,105 @@
+// Internal objects behind the C ABI of include/phelm_gpu.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/phelm_gpu.h"
+#include "mesh.h"
+
+namespace phg {
+
+void set_error(const std::string& msg);
+
+#define PHG_HIP(call)                                                          \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) {                                                \
+            ::phg::set_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+            return PHG_ERR_HIP;                                                \
+        }                                                                      \
+    } while (0)
+
+// Device buffer owned by an object; freed in the destructor.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count)
+    {
+        release();
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc((void**)&p, count * sizeof(T));
+    }
+    hipError_t upload(const T* h, size_t count, hipStream_t s = nullptr)
+    {
+        hipError_t e = alloc(count);
+        if (e != hipSuccess || count == 0) return e;
+        e = hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return e;
+        return hipStreamSynchronize(s);
+    }
+};
+
+// ELL copy of a CSR matrix in slot-major order: entry k of row i sits at
+// k * n_rows + i, so a wavefront of consecutive rows reads consecutive words.
+// Short rows are padded with value 0 and a column of the same row.
+template <typename V>
+struct Ell {
+    int width = 0;
+    DevBuf<int> col;
+    DevBuf<V> val;
+};
+
+struct GmresWork;  // gmres.hip
+void destroy_gmres_work(GmresWork* w);
+struct GmresWorkDeleter {
+    void operator()(GmresWork* w) const { destroy_gmres_work(w); }
+};
+
+}  // namespace phg
+
+struct phg_mesh_s {
+    phg::SphereMesh h;
+    // device copies for gather/scatter and assembly (uploaded on first use)
+    phg::DevBuf<int> d_inner, d_p2io, d_isb;
+    phg::DevBuf<double> d_coords;
+    phg::DevBuf<int> d_tri;
+};
+
+struct phg_matrix_s {
+    int n_rows = 0, n_cols = 0;
+    long nnz = 0;
+    std::vector<int> h_rp, h_ci;   // host CSR structure (always kept)
+    phg::DevBuf<int> rp, ci;
+    phg::DevBuf<double> val;       // device CSR values (FP64)
+    phg::Ell<double> ell64;
+    phg::Ell<float> ell32;
+    phg::Ell<uint16_t> ell16;      // bfloat16 bit patterns
+    std::unique_ptr<phg::GmresWork, phg::GmresWorkDeleter> work[3];  // per precision
+    ~phg_matrix_s();
+};
+
+namespace phg {
+int ensure_mesh_device(phg_mesh_s* m);
+int ensure_ell64(phg_matrix_s* A, hipStream_t s);
+int ensure_ell32(phg_matrix_s* A, hipStream_t s);
+int ensure_ell16(phg_matrix_s* A, hipStream_t s);
+// matrix with host structure given, device values uploaded from host
+int matrix_from_csr_host(int n_rows, int n_cols, const int* rp, const int* ci,
+                         const double* vals, phg_matrix_s** out);
+}  // namespace phg

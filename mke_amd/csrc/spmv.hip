@@ -1,0 +1,288 @@
+// Sparse products, BLAS-1 and gather/scatter on gfx950.
+//
+// SpMV: thread per row over a slot-major ELL copy of the CSR.  Rows of the
+// sphere operators hold 6-7 entries (nnz = 70*4^k - 11 for n = 10*4^k + 1
+// rows), so ELL of width 7 carries almost no padding and every value/column
+// load of a wavefront is one contiguous 256/512-byte segment.  The x gathers
+// stay within the Cuthill-McKee bandwidth of the row block and hit L2.
+// FP64 products are accumulated in CSR column order from 0.0 without fused
+// multiply-add (this file is compiled with -ffp-contract=off), so y = A x is
+// bit-identical to a plain CPU CSR loop.
+#include "phg_internal.h"
+
+namespace phg {
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t b)
+{
+    return __uint_as_float(((uint32_t)b) << 16);
+}
+
+__host__ __device__ inline uint16_t f64_to_bf16(double v)
+{
+    // round to nearest even on the float value (NaN kept quiet)
+    float f = (float)v;
+    uint32_t u;
+    __builtin_memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (uint16_t)((u >> 16) | 0x40);
+    uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return (uint16_t)(u >> 16);
+}
+
+template <typename V>
+__device__ __forceinline__ float val_f32(V v);
+template <>
+__device__ __forceinline__ float val_f32<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ float val_f32<uint16_t>(uint16_t v) { return bf16_to_f32(v); }
+
+namespace {
+
+template <typename V>
+__global__ void csr_to_ell(int n, int width, const int* __restrict__ rp,
+                           const int* __restrict__ ci, const double* __restrict__ val,
+                           int* __restrict__ ecol, V* __restrict__ eval)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int r0 = rp[i], r1 = rp[i + 1];
+    const int pad_col = r1 > r0 ? ci[r0] : 0;
+    for (int k = 0; k < width; ++k) {
+        const int e = r0 + k;
+        const size_t o = (size_t)k * n + i;
+        if (e < r1) {
+            ecol[o] = ci[e];
+            if constexpr (sizeof(V) == 8) eval[o] = val[e];
+            else if constexpr (sizeof(V) == 4) eval[o] = (float)val[e];
+            else eval[o] = f64_to_bf16(val[e]);
+        } else {
+            ecol[o] = pad_col;
+            eval[o] = V(0);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+ell_spmv_f64(int n, int width, const int* __restrict__ col, const double* __restrict__ val,
+             const double* __restrict__ x, double* __restrict__ y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int k = 0; k < width; ++k) {
+        const size_t o = (size_t)k * n + i;
+        s += val[o] * x[col[o]];
+    }
+    y[i] = s;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256)
+ell_spmv_f32(int n, int width, const int* __restrict__ col, const V* __restrict__ val,
+             const float* __restrict__ x, float* __restrict__ y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int k = 0; k < width; ++k) {
+        const size_t o = (size_t)k * n + i;
+        s = fmaf(val_f32(val[o]), x[col[o]], s);
+    }
+    y[i] = s;
+}
+
+// Y = A X, X/Y row-major n x nrhs.  One wavefront per row: lane r owns
+// right-hand sides r, r+64, ...; the row's columns/values are wave-uniform.
+template <typename V>
+__global__ void __launch_bounds__(256)
+ell_spmm_f32(int n, int width, int nrhs, const int* __restrict__ col,
+             const V* __restrict__ val, const float* __restrict__ X, float* __restrict__ Y)
+{
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (row >= n) return;
+    for (int r = lane; r < nrhs; r += 64) {
+        float s = 0.0f;
+        for (int k = 0; k < width; ++k) {
+            const size_t o = (size_t)k * n + row;
+            s = fmaf(val_f32(val[o]), X[(size_t)col[o] * nrhs + r], s);
+        }
+        Y[(size_t)row * nrhs + r] = s;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+dot_partial(int n, const double* __restrict__ x, const double* __restrict__ y,
+            double* __restrict__ part)
+{
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        s += x[i] * y[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void axpby_kernel(int n, double k1, const double* __restrict__ a, double k2,
+                             const double* __restrict__ b, double* __restrict__ r)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = a[i] * k1 + b[i] * k2;
+}
+
+__global__ void u2p_kernel(int rs, const int* __restrict__ inner, const double* __restrict__ u,
+                           double* __restrict__ p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < rs) p[i] = u[inner[i]];
+}
+
+__global__ void p2u_kernel(int sz, const int* __restrict__ isb, const int* __restrict__ p2io,
+                           const double* __restrict__ p, const double* __restrict__ bnd,
+                           double* __restrict__ u)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= sz) return;
+    if (isb[i]) u[i] = bnd ? bnd[p2io[i]] : 0.0;
+    else u[i] = p[p2io[i]];
+}
+
+int max_row(const phg_matrix_s* A)
+{
+    int w = 0;
+    for (int i = 0; i < A->n_rows; ++i) w = std::max(w, A->h_rp[i + 1] - A->h_rp[i]);
+    return std::max(w, 1);
+}
+
+template <typename V>
+int build_ell(phg_matrix_s* A, Ell<V>& E, hipStream_t s)
+{
+    if (E.width) return PHG_OK;
+    const int n = A->n_rows;
+    const int w = max_row(A);
+    PHG_HIP(E.col.alloc((size_t)w * n));
+    PHG_HIP(E.val.alloc((size_t)w * n));
+    if (n > 0) {
+        hipLaunchKernelGGL(csr_to_ell<V>, dim3((n + 255) / 256), dim3(256), 0, s, n, w,
+                           A->rp.p, A->ci.p, A->val.p, E.col.p, E.val.p);
+        PHG_HIP(hipGetLastError());
+    }
+    PHG_HIP(hipStreamSynchronize(s));
+    E.width = w;
+    return PHG_OK;
+}
+
+}  // namespace
+
+int ensure_ell64(phg_matrix_s* A, hipStream_t s) { return build_ell(A, A->ell64, s); }
+int ensure_ell32(phg_matrix_s* A, hipStream_t s) { return build_ell(A, A->ell32, s); }
+int ensure_ell16(phg_matrix_s* A, hipStream_t s) { return build_ell(A, A->ell16, s); }
+
+int spmv_f64(phg_matrix_s* A, const double* x, double* y, hipStream_t s)
+{
+    int rc = ensure_ell64(A, s);
+    if (rc) return rc;
+    const int n = A->n_rows;
+    if (n == 0) return PHG_OK;
+    hipLaunchKernelGGL(ell_spmv_f64, dim3((n + 255) / 256), dim3(256), 0, s, n,
+                       A->ell64.width, A->ell64.col.p, A->ell64.val.p, x, y);
+    PHG_HIP(hipGetLastError());
+    return PHG_OK;
+}
+
+int spmv_f32(phg_matrix_s* A, int prec, const float* x, float* y, hipStream_t s)
+{
+    const int n = A->n_rows;
+    if (prec == PHG_BF16) {
+        int rc = ensure_ell16(A, s);
+        if (rc) return rc;
+        if (n)
+            hipLaunchKernelGGL(ell_spmv_f32<uint16_t>, dim3((n + 255) / 256), dim3(256), 0, s,
+                               n, A->ell16.width, A->ell16.col.p, A->ell16.val.p, x, y);
+    } else {
+        int rc = ensure_ell32(A, s);
+        if (rc) return rc;
+        if (n)
+            hipLaunchKernelGGL(ell_spmv_f32<float>, dim3((n + 255) / 256), dim3(256), 0, s, n,
+                               A->ell32.width, A->ell32.col.p, A->ell32.val.p, x, y);
+    }
+    PHG_HIP(hipGetLastError());
+    return PHG_OK;
+}
+
+int spmm_f32(phg_matrix_s* A, int prec, int nrhs, const float* X, float* Y, hipStream_t s)
+{
+    const int n = A->n_rows;
+    const dim3 grid((n + 3) / 4), block(256);
+    if (prec == PHG_BF16) {
+        int rc = ensure_ell16(A, s);
+        if (rc) return rc;
+        if (n)
+            hipLaunchKernelGGL(ell_spmm_f32<uint16_t>, grid, block, 0, s, n, A->ell16.width,
+                               nrhs, A->ell16.col.p, A->ell16.val.p, X, Y);
+    } else {
+        int rc = ensure_ell32(A, s);
+        if (rc) return rc;
+        if (n)
+            hipLaunchKernelGGL(ell_spmm_f32<float>, grid, block, 0, s, n, A->ell32.width, nrhs,
+                               A->ell32.col.p, A->ell32.val.p, X, Y);
+    }
+    PHG_HIP(hipGetLastError());
+    return PHG_OK;
+}
+
+int dot_f64(int n, const double* x, const double* y, double* result, hipStream_t s)
+{
+    const int G = std::max(1, std::min(256, (n + 255) / 256));
+    DevBuf<double> part;
+    PHG_HIP(part.alloc(G));
+    hipLaunchKernelGGL(dot_partial, dim3(G), dim3(256), 0, s, n, x, y, part.p);
+    PHG_HIP(hipGetLastError());
+    std::vector<double> h(G);
+    PHG_HIP(hipMemcpyAsync(h.data(), part.p, G * sizeof(double), hipMemcpyDeviceToHost, s));
+    PHG_HIP(hipStreamSynchronize(s));
+    double r = 0.0;
+    for (double v : h) r += v;
+    *result = r;
+    return PHG_OK;
+}
+
+int axpby_f64(int n, double k1, const double* a, double k2, const double* b, double* r,
+              hipStream_t s)
+{
+    if (n <= 0) return PHG_OK;
+    hipLaunchKernelGGL(axpby_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, k1, a, k2, b, r);
+    PHG_HIP(hipGetLastError());
+    return PHG_OK;
+}
+
+int u2p_f64(phg_mesh_s* m, const double* u, double* p, hipStream_t s)
+{
+    int rc = ensure_mesh_device(m);
+    if (rc) return rc;
+    const int rs = (int)m->h.inner.size();
+    if (rs)
+        hipLaunchKernelGGL(u2p_kernel, dim3((rs + 255) / 256), dim3(256), 0, s, rs,
+                           m->d_inner.p, u, p);
+    PHG_HIP(hipGetLastError());
+    return PHG_OK;
+}
+
+int p2u_f64(phg_mesh_s* m, const double* p, const double* bnd, double* u, hipStream_t s)
+{
+    int rc = ensure_mesh_device(m);
+    if (rc) return rc;
+    const int sz = m->h.n_points;
+    if (sz)
+        hipLaunchKernelGGL(p2u_kernel, dim3((sz + 255) / 256), dim3(256), 0, s, sz, m->d_isb.p,
+                           m->d_p2io.p, p, bnd, u);
+    PHG_HIP(hipGetLastError());
+    return PHG_OK;
+}
+
+}  // namespace phg

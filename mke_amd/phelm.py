@@ -1,0 +1,341 @@
+"""Python host mirror of the reference's solver/operator interface over the
+C ABI of include/phelm_gpu.h (libphelm_hip.so, built in-tree for gfx950).
+
+Names follow the reference: ``Mesh`` (src/mesh.h:354-427, Mesh::load),
+``Solver`` (the external Solver<T>: ctor(n), add, mult_vector, solve),
+``SphereLaplace`` (src/slaplace.h:60-108) and ``Helmholtz`` for the
+BarVortex/SphereChafe implicit-step operator (examples/barvortex_impl.h:57-80,
+examples/schafe.cpp:67-89).  Vectors are device tensors (torch is only used
+for device memory and streams); all arithmetic runs in the HIP library.
+There is no CPU fallback: if the library is missing, import fails.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libphelm_hip.so")
+
+F64, F32, BF16 = 0, 1, 2
+MASS, LAPLACE, HELMHOLTZ = 0, 1, 2
+MGS, CGS2 = 0, 1
+OK, ERR_NOCONV = 0, 1
+
+_I, _D, _L, _P = C.c_int, C.c_double, C.c_long, C.c_void_p
+_PI, _PD = C.POINTER(C.c_int), C.POINTER(C.c_double)
+
+
+class GmresOpts(C.Structure):
+    _fields_ = [("restart", _I), ("max_iters", _I), ("rtol", _D), ("ortho", _I),
+                ("precision", _I), ("inner_rtol", _D), ("use_x0", _I)]
+
+
+class GmresInfo(C.Structure):
+    _fields_ = [("iterations", _I), ("restarts", _I), ("refinements", _I),
+                ("rel_residual", _D), ("converged", _I)]
+
+
+# every symbol of include/phelm_gpu.h with (restype, argtypes)
+SIGNATURES = {
+    "phg_version": (C.c_char_p, []),
+    "phg_last_error": (C.c_char_p, []),
+    "phg_init": (_I, [_I]),
+    "phg_mesh_sphere_text": (_I, [_I, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]),
+    "phg_free": (None, [_P]),
+    "phg_mesh_load": (_I, [C.c_char_p, C.c_size_t, C.POINTER(_P)]),
+    "phg_mesh_sphere": (_I, [_I, C.POINTER(_P)]),
+    "phg_mesh_info": (_I, [_P, _PI, _PI, _PI, _PI, _PI]),
+    "phg_mesh_arrays": (_I, [_P, _PD, _PD, _PI, _PI, _PI, _PI, _PI, _PI, _PI, _PI]),
+    "phg_mesh_destroy": (None, [_P]),
+    "phg_mesh_csr_structure": (_I, [_P, _PI, _PI, C.POINTER(_L)]),
+    "phg_matrix_from_triplets": (_I, [_I, _I, _L, _PI, _PI, _PD, C.POINTER(_P)]),
+    "phg_matrix_from_csr": (_I, [_I, _I, _PI, _PI, _PD, C.POINTER(_P)]),
+    "phg_assemble": (_I, [_P, _I, _D, _D, _P, C.POINTER(_P)]),
+    "phg_assemble_boundary": (_I, [_P, _I, _D, _D, C.POINTER(_P)]),
+    "phg_matrix_info": (_I, [_P, _PI, _PI, C.POINTER(_L)]),
+    "phg_matrix_csr": (_I, [_P, _PI, _PI, _PD]),
+    "phg_matrix_destroy": (None, [_P]),
+    "phg_spmv": (_I, [_P, _P, _P, _P]),
+    "phg_spmv_f32": (_I, [_P, _I, _P, _P, _P]),
+    "phg_spmm_f32": (_I, [_P, _I, _I, _P, _P, _P]),
+    "phg_dot": (_I, [_I, _P, _P, _PD, _P]),
+    "phg_nrm2": (_I, [_I, _P, _PD, _P]),
+    "phg_axpby": (_I, [_I, _D, _P, _D, _P, _P, _P]),
+    "phg_u2p": (_I, [_P, _P, _P, _P]),
+    "phg_p2u": (_I, [_P, _P, _P, _P, _P]),
+    "phg_gmres_default_opts": (None, [C.POINTER(GmresOpts)]),
+    "phg_gmres": (_I, [_P, _P, _P, C.POINTER(GmresOpts), C.POINTER(GmresInfo), _P]),
+}
+
+_lib = None
+
+
+class PhelmError(RuntimeError):
+    pass
+
+
+def lib():
+    """Loads libphelm_hip.so; raises if it was not built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PhelmError("libphelm_hip.so not built: run __graft_entry__.build() "
+                             "or python -m mke_amd.build")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what="phelm call"):
+    if rc < 0:
+        raise PhelmError("%s failed (%d): %s" % (what, rc, lib().phg_last_error().decode()))
+    return rc
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (or an int address)."""
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    return getattr(stream, "cuda_stream", stream)
+
+
+def _pi(a):
+    return a.ctypes.data_as(_PI)
+
+
+def _pd(a):
+    return a.ctypes.data_as(_PD)
+
+
+def sphere_mesh_text(level):
+    """Text of ``mke_mesh_sphere --type full --coord local --iter level``."""
+    buf, n = C.c_void_p(), C.c_size_t()
+    check(lib().phg_mesh_sphere_text(level, C.byref(buf), C.byref(n)), "phg_mesh_sphere_text")
+    try:
+        return C.string_at(buf, n.value).decode()
+    finally:
+        lib().phg_free(buf)
+
+
+class Mesh:
+    """Mesh::load result (src/mesh.cpp:323-519)."""
+
+    def __init__(self, handle):
+        self.h = C.c_void_p(handle) if isinstance(handle, int) else handle
+        a = [C.c_int() for _ in range(5)]
+        check(lib().phg_mesh_info(self.h, *[C.byref(v) for v in a]), "phg_mesh_info")
+        (self.n_points, self.n_triangles, self.inner_size, self.outer_size,
+         self.n_zones) = [v.value for v in a]
+
+    @classmethod
+    def load(cls, text):
+        if isinstance(text, str):
+            text = text.encode()
+        h = C.c_void_p()
+        check(lib().phg_mesh_load(text, len(text), C.byref(h)), "phg_mesh_load")
+        return cls(h)
+
+    @classmethod
+    def sphere(cls, level):
+        h = C.c_void_p()
+        check(lib().phg_mesh_sphere(level, C.byref(h)), "phg_mesh_sphere")
+        return cls(h)
+
+    def arrays(self):
+        V, T, nz = self.n_points, self.n_triangles, self.n_zones
+        out = dict(pr=np.zeros((V, 3)), coords=np.zeros((V, nz, 2)),
+                   tri=np.zeros((T, 3), np.int32), tri_zone=np.zeros(T, np.int32),
+                   adj_ptr=np.zeros(V + 1, np.int32), adj=np.zeros(3 * T, np.int32),
+                   is_boundary=np.zeros(V, np.int32), inner=np.zeros(self.inner_size, np.int32),
+                   outer=np.zeros(max(self.outer_size, 1), np.int32), p2io=np.zeros(V, np.int32))
+        check(lib().phg_mesh_arrays(self.h, _pd(out["pr"]), _pd(out["coords"]), _pi(out["tri"]),
+                                    _pi(out["tri_zone"]), _pi(out["adj_ptr"]), _pi(out["adj"]),
+                                    _pi(out["is_boundary"]), _pi(out["inner"]),
+                                    _pi(out["outer"]), _pi(out["p2io"])), "phg_mesh_arrays")
+        out["outer"] = out["outer"][:self.outer_size]
+        return out
+
+    def csr_structure(self):
+        rp = np.zeros(self.inner_size + 1, np.int32)
+        nnz = C.c_long()
+        check(lib().phg_mesh_csr_structure(self.h, _pi(rp), None, C.byref(nnz)))
+        ci = np.zeros(max(nnz.value, 1), np.int32)
+        check(lib().phg_mesh_csr_structure(self.h, _pi(rp), _pi(ci), C.byref(nnz)))
+        return rp, ci[:nnz.value]
+
+    def u2p(self, u, p, stream=None):
+        check(lib().phg_u2p(self.h, _ptr(u), _ptr(p), _stream(stream)), "phg_u2p")
+
+    def p2u(self, p, bnd, u, stream=None):
+        check(lib().phg_p2u(self.h, _ptr(p), _ptr(bnd), _ptr(u), _stream(stream)), "phg_p2u")
+
+    def close(self):
+        if self.h:
+            lib().phg_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Matrix:
+    """A device CSR matrix (the storage behind Solver<T>)."""
+
+    def __init__(self, handle):
+        self.h = handle
+        r, c, nnz = C.c_int(), C.c_int(), C.c_long()
+        check(lib().phg_matrix_info(self.h, C.byref(r), C.byref(c), C.byref(nnz)))
+        self.n_rows, self.n_cols, self.nnz = r.value, c.value, nnz.value
+
+    @classmethod
+    def assemble(cls, mesh, kind, c_mass=1.0, c_lap=0.0, stream=None):
+        h = C.c_void_p()
+        check(lib().phg_assemble(mesh.h, kind, c_mass, c_lap, _stream(stream), C.byref(h)),
+              "phg_assemble")
+        return cls(h)
+
+    @classmethod
+    def assemble_boundary(cls, mesh, kind, c_mass=1.0, c_lap=0.0):
+        h = C.c_void_p()
+        check(lib().phg_assemble_boundary(mesh.h, kind, c_mass, c_lap, C.byref(h)),
+              "phg_assemble_boundary")
+        return cls(h)
+
+    @classmethod
+    def from_csr(cls, n_rows, n_cols, row_ptr, col_idx, vals):
+        rp = np.ascontiguousarray(row_ptr, np.int32)
+        ci = np.ascontiguousarray(col_idx, np.int32)
+        v = np.ascontiguousarray(vals, np.float64)
+        h = C.c_void_p()
+        check(lib().phg_matrix_from_csr(n_rows, n_cols, _pi(rp), _pi(ci) if len(ci) else None,
+                                        _pd(v) if len(v) else None, C.byref(h)),
+              "phg_matrix_from_csr")
+        return cls(h)
+
+    @classmethod
+    def from_triplets(cls, n_rows, n_cols, rows, cols, vals):
+        r = np.ascontiguousarray(rows, np.int32)
+        c = np.ascontiguousarray(cols, np.int32)
+        v = np.ascontiguousarray(vals, np.float64)
+        h = C.c_void_p()
+        check(lib().phg_matrix_from_triplets(n_rows, n_cols, len(r), _pi(r), _pi(c), _pd(v),
+                                             C.byref(h)), "phg_matrix_from_triplets")
+        return cls(h)
+
+    def csr(self):
+        rp = np.zeros(self.n_rows + 1, np.int32)
+        ci = np.zeros(max(self.nnz, 1), np.int32)
+        v = np.zeros(max(self.nnz, 1))
+        check(lib().phg_matrix_csr(self.h, _pi(rp), _pi(ci), _pd(v)), "phg_matrix_csr")
+        return rp, ci[:self.nnz], v[:self.nnz]
+
+    def mult_vector(self, y, x, stream=None):
+        """y = A x (Solver::mult_vector), FP64 device tensors."""
+        check(lib().phg_spmv(self.h, _ptr(x), _ptr(y), _stream(stream)), "phg_spmv")
+
+    def mult_vector_f32(self, y, x, prec=F32, stream=None):
+        check(lib().phg_spmv_f32(self.h, prec, _ptr(x), _ptr(y), _stream(stream)), "phg_spmv_f32")
+
+    def mult_matrix_f32(self, Y, X, nrhs, prec=F32, stream=None):
+        check(lib().phg_spmm_f32(self.h, prec, nrhs, _ptr(X), _ptr(Y), _stream(stream)),
+              "phg_spmm_f32")
+
+    def solve(self, x, b, restart=30, rtol=1e-10, max_iters=200000, ortho=MGS, precision=F64,
+              inner_rtol=1e-5, use_x0=False, stream=None):
+        """Solves A x = b (Solver::solve) with GMRES(restart); returns the info dict."""
+        o = GmresOpts()
+        lib().phg_gmres_default_opts(C.byref(o))
+        o.restart, o.max_iters, o.rtol = restart, max_iters, rtol
+        o.ortho, o.precision, o.inner_rtol, o.use_x0 = ortho, precision, inner_rtol, int(use_x0)
+        info = GmresInfo()
+        check(lib().phg_gmres(self.h, _ptr(b), _ptr(x), C.byref(o), C.byref(info),
+                              _stream(stream)), "phg_gmres")
+        return dict(iterations=info.iterations, restarts=info.restarts,
+                    refinements=info.refinements, rel_residual=info.rel_residual,
+                    converged=bool(info.converged))
+
+    def close(self):
+        if self.h:
+            lib().phg_matrix_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Solver:
+    """Solver<T>(n) of the reference's solver library: add(i, j, a) stages
+    triplets on the host (duplicates accumulate in call order); the first
+    mult_vector/solve builds the device matrix."""
+
+    def __init__(self, n_rows, n_cols=None):
+        self.n_rows = n_rows
+        self.n_cols = n_rows if n_cols is None else n_cols
+        self._r, self._c, self._v = [], [], []
+        self._m = None
+
+    def add(self, i, j, a):
+        self._r.append(i)
+        self._c.append(j)
+        self._v.append(a)
+        self._m = None
+
+    def matrix(self):
+        if self._m is None:
+            self._m = Matrix.from_triplets(self.n_rows, self.n_cols, self._r, self._c, self._v)
+        return self._m
+
+    def mult_vector(self, y, x, stream=None):
+        self.matrix().mult_vector(y, x, stream)
+
+    def solve(self, x, b, **kw):
+        return self.matrix().solve(x, b, **kw)
+
+
+def helmholtz_coeffs(tau=1.0, sigma=0.0, theta=1.0, mu=1e-2):
+    """c_mass, c_lap of bv_private::integrate_cb (examples/barvortex_impl.h:57-80):
+    (1/tau + sigma*theta) * M - theta*mu * L."""
+    return 1.0 / tau + sigma * theta, -theta * mu
+
+
+class SphereLaplace:
+    """SphereLaplace<T, Matrix> (src/slaplace.h:60-108): idt_ (mass),
+    laplace_ (stiffness) and the boundary matrices, assembled on the GPU."""
+
+    def __init__(self, mesh, stream=None):
+        self.m = mesh
+        self.idt_ = Matrix.assemble(mesh, MASS, stream=stream)
+        self.laplace_ = Matrix.assemble(mesh, LAPLACE, stream=stream)
+        self.bnd1_ = Matrix.assemble_boundary(mesh, MASS)       # laplace_bnd1_cb
+        self.bnd3_ = Matrix.assemble_boundary(mesh, LAPLACE)    # slaplace_integrate_cb
+
+
+class Helmholtz:
+    """Implicit-step operator A = c_mass*M + c_lap*L of BarVortex
+    (examples/barvortex_impl.h:57-80, :149-153) and SphereChafe
+    (examples/schafe.cpp:67-89), assembled per triangle on the GPU."""
+
+    def __init__(self, mesh, tau=1.0, sigma=0.0, theta=1.0, mu=1e-2, stream=None):
+        self.m = mesh
+        self.c_mass, self.c_lap = helmholtz_coeffs(tau, sigma, theta, mu)
+        self.A_ = Matrix.assemble(mesh, HELMHOLTZ, self.c_mass, self.c_lap, stream=stream)
+        self.bnd_ = Matrix.assemble_boundary(mesh, HELMHOLTZ, self.c_mass, self.c_lap)

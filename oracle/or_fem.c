@@ -1,0 +1,639 @@
+/*
+ * CPU oracle: quadrature, trapezoid integrals, Polynom arithmetic, P1 basis
+ * and row-wise FE assembly.  TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Every routine keeps the reference's floating-point operation order; build
+ * with -ffp-contract=off so that no multiply-add is fused behind its back.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+
+/* ------------------------------------------------------------------------ */
+/* cos / sin: Cody-Waite reduction + Taylor polynomials (see oracle.h)       */
+/* ------------------------------------------------------------------------ */
+
+/* pi/2 = PIO2_A + PIO2_B + PIO2_C; A and B carry 33 significant bits so that
+ * k * A and k * B are exact for |k| < 2^20. */
+static const double PIO2_A = 0x1.921fb54400000p+0;
+static const double PIO2_B = 0x1.0b4611a600000p-34;
+static const double PIO2_C = 0x1.3198a2e037073p-69;
+static const double TWO_OVER_PI = 0x1.45f306dc9c883p-1;
+/* (-1)^k / (2k)!  and  (-1)^k / (2k+1)!,  k = 1..8, rounded to double */
+static const double COS_C[8] = {
+    -0x1.0000000000000p-1,  0x1.5555555555555p-5, -0x1.6c16c16c16c17p-10,
+     0x1.a01a01a01a01ap-16, -0x1.27e4fb7789f5cp-22, 0x1.1eed8eff8d898p-29,
+    -0x1.93974a8c07c9dp-37,  0x1.ae7f3e733b81fp-45};
+static const double SIN_C[8] = {
+    -0x1.5555555555555p-3,  0x1.1111111111111p-7, -0x1.a01a01a01a01ap-13,
+     0x1.71de3a556c734p-19, -0x1.ae64567f544e4p-26, 0x1.6124613a86d09p-33,
+    -0x1.ae7f3e733b81fp-41,  0x1.952c77030ad4ap-49};
+
+static int g_libm_cos = 0;
+void or_use_libm_cos(int on) { g_libm_cos = on; }
+
+static double poly_cos(double r)
+{
+    double z = r * r;
+    double p = COS_C[7];
+    for (int i = 6; i >= 0; --i) p = fma(p, z, COS_C[i]);
+    return fma(z, p, 1.0);
+}
+
+static double poly_sin(double r)
+{
+    double z = r * r;
+    double p = SIN_C[7];
+    for (int i = 6; i >= 0; --i) p = fma(p, z, SIN_C[i]);
+    return fma(r * z, p, r);
+}
+
+static double reduce_pio2(double x, int *q)
+{
+    double k = rint(x * TWO_OVER_PI);
+    double r = fma(-k, PIO2_A, x);
+    r = fma(-k, PIO2_B, r);
+    r = fma(-k, PIO2_C, r);
+    *q = ((int)(long long)k) & 3;
+    return r;
+}
+
+double or_cos(double x)
+{
+    if (g_libm_cos) return cos(x);
+    int q;
+    double r = reduce_pio2(x, &q);
+    switch (q) {
+    case 0: return poly_cos(r);
+    case 1: return -poly_sin(r);
+    case 2: return -poly_cos(r);
+    default: return poly_sin(r);
+    }
+}
+
+double or_sin(double x)
+{
+    if (g_libm_cos) return sin(x);
+    int q;
+    double r = reduce_pio2(x, &q);
+    switch (q) {
+    case 0: return poly_sin(r);
+    case 1: return poly_cos(r);
+    case 2: return -poly_sin(r);
+    default: return -poly_cos(r);
+    }
+}
+
+double or_ipow(double x, int p)
+{
+    double r = 1.0;
+    for (int i = 0; i < p; ++i) r *= x;
+    return r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Quadrature: src/quadrature.c:49-171.  The node/weight tables are data    */
+/* that must match the reference digit for digit (15-digit truncations).    */
+/* ------------------------------------------------------------------------ */
+static const double K15N[15][2] = {
+    {+0.949107912342759, 0.063092092629979},
+    {-0.949107912342759, 0.063092092629979},
+    {+0.741531185599394, 0.140653259715525},
+    {-0.741531185599394, 0.140653259715525},
+    {+0.405845151377397, 0.190350578064785},
+    {-0.405845151377397, 0.190350578064785},
+    { 0.000000000000000, 0.209482141084728},
+    {+0.991455371120813, 0.022935322010529},
+    {-0.991455371120813, 0.022935322010529},
+    {+0.864864423359769, 0.104790010322250},
+    {-0.864864423359769, 0.104790010322250},
+    {+0.586087235467691, 0.169004726639267},
+    {-0.586087235467691, 0.169004726639267},
+    {+0.207784955007898, 0.204432940075298},
+    {-0.207784955007898, 0.204432940075298},
+};
+
+/* quadrature.c:100-128: nodes 0..6 then 7..14, k15 *= p1 at the end */
+double or_gauss_kronrod15(double a, double b, or_fx_t f, void *d)
+{
+    double k15 = 0.0;
+    double p1 = 0.5 * (b - a);
+    double p2 = 0.5 * (b + a);
+    for (int i = 0; i < 15; ++i) k15 += K15N[i][1] * f(p1 * K15N[i][0] + p2, d);
+    return k15 * p1;
+}
+
+#define CW_A1 0.7974269853530880
+#define CW_B1 0.1012865073234560
+#define CW_A2 0.0597158717897742
+#define CW_B2 0.4701420641051129
+static const double W7[7] = {0.225, 0.1259391805448265, 0.1259391805448265,
+                             0.1259391805448265, 0.1323941527885112,
+                             0.1323941527885112, 0.1323941527885112};
+static const double C7[7][3] = {
+    {1. / 3., 1. / 3., 1. / 3.}, {CW_A1, CW_B1, CW_B1}, {CW_B1, CW_A1, CW_B1},
+    {CW_B1, CW_B1, CW_A1},       {CW_A2, CW_B2, CW_B2}, {CW_B2, CW_A2, CW_B2},
+    {CW_B2, CW_B2, CW_A2}};
+
+/* quadrature.c:151-171 */
+double or_cubature7(double x1, double y1, double x2, double y2, double x3,
+                    double y3, or_fxy_t f, void *d)
+{
+    double s = 0.0;
+    for (int i = 0; i < 7; ++i) {
+        double a = C7[i][0], b = C7[i][1], c = C7[i][2];
+        double u = a * x1 + b * x2 + c * x3;
+        double v = a * y1 + b * y2 + c * y3;
+        s += W7[i] * f(u, v, d);
+    }
+    s *= 0.5 * fabs(((x1 - x3) * (y2 - y3) - (y1 - y3) * (x2 - x3)));
+    return s;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Trapezoid integrals: src/util.cpp:97-251                                 */
+/* ------------------------------------------------------------------------ */
+static double binom(int n, int k)  /* cnk_6.h rows, k <= n: exact integers */
+{
+    double r = 1.0;
+    for (int i = 1; i <= k; ++i) r = r * (double)(n - k + i) / (double)i;
+    return r;
+}
+
+/* util.cpp:106-160 */
+static double ssum(double x, int k, int n, double k1, double b1, double k2,
+                   double b2)
+{
+    double sum1 = 0, sum2 = 0;
+    if (k2 != 0 && b2 != 0) {
+        for (int i = 0; i <= n + 1; i++)
+            sum1 += binom(n + 1, i) * or_ipow(k2, n + 1 - i) *
+                    or_ipow(x, n + k + 2 - i) * or_ipow(b2, i) /
+                    (double)(n + k + 2 - i);
+    } else if (k2 != 0 && b2 == 0) {
+        sum1 = or_ipow(k2, n + 1) * or_ipow(x, k + n + 2) / (double)(k + n + 2);
+    } else if (k2 == 0 && b2 != 0) {
+        sum1 = or_ipow(b2, n + 1) * or_ipow(x, k + 1) / (double)(k + 1);
+    } else {
+        sum1 = 0;
+    }
+    if (k1 != 0 && b1 != 0) {
+        for (int i = 0; i <= n + 1; i++)
+            sum2 += binom(n + 1, i) * or_ipow(k1, n + 1 - i) *
+                    or_ipow(x, n + k + 2 - i) * or_ipow(b1, i) /
+                    (double)(n + k + 2 - i);
+    } else if (k1 != 0 && b1 == 0) {
+        sum2 = or_ipow(k1, n + 1) * or_ipow(x, k + n + 2) / (double)(k + n + 2);
+    } else if (k1 == 0 && b1 != 0) {
+        sum2 = or_ipow(b1, n + 1) * or_ipow(x, k + 1) / (double)(k + 1);
+    } else {
+        sum2 = 0;
+    }
+    return sum1 - sum2;
+}
+
+/* util.cpp:166-175 */
+double or_trapezoid_integral(int k, int n, double k1, double b1, double k2,
+                             double b2, double x1, double x3)
+{
+    return (ssum(x3, k, n, k1, b1, k2, b2) - ssum(x1, k, n, k1, b1, k2, b2)) /
+           (double)(n + 1);
+}
+
+typedef struct { int k, n; double k1, b1, k2, b2; } fcos_data;
+
+/* util.cpp:194-215: x^k [(k2x+b2)^{n+1} - (k1x+b1)^{n+1}]/(n+1) w(x) */
+static double pt1_of(double x, const fcos_data *d)
+{
+    return (or_ipow((d->k2 * x + d->b2), d->n + 1) -
+            or_ipow((d->k1 * x + d->b1), d->n + 1)) / (double)(d->n + 1);
+}
+static double f_cos(double x, void *v)
+{
+    const fcos_data *d = (const fcos_data *)v;
+    double pt1 = pt1_of(x, d);
+    return or_ipow(x, d->k) * pt1 * or_cos(x);
+}
+static double f_sin(double x, void *v)
+{
+    const fcos_data *d = (const fcos_data *)v;
+    double pt1 = pt1_of(x, d);
+    return or_ipow(x, d->k) * pt1 * or_sin(x);
+}
+static double f_1_cos(double x, void *v)
+{
+    const fcos_data *d = (const fcos_data *)v;
+    double pt1 = pt1_of(x, d);
+    return or_ipow(x, d->k) * pt1 / or_cos(x);
+}
+
+double or_trapezoid_integral_cos(int k, int n, double k1, double b1, double k2,
+                                 double b2, double x1, double x3)
+{
+    fcos_data d = {k, n, k1, b1, k2, b2};
+    return or_gauss_kronrod15(x1, x3, f_cos, &d);
+}
+double or_trapezoid_integral_sin(int k, int n, double k1, double b1, double k2,
+                                 double b2, double x1, double x3)
+{
+    fcos_data d = {k, n, k1, b1, k2, b2};
+    return or_gauss_kronrod15(x1, x3, f_sin, &d);
+}
+double or_trapezoid_integral_1_cos(int k, int n, double k1, double b1,
+                                   double k2, double b2, double x1, double x3)
+{
+    fcos_data d = {k, n, k1, b1, k2, b2};
+    return or_gauss_kronrod15(x1, x3, f_1_cos, &d);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Polynom: src/polynom.h:81-300, src/polynom.cpp:63-184,667-729             */
+/* ------------------------------------------------------------------------ */
+static void poly_init(or_poly *p, int xd, int yd)
+{
+    p->x_deg = xd;
+    p->y_deg = yd;
+    memset(p->koef, 0, sizeof(p->koef));
+}
+static double poly_k(const or_poly *p, int i, int j)  /* polynom.h:127-131 */
+{
+    if (i > p->x_deg || j > p->y_deg) return 0;
+    return p->koef[i * (p->y_deg + 1) + j];
+}
+
+double or_poly_apply(const or_poly *a, double x, double y) /* polynom.cpp:63-89 */
+{
+    double xi[8], yj[8], r = 0.0;
+    for (int i = 0; i <= a->x_deg; i++) xi[i] = or_ipow(x, i);
+    for (int j = 0; j <= a->y_deg; j++) yj[j] = or_ipow(y, j);
+    for (int i = 0; i <= a->x_deg; i++)
+        for (int j = 0; j <= a->y_deg; j++)
+            r += a->koef[i * (a->y_deg + 1) + j] * xi[i] * yj[j];
+    return r;
+}
+
+void or_poly_diff(const or_poly *p, int d, or_poly *r) /* polynom.cpp:147-184 */
+{
+    int xd = p->x_deg, yd = p->y_deg, deg = yd;
+    if (d == 0) {
+        xd -= 1;
+        poly_init(r, xd, yd);
+        for (int j = 0; j <= yd; ++j)
+            for (int i = 0; i <= xd; ++i)
+                r->koef[i * (yd + 1) + j] =
+                    (double)(i + 1) * p->koef[(i + 1) * (deg + 1) + j];
+    } else {
+        yd -= 1;
+        poly_init(r, xd, yd);
+        for (int i = 0; i <= xd; ++i)
+            for (int j = 0; j <= yd; ++j)
+                r->koef[i * (yd + 1) + j] =
+                    (double)(j + 1) * p->koef[i * (deg + 1) + j + 1];
+    }
+}
+
+void or_poly_mul(const or_poly *p1, const or_poly *p2, or_poly *p) /* :667-696 */
+{
+    poly_init(p, p1->x_deg + p2->x_deg, p1->y_deg + p2->y_deg);
+    int deg = p->y_deg;
+    for (int i = 0; i <= p1->x_deg; ++i)
+        for (int j = 0; j <= p1->y_deg; ++j)
+            for (int k = 0; k <= p2->x_deg; ++k)
+                for (int m = 0; m <= p2->y_deg; ++m)
+                    p->koef[(i + k) * (deg + 1) + (j + m)] +=
+                        p1->koef[i * (p1->y_deg + 1) + j] *
+                        p2->koef[k * (p2->y_deg + 1) + m];
+}
+
+static void poly_sub(const or_poly *p1, const or_poly *p2, or_poly *r) /* :698-713 */
+{
+    int xd = p1->x_deg > p2->x_deg ? p1->x_deg : p2->x_deg;
+    int yd = p1->y_deg > p2->y_deg ? p1->y_deg : p2->y_deg;
+    poly_init(r, xd, yd);
+    for (int i = 0; i <= xd; ++i)
+        for (int j = 0; j <= yd; ++j)
+            r->koef[i * (yd + 1) + j] = poly_k(p1, i, j) - poly_k(p2, i, j);
+}
+
+/* mesh.cpp:578-580: L(p1, p2) = (X - p1.x)(p2.y - p1.y) - (Y - p1.y)(p2.x - p1.x) */
+static void basis_line(double p1x, double p1y, double p2x, double p2y,
+                       or_poly *out)
+{
+    or_poly a, b;
+    poly_init(&a, 1, 0);               /* P2X */
+    a.koef[1] = 1.0;
+    a.koef[0] -= p1x;                  /* polynom.h:266-271 */
+    double dy = p2y - p1y;
+    a.koef[0] *= dy;                   /* polynom.h:281-290 */
+    a.koef[1] *= dy;
+    poly_init(&b, 0, 1);               /* P2Y */
+    b.koef[1] = 1.0;
+    b.koef[0] -= p1y;
+    double dx = p2x - p1x;
+    b.koef[0] *= dx;
+    b.koef[1] *= dx;
+    poly_sub(&a, &b, out);
+}
+
+/* mesh.cpp:582-607 (prepare_basis1) */
+void or_basis_p1(const double xs[3], const double ys[3], or_poly out[3])
+{
+    basis_line(xs[1], ys[1], xs[2], ys[2], &out[0]);
+    basis_line(xs[0], ys[0], xs[2], ys[2], &out[1]);
+    basis_line(xs[0], ys[0], xs[1], ys[1], &out[2]);
+    for (int i = 0; i < 3; ++i) {
+        double k = or_poly_apply(&out[i], xs[i], ys[i]);
+        double k1 = 1.0 / k;           /* polynom.h:136-144 */
+        int sz = (out[i].x_deg + 1) * (out[i].y_deg + 1);
+        for (int c = 0; c < sz; ++c) out[i].koef[c] *= k1;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* integrate1: src/polynom.cpp:188-315                                        */
+/* ------------------------------------------------------------------------ */
+typedef double (*t_int)(int, int, double, double, double, double, double,
+                        double);
+
+static double integrate1(const or_poly *p, const double xs[3],
+                         const double ys[3], t_int trapezoid)
+{
+    double k1, b1, k2, b2, k3, b3, t;
+    double x1 = xs[0], x2 = xs[1], x3 = xs[2];
+    double y1 = ys[0], y2 = ys[1], y3 = ys[2];
+    double int1 = 0.0, int2 = 0.0;
+
+    /* polynom.cpp:205-266: afterwards x1 <= x3 <= x2 */
+    if (x1 <= x2 && x2 <= x3) {
+        t = x2; x2 = x3; x3 = t;
+        t = y2; y2 = y3; y3 = t;
+    } else if (x2 <= x1 && x1 <= x3) {
+        t = x1; x1 = x2; x2 = t;
+        t = x2; x2 = x3; x3 = t;
+        t = y1; y1 = y2; y2 = t;
+        t = y2; y2 = y3; y3 = t;
+    } else if (x2 <= x3 && x3 <= x1) {
+        t = x1; x1 = x2; x2 = t;
+        t = y1; y1 = y2; y2 = t;
+    } else if (x3 <= x1 && x1 <= x2) {
+        t = x1; x1 = x3; x3 = t;
+        t = y1; y1 = y3; y3 = t;
+    } else if (x3 <= x2 && x2 <= x1) {
+        t = x1; x1 = x3; x3 = t;
+        t = x2; x2 = x3; x3 = t;
+        t = y1; y1 = y3; y3 = t;
+        t = y2; y2 = y3; y3 = t;
+    }
+
+    k1 = (y2 - y1) / (x2 - x1);
+    b1 = (y1 * x2 - x1 * y2) / (x2 - x1);
+
+    if (fabs(x1 - x3) > 1e-15) {
+        k3 = (y3 - y1) / (x3 - x1);
+        b3 = (y1 * x3 - x1 * y3) / (x3 - x1);
+        for (int k = 0; k <= p->x_deg; ++k)
+            for (int n = 0; n <= p->y_deg; ++n)
+                int1 += p->koef[k * (p->y_deg + 1) + n] *
+                        trapezoid(k, n, k1, b1, k3, b3, x1, x3);
+    } else {
+        int1 = 0;
+    }
+
+    if (fabs(x3 - x2) > 1e-15) {
+        k2 = (y2 - y3) / (x2 - x3);
+        b2 = (y3 * x2 - x3 * y2) / (x2 - x3);
+        for (int k = 0; k <= p->x_deg; ++k)
+            for (int n = 0; n <= p->y_deg; ++n)
+                int2 += p->koef[k * (p->y_deg + 1) + n] *
+                        trapezoid(k, n, k1, b1, k2, b2, x3, x2);
+    } else {
+        int2 = 0;
+    }
+
+    if (y3 >= y1 && y3 >= y2) return int1 + int2;
+    else if (y3 <= y1 && y3 <= y2) return -int1 - int2;
+    else if (y3 >= k1 * x3 + b1) return int1 + int2;
+    else return -int1 - int2;
+}
+
+double or_integrate(const or_poly *p, const double xs[3], const double ys[3],
+                    int weight)
+{
+    switch (weight) {
+    case 0: return integrate1(p, xs, ys, or_trapezoid_integral);
+    case 1: return integrate1(p, xs, ys, or_trapezoid_integral_cos);
+    case 2: return integrate1(p, xs, ys, or_trapezoid_integral_sin);
+    default: return integrate1(p, xs, ys, or_trapezoid_integral_1_cos);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Element callbacks                                                         */
+/* ------------------------------------------------------------------------ */
+/* slaplace.cpp:75-105: slaplace(a, b) = laplace1(a, b) + laplace2(a, b) */
+static double slaplace(const or_poly *a, const or_poly *b, const double xs[3],
+                       const double ys[3])
+{
+    or_poly da, db, poly1;
+    or_poly_diff(a, 0, &da);
+    or_poly_diff(b, 0, &db);
+    or_poly_mul(&da, &db, &poly1);
+    double l1 = -or_integrate(&poly1, xs, ys, 1);
+    or_poly_diff(a, 1, &da);
+    or_poly_diff(b, 1, &db);
+    or_poly_mul(&da, &db, &poly1);
+    double l2 = -or_integrate(&poly1, xs, ys, 3);
+    return l1 + l2;
+}
+
+static double jac_cos_func(double x, double y, void *d)  /* sjacobian.cpp */
+{
+    return or_cos(x) * or_poly_apply((const or_poly *)d, x, y);
+}
+
+double or_element(int kind, const or_poly *phi_i, const or_poly *phi_j,
+                  const double xs[3], const double ys[3], double c_mass,
+                  double c_lap)
+{
+    or_poly poly;
+    switch (kind) {
+    case 0:   /* id_cb: slaplace.cpp:185-197 */
+        or_poly_mul(phi_i, phi_j, &poly);
+        return or_integrate(&poly, xs, ys, 1);
+    case 1:   /* slaplace_integrate_cb: slaplace.cpp:171-183 */
+        return slaplace(phi_j, phi_i, xs, ys);
+    case 2: { /* bv_private::integrate_cb: barvortex_impl.h:57-80 */
+        or_poly_mul(phi_i, phi_j, &poly);
+        double pt1 = or_integrate(&poly, xs, ys, 1);
+        pt1 *= c_mass;
+        double pt2 = slaplace(phi_j, phi_i, xs, ys);
+        pt2 *= c_lap;
+        return pt1 + pt2;
+    }
+    default:  /* SphereJacobian id_cb: sjacobian.cpp:60-74 */
+        or_poly_mul(phi_i, phi_j, &poly);
+        return or_cubature7(xs[0], ys[0], xs[1], ys[1], xs[2], ys[2],
+                            jac_cos_func, &poly);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Assembly: src/generators.cpp:57-212                                       */
+/* ------------------------------------------------------------------------ */
+static void tri_coords(const or_mesh *m, int t, int zone, double xs[3],
+                       double ys[3])
+{
+    for (int a = 0; a < 3; ++a) {
+        int p = m->tri[3 * t + a];
+        xs[a] = m->coords[((long)p * m->n_zones + zone) * 2 + 0];
+        ys[a] = m->coords[((long)p * m->n_zones + zone) * 2 + 1];
+    }
+}
+
+static int local_index(const or_mesh *m, int t, int p)  /* mesh.cpp:550-561 */
+{
+    for (int a = 0; a < 3; ++a)
+        if (m->tri[3 * t + a] == p) return a;
+    abort();
+}
+
+static int cmp_int(const void *a, const void *b)
+{
+    int x = *(const int *)a, y = *(const int *)b;
+    return (x > y) - (x < y);
+}
+
+/* columns of row i: sorted unique p2io of inner vertices around inner[i] */
+static int row_columns(const or_mesh *m, int i, int *cols)
+{
+    int p = m->inner[i], cnt = 0, tmp[64];
+    for (int e = m->adj_ptr[p]; e < m->adj_ptr[p + 1]; ++e) {
+        int t = m->adj[e];
+        for (int a = 0; a < 3; ++a) {
+            int q = m->tri[3 * t + a];
+            if (!m->is_boundary[q]) tmp[cnt++] = m->p2io[q];
+        }
+    }
+    qsort(tmp, cnt, sizeof(int), cmp_int);
+    int u = 0;
+    for (int c = 0; c < cnt; ++c)
+        if (u == 0 || tmp[c] != tmp[u - 1]) tmp[u++] = tmp[c];
+    if (cols) memcpy(cols, tmp, u * sizeof(int));
+    return u;
+}
+
+long or_csr_structure(const or_mesh *m, int *row_ptr, int *col_idx)
+{
+    long nnz = 0;
+    int cols[64];
+    for (int i = 0; i < m->n_inner; ++i) {
+        row_ptr[i] = (int)nnz;
+        int c = row_columns(m, i, cols);
+        if (col_idx) memcpy(col_idx + nnz, cols, c * sizeof(int));
+        nnz += c;
+    }
+    row_ptr[m->n_inner] = (int)nnz;
+    return nnz;
+}
+
+static long find_pos(const int *row_ptr, const int *col_idx, int i, int j)
+{
+    for (int e = row_ptr[i]; e < row_ptr[i + 1]; ++e)
+        if (col_idx[e] == j) return e;
+    abort();
+}
+
+int or_assemble(const or_mesh *m, int kind, double c_mass, double c_lap,
+                const int *row_ptr, const int *col_idx, double *vals)
+{
+    int rs = m->n_inner;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < rs; ++i) {
+        for (int e = row_ptr[i]; e < row_ptr[i + 1]; ++e) vals[e] = 0.0;
+        int p = m->inner[i];
+        int zone = m->tri_zone[m->adj[m->adj_ptr[p]]];   /* generators.cpp:76 */
+        for (int e = m->adj_ptr[p]; e < m->adj_ptr[p + 1]; ++e) {
+            int t = m->adj[e];
+            double xs[3], ys[3];
+            or_poly phik[3];
+            tri_coords(m, t, zone, xs, ys);
+            or_basis_p1(xs, ys, phik);
+            const or_poly *phi_i = &phik[local_index(m, t, p)];
+            for (int i0 = 0; i0 < 3; ++i0) {
+                int p2 = m->tri[3 * t + i0];
+                if (m->is_boundary[p2]) continue;
+                int j = m->p2io[p2];
+                double a = or_element(kind, phi_i, &phik[i0], xs, ys, c_mass,
+                                      c_lap);
+                vals[find_pos(row_ptr, col_idx, i, j)] += a;
+            }
+        }
+    }
+    return 0;
+}
+
+/* boundary matrix rows = inner points touching an outer point */
+long or_boundary_structure(const or_mesh *m, int *row_ptr, int *col_idx)
+{
+    int rs = m->n_inner;
+    int *cnt = (int *)calloc(rs + 1, sizeof(int));
+    int **cols = (int **)calloc(rs, sizeof(int *));
+    for (int j = 0; j < m->n_outer; ++j) {
+        int p2 = m->outer[j];
+        for (int e = m->adj_ptr[p2]; e < m->adj_ptr[p2 + 1]; ++e) {
+            int t = m->adj[e];
+            for (int a = 0; a < 3; ++a) {
+                int p = m->tri[3 * t + a];
+                if (m->is_boundary[p]) continue;
+                int i = m->p2io[p];
+                int dup = 0;
+                for (int c = 0; c < cnt[i]; ++c) dup |= cols[i][c] == j;
+                if (dup) continue;
+                cols[i] = (int *)realloc(cols[i], (cnt[i] + 1) * sizeof(int));
+                cols[i][cnt[i]++] = j;
+            }
+        }
+    }
+    long nnz = 0;
+    for (int i = 0; i < rs; ++i) {
+        row_ptr[i] = (int)nnz;
+        if (cnt[i]) qsort(cols[i], cnt[i], sizeof(int), cmp_int);
+        if (col_idx && cnt[i]) memcpy(col_idx + nnz, cols[i], cnt[i] * sizeof(int));
+        nnz += cnt[i];
+        free(cols[i]);
+    }
+    row_ptr[rs] = (int)nnz;
+    free(cols);
+    free(cnt);
+    return nnz;
+}
+
+/* generators.cpp:141-186: zone of the outer point, phi_i = inner basis */
+int or_assemble_boundary(const or_mesh *m, int kind, double c_mass,
+                         double c_lap, const int *row_ptr, const int *col_idx,
+                         double *vals)
+{
+    for (int e = 0; e < row_ptr[m->n_inner]; ++e) vals[e] = 0.0;
+    for (int j = 0; j < m->n_outer; ++j) {
+        int p2 = m->outer[j];
+        int zone = m->tri_zone[m->adj[m->adj_ptr[p2]]];
+        for (int e = m->adj_ptr[p2]; e < m->adj_ptr[p2 + 1]; ++e) {
+            int t = m->adj[e];
+            double xs[3], ys[3];
+            or_poly phik[3];
+            tri_coords(m, t, zone, xs, ys);
+            or_basis_p1(xs, ys, phik);
+            const or_poly *phi_j = &phik[local_index(m, t, p2)];
+            for (int i0 = 0; i0 < 3; ++i0) {
+                int p = m->tri[3 * t + i0];
+                if (m->is_boundary[p]) continue;
+                int i = m->p2io[p];
+                double a = or_element(kind, &phik[i0], phi_j, xs, ys, c_mass,
+                                      c_lap);
+                vals[find_pos(row_ptr, col_idx, i, j)] += a;
+            }
+        }
+    }
+    return 0;
+}
