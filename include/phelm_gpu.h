@@ -145,6 +145,7 @@ typedef struct {
                         FP64 iterative refinement                          */
     double inner_rtol; /* relative reduction per refinement step (mixed)   */
     int use_x0;      /* 0: start from x = 0; 1: start from the given x      */
+    int graph;       /* 1: replay each restart cycle as one captured hipGraph */
 } phg_gmres_opts;
 
 typedef struct {

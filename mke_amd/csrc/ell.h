@@ -1,0 +1,93 @@
+// Row dot products over the slot-major ELL layout (entry k of row i at
+// k * n + i), shared by the SpMV, residual and Arnoldi kernels.
+//
+// For rows of at most ELL_MAXW entries (the sphere operators have 6-7) the
+// column/value loads of all slots are issued first, then all x gathers, then
+// the products are added in slot order; a row's memory latency is paid twice
+// instead of 2 * width times.  Sums start from 0 and add slots in order, so
+// FP64 results equal a plain CSR loop over the same row.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace phg {
+
+constexpr int ELL_MAXW = 8;
+
+template <typename MT>
+__device__ __forceinline__ float ell_val_f32(MT v);
+template <>
+__device__ __forceinline__ float ell_val_f32<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ float ell_val_f32<uint16_t>(uint16_t v)
+{
+    return __uint_as_float(((uint32_t)v) << 16);
+}
+
+// s = sum_k val[k] * (x[col[k]] * scale), FP64 (no fused multiply-add; the
+// including file is compiled with -ffp-contract=off).  scale = 1 multiplies
+// exactly.
+__device__ __forceinline__ double ell_row_f64(int n, int width, int row,
+                                              const int* __restrict__ col,
+                                              const double* __restrict__ val,
+                                              const double* __restrict__ x, double scale,
+                                              bool scaled)
+{
+    double s = 0.0;
+    if (width <= ELL_MAXW) {
+        int c[ELL_MAXW];
+        double v[ELL_MAXW], xv[ELL_MAXW];
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k) {
+            const size_t o = (size_t)k * n + row;
+            c[k] = k < width ? col[o] : 0;
+            v[k] = k < width ? val[o] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k) xv[k] = k < width ? x[c[k]] : 0.0;
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k)
+            if (k < width) s += v[k] * (scaled ? xv[k] * scale : xv[k]);
+    } else {
+        for (int k = 0; k < width; ++k) {
+            const size_t o = (size_t)k * n + row;
+            s += val[o] * (scaled ? x[col[o]] * scale : x[col[o]]);
+        }
+    }
+    return s;
+}
+
+// FP32 vectors, values stored as float or bfloat16, fused multiply-add.
+template <typename MT>
+__device__ __forceinline__ float ell_row_f32(int n, int width, int row,
+                                             const int* __restrict__ col,
+                                             const MT* __restrict__ val,
+                                             const float* __restrict__ x, float scale)
+{
+    float s = 0.0f;
+    if (width <= ELL_MAXW) {
+        int c[ELL_MAXW];
+        MT v[ELL_MAXW];
+        float xv[ELL_MAXW];
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k) {
+            const size_t o = (size_t)k * n + row;
+            c[k] = k < width ? col[o] : 0;
+            v[k] = k < width ? val[o] : MT(0);
+        }
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k) xv[k] = k < width ? x[c[k]] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k)
+            if (k < width) s = fmaf(ell_val_f32(v[k]), xv[k] * scale, s);
+    } else {
+        for (int k = 0; k < width; ++k) {
+            const size_t o = (size_t)k * n + row;
+            s = fmaf(ell_val_f32(val[o]), x[col[o]] * scale, s);
+        }
+    }
+    return s;
+}
+
+}  // namespace phg

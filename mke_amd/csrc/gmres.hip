@@ -16,16 +16,20 @@
 // reference form) or classical Gram-Schmidt with one re-orthogonalisation
 // pass (CGS2: the j+1 dots of a pass are computed together).
 //
-// Reductions are deterministic: every block writes one partial per dot, and
-// the consumer adds the partials in block order (redundantly in every block
-// of the next kernel, or in a one-block kernel).  Results therefore depend
-// only on the inputs and the launch shape, never on timing.
+// Kernels per Arnoldi step j: MGS j+2 (SpMV, j MGS passes, final pass with
+// the norm); CGS2 5.  The Givens rotation of step j-1 and the decision to
+// continue are computed redundantly by every block at the start of the SpMV
+// kernel of step j (all blocks read the same inputs and agree); block 0
+// records them.  A whole restart cycle is captured once in a hipGraph and
+// replayed; kernels of a finished cycle return at once.
 //
-// Scalars (H, Givens rotations, g, convergence flags) live in device memory;
-// the host reads 40 bytes once per restart cycle.
+// Reductions are deterministic: every block writes one partial per dot, and
+// the consumer adds the partials in block order.  Results depend only on the
+// inputs and the launch shape, never on timing.
 #include <cmath>
 #include <cstring>
 
+#include "ell.h"
 #include "phg_internal.h"
 
 namespace phg {
@@ -36,11 +40,15 @@ constexpr int GM_RPT = 4;     // max rows per thread
 constexpr int GM_MAXG = 1024; // max blocks
 
 struct GmresState {
-    double H[(GM_MAXM + 1) * GM_MAXM];
-    double cs[GM_MAXM], sn[GM_MAXM], g[GM_MAXM + 1], y[GM_MAXM];
-    double hc[GM_MAXM + 1];        // CGS2 coefficients of the current pass
-    double beta, bnorm, rel, rtol, inner_rtol, vscale;
-    int m, active, k, iters, max_iters, converged, cycles, pad;
+    double H[(GM_MAXM + 1) * GM_MAXM];  // Hessenberg columns as computed (unrotated)
+    double R[GM_MAXM * GM_MAXM];        // rotated (upper triangular) columns
+    double cs[GM_MAXM], sn[GM_MAXM];
+    double g[GM_MAXM];                  // rotated right-hand side, final entries
+    double carry[GM_MAXM + 1];          // carry[j+1] = last entry after step j
+    double y[GM_MAXM];
+    double hc[GM_MAXM + 1];             // CGS2 coefficients of the current pass
+    double beta, bnorm, rel, rtol, inner_rtol;
+    int m, active, k, iters, iters0, max_iters, converged, cycles;
 };
 
 struct GmresWork {
@@ -50,8 +58,15 @@ struct GmresWork {
     DevBuf<double> part;   // (m+2) x G partial sums
     DevBuf<GmresState> st;
     GmresState* host = nullptr;  // pinned mirror
+    // captured restart cycle
+    hipStream_t cap = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void* key[4] = {nullptr, nullptr, nullptr, nullptr};
+    int key_m = 0, key_ortho = -1;
     ~GmresWork()
     {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (cap) (void)hipStreamDestroy(cap);
         if (host) (void)hipHostFree(host);
     }
 };
@@ -81,12 +96,22 @@ __device__ __forceinline__ double block_sum(double v, double* sh)
     return r;
 }
 
-// Sum of G block partials in a fixed order, computed by each wavefront.
+// Sum of G <= GM_MAXG block partials in a fixed order, computed by each
+// wavefront.  The partials were written by other XCDs in the previous
+// kernel, so every load is a round trip to memory: all of a lane's loads are
+// issued before the first add.
 __device__ __forceinline__ double sum_partials(const double* __restrict__ p, int G)
 {
     const int lane = threadIdx.x & 63;
+    double v[GM_MAXG / 64];
+#pragma unroll
+    for (int k = 0; k < GM_MAXG / 64; ++k) {
+        const int b = lane + 64 * k;
+        v[k] = (b < G) ? p[b] : 0.0;
+    }
     double s = 0.0;
-    for (int b = lane; b < G; b += 64) s += p[b];
+#pragma unroll
+    for (int k = 0; k < GM_MAXG / 64; ++k) s += v[k];
     return wave_sum(s);
 }
 
@@ -108,12 +133,7 @@ struct RowDot<double, double> {
                                                  const double* val, const double* x,
                                                  double inv)
     {
-        double s = 0.0;
-        for (int k = 0; k < width; ++k) {
-            const size_t o = (size_t)k * n + row;
-            s += val[o] * (x[col[o]] * inv);
-        }
-        return s;
+        return ell_row_f64(n, width, row, col, val, x, inv, true);
     }
 };
 template <typename MT>
@@ -121,14 +141,126 @@ struct RowDot<float, MT> {
     __device__ __forceinline__ static float run(int n, int width, int row, const int* col,
                                                 const MT* val, const float* x, float inv)
     {
-        float s = 0.0f;
-        for (int k = 0; k < width; ++k) {
-            const size_t o = (size_t)k * n + row;
-            s = fmaf(mval(val[o]), x[col[o]] * inv, s);
-        }
-        return s;
+        return ell_row_f32<MT>(n, width, row, col, val, x, inv);
     }
 };
+
+constexpr int GM_LB = 8;  // basis vectors loaded per batch
+
+// Block partials of the dots <wv, v_i>, i < nd, over this thread's rows
+// (v_skip is taken from `own` instead of memory); nd = 0 gives ||wv||^2.
+// The basis loads are issued GM_LB vectors at a time so that their memory
+// latency overlaps; one barrier for all dots.  Writes
+// part[(out0 + i) * G + block].
+template <typename VT>
+__device__ __forceinline__ void block_dots(int nd, int skip, const VT* own, const VT* __restrict__ V,
+                                           size_t ldv, int base, int rpt, int n, const VT* wv,
+                                           double* __restrict__ part, int G, int out0)
+{
+    __shared__ double shd[GM_BS / 64][GM_MAXM + 1];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (nd == 0) {
+        double acc = 0.0;
+#pragma unroll
+        for (int r = 0; r < GM_RPT; ++r) {
+            const int row = base + r * GM_BS;
+            if (r < rpt && row < n) acc += to_d(wv[r]) * to_d(wv[r]);
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) shd[wid][0] = acc;
+    }
+    for (int i0 = 0; i0 < nd; i0 += GM_LB) {
+        VT vb[GM_LB][GM_RPT];
+#pragma unroll
+        for (int q = 0; q < GM_LB; ++q) {
+            const int i = i0 + q;
+#pragma unroll
+            for (int r = 0; r < GM_RPT; ++r) {
+                const int row = base + r * GM_BS;
+                vb[q][r] = (i < nd && i != skip && r < rpt && row < n) ? V[(size_t)i * ldv + row]
+                                                                      : VT(0);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < GM_LB; ++q) {
+            const int i = i0 + q;
+            if (i >= nd) break;
+            double acc = 0.0;
+#pragma unroll
+            for (int r = 0; r < GM_RPT; ++r) {
+                const int row = base + r * GM_BS;
+                if (r < rpt && row < n) acc += to_d(wv[r]) * to_d(i == skip ? own[r] : vb[q][r]);
+            }
+            acc = wave_sum(acc);
+            if (lane == 0) shd[wid][i] = acc;
+        }
+    }
+    __syncthreads();
+    const int outs = nd == 0 ? 1 : nd;
+    if ((int)threadIdx.x < outs) {
+        double s = 0.0;
+        for (int q = 0; q < GM_BS / 64; ++q) s += shd[q][threadIdx.x];
+        part[(size_t)(out0 + threadIdx.x) * G + blockIdx.x] = s;
+    }
+}
+
+// Givens rotation of step j (j < 64) from the unrotated column H(0..j, j)
+// and the norm partials.  Executed by all 64 lanes of one wavefront, which
+// compute identical values (lane i loads row i, shuffles broadcast them);
+// lane 0 stores the results when `write`.  Returns whether the cycle goes
+// on, and hn = ||w_j||.
+struct GivensOut {
+    bool cont;
+    double hn;
+};
+
+// Value of v in lane `lane` (wave-uniform index): two v_readlane_b32.
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+    const unsigned long long u = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, lane);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ GivensOut givens_step(GmresState* __restrict__ st, int j,
+                                 const double* __restrict__ part, int G, bool write)
+{
+    const int lane = threadIdx.x & 63;
+    const int m = st->m;
+    const double s = sum_partials(part + (size_t)(j + 1) * G, G);
+    const double hreg = lane <= j ? st->H[lane * m + j] : 0.0;
+    const double creg = lane < j ? st->cs[lane] : 0.0;
+    const double sreg = lane < j ? st->sn[lane] : 0.0;
+    const double hn = sqrt(s);
+    double cur = readlane_d(hreg, 0);
+    for (int i = 0; i < j; ++i) {
+        const double c = readlane_d(creg, i), sn = readlane_d(sreg, i);
+        const double nxt = readlane_d(hreg, i + 1);
+        const double t = c * cur + sn * nxt;
+        cur = -sn * cur + c * nxt;
+        if (write && lane == 0) st->R[i * m + j] = t;
+    }
+    const double r = hypot(cur, hn);
+    const double cj = cur / r, sj = hn / r;
+    const double carry = st->carry[j];
+    const double gj = cj * carry, cn = -sj * carry;
+    const int iters = st->iters0 + j + 1;
+    const double stop = fmax(st->rtol * st->bnorm, st->inner_rtol * st->beta);
+    const bool cont = !(fabs(cn) <= stop || !(hn > 0.0) || iters >= st->max_iters || j + 1 == m);
+    if (write && lane == 0) {
+        st->R[j * m + j] = r;
+        st->H[(j + 1) * m + j] = hn;
+        st->cs[j] = cj;
+        st->sn[j] = sj;
+        st->g[j] = gj;
+        st->carry[j + 1] = cn;
+        st->k = j + 1;
+        st->iters = iters;
+        if (!cont) st->active = 0;
+    }
+    return GivensOut{cont, hn};
+}
 
 // ---------------------------------------------------------------- kernels --
 // r = b - A x in FP64 (the refinement residual); W0 = r; partial ||r||^2.
@@ -146,11 +278,7 @@ k_residual(int n, int rpt, int width, const int* __restrict__ col,
     for (int r = 0; r < rpt; ++r) {
         const int row = base + r * GM_BS;
         if (row >= n) break;
-        double s = 0.0;
-        for (int k = 0; k < width; ++k) {
-            const size_t o = (size_t)k * n + row;
-            s += val[o] * x[col[o]];
-        }
+        const double s = ell_row_f64(n, width, row, col, val, x, 1.0, false);
         const double rr = b[row] - s;
         W0[row] = (VT)rr;
         acc += rr * rr;
@@ -198,6 +326,7 @@ __global__ void k_cycle_start(GmresState* st, const double* __restrict__ part, i
         st->beta = beta;
         st->rel = beta / st->bnorm;
         st->k = 0;
+        st->iters0 = st->iters;
         if (st->rel <= st->rtol || beta == 0.0) {
             st->converged = 1;
             st->active = 0;
@@ -205,53 +334,57 @@ __global__ void k_cycle_start(GmresState* st, const double* __restrict__ part, i
             st->active = 0;
         } else {
             st->active = 1;
-            st->vscale = beta;
-            st->g[0] = beta;
-            for (int i = 1; i <= st->m; ++i) st->g[i] = 0.0;
+            st->carry[0] = beta;
             st->cycles += 1;
         }
     }
 }
 
-// Arnoldi step j: v_j = W_src / vscale (stored), W_dst = A v_j, and the
-// partial of <W_dst, v_0> (MGS) when ndots = 1, or of <W_dst, v_i> for all
-// i <= j (CGS2 first pass) when ndots = j + 1.
+// Arnoldi step j.  For j >= 1 first the Givens rotation of step j-1 (and
+// the decision to go on); then v_j = W_src / h (stored), W_dst = A v_j, and
+// the partial of <W_dst, v_0> (MGS, ndots = 1) or of <W_dst, v_i> for all
+// i <= j (CGS2 first pass, ndots = j + 1).
 template <typename VT, typename MT>
 __global__ void __launch_bounds__(GM_BS)
 k_step_spmv(int n, int rpt, int width, const int* __restrict__ col, const MT* __restrict__ val,
             const VT* __restrict__ Wsrc, VT* __restrict__ V, size_t ldv, VT* __restrict__ Wdst,
-            int j, int ndots, double* __restrict__ part, int G, const GmresState* __restrict__ st)
+            int j, int ndots, double* __restrict__ part, int G, GmresState* __restrict__ st)
 {
     __shared__ double sh[GM_BS / 64];
+    __shared__ double sh_h;
+    __shared__ int sh_go;
     if (!st->active) return;
-    const VT inv = (VT)(1.0 / st->vscale);
+    if (j == 0) {
+        if (threadIdx.x == 0) {
+            sh_h = st->beta;
+            sh_go = 1;
+        }
+    } else if (threadIdx.x < 64) {
+        const GivensOut g = givens_step(st, j - 1, part, G, blockIdx.x == 0);
+        if (threadIdx.x == 0) {
+            sh_h = g.hn;
+            sh_go = g.cont;
+        }
+    }
+    __syncthreads();
+    if (!sh_go) return;
+    const VT inv = (VT)(1.0 / sh_h);
     VT* Vj = V + (size_t)j * ldv;
-    VT w[GM_RPT];
+    VT w[GM_RPT], vj[GM_RPT];
     const int base = blockIdx.x * GM_BS * rpt + threadIdx.x;
 #pragma unroll
     for (int r = 0; r < GM_RPT; ++r) {
         const int row = base + r * GM_BS;
         w[r] = VT(0);
+        vj[r] = VT(0);
         if (r < rpt && row < n) {
-            Vj[row] = Wsrc[row] * inv;
+            vj[r] = Wsrc[row] * inv;
+            Vj[row] = vj[r];
             w[r] = RowDot<VT, MT>::run(n, width, row, col, val, Wsrc, inv);
             Wdst[row] = w[r];
         }
     }
-    for (int i = 0; i < ndots; ++i) {
-        const VT* Vi = V + (size_t)i * ldv;
-        double acc = 0.0;
-#pragma unroll
-        for (int r = 0; r < GM_RPT; ++r) {
-            const int row = base + r * GM_BS;
-            if (r < rpt && row < n) {
-                const VT vi = (i == j) ? Wsrc[row] * inv : Vi[row];
-                acc += to_d(w[r]) * to_d(vi);
-            }
-        }
-        acc = block_sum(acc, sh);
-        if (threadIdx.x == 0) part[(size_t)i * G + blockIdx.x] = acc;
-    }
+    block_dots<VT>(ndots, j, vj, V, ldv, base, rpt, n, w, part, G, 0);
 }
 
 // MGS pass i (1 <= i <= j+1): h_{i-1,j} from the partials, w -= h v_{i-1},
@@ -262,40 +395,56 @@ k_mgs(int n, int rpt, VT* __restrict__ w, const VT* __restrict__ V, size_t ldv, 
       double* __restrict__ part, int G, GmresState* __restrict__ st)
 {
     __shared__ double sh[GM_BS / 64];
-    if (!st->active) return;
+    // every load of the kernel is issued before the first use: the flag, the
+    // G partials and the row data are independent round trips
+    const int active = st->active;
     const double h = sum_partials(part + (size_t)(i - 1) * G, G);
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->H[(i - 1) * st->m + j] = h;
-    const VT hv = (VT)h;
     const VT* Vp = V + (size_t)(i - 1) * ldv;
     const VT* Vi = V + (size_t)i * ldv;
     const bool last = (i == j + 1);
-    double acc = 0.0;
     const int base = blockIdx.x * GM_BS * rpt + threadIdx.x;
-    for (int r = 0; r < rpt; ++r) {
+    VT wv[GM_RPT], pv[GM_RPT], iv[GM_RPT];
+#pragma unroll
+    for (int r = 0; r < GM_RPT; ++r) {
         const int row = base + r * GM_BS;
-        if (row >= n) break;
-        const VT wr = w[row] - hv * Vp[row];
-        w[row] = wr;
-        const double o = last ? to_d(wr) : to_d(Vi[row]);
-        acc += to_d(wr) * o;
+        const bool in = r < rpt && row < n;
+        wv[r] = in ? w[row] : VT(0);
+        pv[r] = in ? Vp[row] : VT(0);
+        iv[r] = (in && !last) ? Vi[row] : VT(0);
+    }
+    if (!active) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->H[(i - 1) * st->m + j] = h;
+    const VT hv = (VT)h;
+    double acc = 0.0;
+#pragma unroll
+    for (int r = 0; r < GM_RPT; ++r) {
+        const int row = base + r * GM_BS;
+        if (r < rpt && row < n) {
+            const VT wr = wv[r] - hv * pv[r];
+            w[row] = wr;
+            const double o = last ? to_d(wr) : to_d(iv[r]);
+            acc += to_d(wr) * o;
+        }
     }
     acc = block_sum(acc, sh);
     if (threadIdx.x == 0) part[(size_t)i * G + blockIdx.x] = acc;
 }
 
-// CGS2: one block adds the partials of the j+1 dots of a pass (thread i
-// handles dot i); pass 0 stores them, pass 1 adds the correction into H.
-__global__ void k_cgs_reduce(int j, int pass, const double* __restrict__ part, int G,
-                             GmresState* __restrict__ st)
+// CGS2: adds the partials of the j+1 dots of a pass, one wavefront per dot;
+// pass 0 stores them in H, pass 1 adds the correction.
+__global__ void __launch_bounds__(GM_BS)
+k_cgs_reduce(int j, int pass, const double* __restrict__ part, int G, GmresState* __restrict__ st)
 {
     if (!st->active) return;
-    const int i = threadIdx.x;
-    if (i > j) return;
-    double s = 0.0;
-    for (int b = 0; b < G; ++b) s += part[(size_t)i * G + b];
-    st->hc[i] = s;
-    if (pass == 0) st->H[i * st->m + j] = s;
-    else st->H[i * st->m + j] += s;
+    const int wid = threadIdx.x >> 6;
+    for (int i = wid; i <= j; i += GM_BS / 64) {
+        const double s = sum_partials(part + (size_t)i * G, G);
+        if ((threadIdx.x & 63) == 0) {
+            st->hc[i] = s;
+            if (pass == 0) st->H[i * st->m + j] = s;
+            else st->H[i * st->m + j] += s;
+        }
+    }
 }
 
 // CGS2 update: w -= sum_i hc_i v_i (i = 0..j); then partial dots with all
@@ -315,74 +464,60 @@ k_cgs_update(int n, int rpt, VT* __restrict__ w, const VT* __restrict__ V, size_
 #pragma unroll
     for (int r = 0; r < GM_RPT; ++r) {
         const int row = base + r * GM_BS;
-        wv[r] = VT(0);
-        if (r < rpt && row < n) {
-            VT a = w[row];
-            for (int i = 0; i <= j; ++i) a = a - (VT)hs[i] * V[(size_t)i * ldv + row];
-            wv[r] = a;
-            w[row] = a;
-        }
+        wv[r] = (r < rpt && row < n) ? w[row] : VT(0);
     }
-    const int ndots = final_pass ? 1 : j + 1;
-    for (int i = 0; i < ndots; ++i) {
-        double acc = 0.0;
+    // w -= sum_i hc_i v_i in order i = 0..j, basis loads batched
+    for (int i0 = 0; i0 <= j; i0 += GM_LB) {
+        VT vb[GM_LB][GM_RPT];
 #pragma unroll
-        for (int r = 0; r < GM_RPT; ++r) {
-            const int row = base + r * GM_BS;
-            if (r < rpt && row < n) {
-                const double o = final_pass ? to_d(wv[r]) : to_d(V[(size_t)i * ldv + row]);
-                acc += to_d(wv[r]) * o;
+        for (int q = 0; q < GM_LB; ++q)
+#pragma unroll
+            for (int r = 0; r < GM_RPT; ++r) {
+                const int row = base + r * GM_BS;
+                vb[q][r] = (i0 + q <= j && r < rpt && row < n)
+                               ? V[(size_t)(i0 + q) * ldv + row] : VT(0);
             }
+#pragma unroll
+        for (int q = 0; q < GM_LB; ++q) {
+            if (i0 + q > j) break;
+            const VT h = (VT)hs[i0 + q];
+#pragma unroll
+            for (int r = 0; r < GM_RPT; ++r) wv[r] = wv[r] - h * vb[q][r];
         }
-        acc = block_sum(acc, sh);
-        if (threadIdx.x == 0) part[(size_t)(final_pass ? j + 1 : i) * G + blockIdx.x] = acc;
     }
+#pragma unroll
+    for (int r = 0; r < GM_RPT; ++r) {
+        const int row = base + r * GM_BS;
+        if (r < rpt && row < n) w[row] = wv[r];
+    }
+    if (final_pass) block_dots<VT>(0, -1, wv, V, ldv, base, rpt, n, wv, part, G, j + 1);
+    else block_dots<VT>(j + 1, -1, wv, V, ldv, base, rpt, n, wv, part, G, 0);
 }
 
-// Norm, Givens rotations and the residual estimate of step j (one wave).
-__global__ void k_givens(int j, const double* __restrict__ part, int G,
-                         GmresState* __restrict__ st)
+// End of a cycle: the pending Givens rotation (if the cycle ran to its last
+// step) and y = R^{-1} g, one block.
+__global__ void __launch_bounds__(GM_BS)
+k_solve_y(const double* __restrict__ part, int G, GmresState* __restrict__ st)
 {
-    if (!st->active) return;
-    const double s = sum_partials(part + (size_t)(j + 1) * G, G);
-    if (threadIdx.x != 0) return;
-    const int m = st->m;
-    double* H = st->H;
-    const double hn = sqrt(s);
-    H[(j + 1) * m + j] = hn;
-    st->vscale = hn;
-    for (int i = 0; i < j; ++i) {
-        const double t = st->cs[i] * H[i * m + j] + st->sn[i] * H[(i + 1) * m + j];
-        H[(i + 1) * m + j] = -st->sn[i] * H[i * m + j] + st->cs[i] * H[(i + 1) * m + j];
-        H[i * m + j] = t;
-    }
-    const double a = H[j * m + j], c = H[(j + 1) * m + j];
-    const double r = hypot(a, c);
-    st->cs[j] = a / r;
-    st->sn[j] = c / r;
-    H[j * m + j] = r;
-    H[(j + 1) * m + j] = 0.0;
-    st->g[j + 1] = -st->sn[j] * st->g[j];
-    st->g[j] = st->cs[j] * st->g[j];
-    st->k = j + 1;
-    st->iters += 1;
-    // end the cycle: target reached (or the refinement step's share), Krylov
-    // space exhausted (hn = 0), step budget spent, or cycle full
-    const double est = fabs(st->g[j + 1]);
-    const double stop = fmax(st->rtol * st->bnorm, st->inner_rtol * st->beta);
-    if (est <= stop || !(hn > 0.0) || st->iters >= st->max_iters || j + 1 == m) st->active = 0;
-}
-
-// y = H^{-1} g (k x k upper triangular), one thread.
-__global__ void k_solve_y(GmresState* __restrict__ st)
-{
-    if (st->converged || threadIdx.x != 0) return;
+    __shared__ double Rs[GM_MAXM * GM_MAXM];
+    __shared__ double gs[GM_MAXM];
+    if (st->converged) return;
+    if (st->active && threadIdx.x < 64) givens_step(st, st->k, part, G, true);
+    __syncthreads();
     const int k = st->k, m = st->m;
+    // stage R and g in LDS with all threads (parallel loads), then solve
+    for (int e = threadIdx.x; e < k * k; e += blockDim.x) Rs[e] = st->R[(e / k) * m + (e % k)];
+    for (int e = threadIdx.x; e < k; e += blockDim.x) gs[e] = st->g[e];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    st->active = 0;
+    double y[GM_MAXM];
     for (int i = k - 1; i >= 0; --i) {
-        double s = st->g[i];
-        for (int l = i + 1; l < k; ++l) s -= st->H[i * m + l] * st->y[l];
-        st->y[i] = s / st->H[i * m + i];
+        double s = gs[i];
+        for (int l = i + 1; l < k; ++l) s -= Rs[i * k + l] * y[l];
+        y[i] = s / Rs[i * k + i];
     }
+    for (int i = 0; i < k; ++i) st->y[i] = y[i];
 }
 
 // x += sum_i y_i v_i, accumulated in FP64.
@@ -449,10 +584,10 @@ int prepare_work(phg_matrix_s* A, int prec, int m, GmresWork** out)
 }
 
 template <typename VT, typename MT>
-int gmres_run(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double* b, double* x,
-              const phg_gmres_opts& o, phg_gmres_info* info, hipStream_t s)
+void enqueue_cycle(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double* b, double* x,
+                   int m, bool cgs, hipStream_t s)
 {
-    const int n = w->n, m = o.restart, G = w->G, rpt = w->rpt;
+    const int n = w->n, G = w->G, rpt = w->rpt;
     const size_t ldv = w->ldv;
     VT* V = (VT*)w->V.p;
     VT* W0 = (VT*)w->W.p;
@@ -460,6 +595,42 @@ int gmres_run(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double* b, 
     double* part = w->part.p;
     GmresState* st = w->st.p;
     const dim3 grid(G), block(GM_BS);
+    for (int j = 0; j < m; ++j) {
+        VT* src = (j & 1) ? W1 : W0;
+        VT* dst = (j & 1) ? W0 : W1;
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_step_spmv<VT, MT>), grid, block, 0, s, n, rpt,
+                           E.width, E.col.p, E.val.p, src, V, ldv, dst, j, cgs ? j + 1 : 1,
+                           part, G, st);
+        if (!cgs) {
+            for (int i = 1; i <= j + 1; ++i)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mgs<VT>), grid, block, 0, s, n, rpt, dst, V,
+                                   ldv, i, j, part, G, st);
+        } else {
+            hipLaunchKernelGGL(k_cgs_reduce, dim3(1), block, 0, s, j, 0, part, G, st);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cgs_update<VT>), grid, block, 0, s, n, rpt, dst,
+                               V, ldv, j, 0, part, G, st);
+            hipLaunchKernelGGL(k_cgs_reduce, dim3(1), block, 0, s, j, 1, part, G, st);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cgs_update<VT>), grid, block, 0, s, n, rpt, dst,
+                               V, ldv, j, 1, part, G, st);
+        }
+    }
+    hipLaunchKernelGGL(k_solve_y, dim3(1), dim3(GM_BS), 0, s, part, G, st);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_xupdate<VT>), grid, block, 0, s, n, rpt, V, ldv, x, st);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_residual<VT>), grid, block, 0, s, n, rpt,
+                       A->ell64.width, A->ell64.col.p, A->ell64.val.p, b, x, W0, part, st);
+    hipLaunchKernelGGL(k_cycle_start, dim3(1), dim3(64), 0, s, st, part, G);
+}
+
+template <typename VT, typename MT>
+int gmres_run(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double* b, double* x,
+              const phg_gmres_opts& o, phg_gmres_info* info, hipStream_t s)
+{
+    const int n = w->n, m = o.restart, G = w->G, rpt = w->rpt;
+    VT* W0 = (VT*)w->W.p;
+    double* part = w->part.p;
+    GmresState* st = w->st.p;
+    const dim3 grid(G), block(GM_BS);
+    const bool cgs = o.ortho == PHG_CGS2;
 
     // parameters of the solve
     GmresState init;
@@ -469,9 +640,7 @@ int gmres_run(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double* b, 
     init.inner_rtol = (o.precision == PHG_F64) ? 0.0 : o.inner_rtol;
     init.max_iters = o.max_iters;
     PHG_HIP(hipMemcpyAsync(st, &init, sizeof init, hipMemcpyHostToDevice, s));
-    if (!o.use_x0) {
-        hipLaunchKernelGGL(k_fill0, dim3((n + 255) / 256), dim3(256), 0, s, n, x);
-    }
+    if (!o.use_x0) hipLaunchKernelGGL(k_fill0, dim3((n + 255) / 256), dim3(256), 0, s, n, x);
     hipLaunchKernelGGL(k_sumsq, grid, block, 0, s, n, rpt, b, part);
     hipLaunchKernelGGL(k_set_bnorm, dim3(1), dim3(64), 0, s, st, part, G);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_residual<VT>), grid, block, 0, s, n, rpt,
@@ -479,40 +648,40 @@ int gmres_run(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double* b, 
     hipLaunchKernelGGL(k_cycle_start, dim3(1), dim3(64), 0, s, st, part, G);
     PHG_HIP(hipGetLastError());
 
-    const bool cgs = o.ortho == PHG_CGS2;
+    if (o.graph) {
+        const void* key[4] = {b, x, E.val.p, A->ell64.val.p};
+        const bool same = w->exec && w->key_m == m && w->key_ortho == o.ortho &&
+                          !memcmp(key, w->key, sizeof key);
+        if (!same) {
+            if (w->exec) {
+                PHG_HIP(hipGraphExecDestroy(w->exec));
+                w->exec = nullptr;
+            }
+            if (!w->cap) PHG_HIP(hipStreamCreateWithFlags(&w->cap, hipStreamNonBlocking));
+            hipGraph_t g = nullptr;
+            PHG_HIP(hipStreamBeginCapture(w->cap, hipStreamCaptureModeThreadLocal));
+            enqueue_cycle<VT, MT>(A, E, w, b, x, m, cgs, w->cap);
+            PHG_HIP(hipStreamEndCapture(w->cap, &g));
+            hipError_t e = hipGraphInstantiate(&w->exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            PHG_HIP(e);
+            memcpy(w->key, key, sizeof key);
+            w->key_m = m;
+            w->key_ortho = o.ortho;
+        }
+    }
+
     for (;;) {
         PHG_HIP(hipMemcpyAsync(w->host, st, sizeof(GmresState), hipMemcpyDeviceToHost, s));
         PHG_HIP(hipStreamSynchronize(s));
         const GmresState& h = *w->host;
         if (h.converged || !h.active) break;
-        for (int j = 0; j < m; ++j) {
-            VT* src = (j & 1) ? W1 : W0;
-            VT* dst = (j & 1) ? W0 : W1;
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_step_spmv<VT, MT>), grid, block, 0, s, n, rpt,
-                               E.width, E.col.p, E.val.p, src, V, ldv, dst, j,
-                               cgs ? j + 1 : 1, part, G, st);
-            if (!cgs) {
-                for (int i = 1; i <= j + 1; ++i)
-                    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mgs<VT>), grid, block, 0, s, n, rpt, dst,
-                                       V, ldv, i, j, part, G, st);
-            } else {
-                hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(64), 0, s, j, 0, part, G, st);
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cgs_update<VT>), grid, block, 0, s, n, rpt,
-                                   dst, V, ldv, j, 0, part, G, st);
-                hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(64), 0, s, j, 1, part, G, st);
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cgs_update<VT>), grid, block, 0, s, n, rpt,
-                                   dst, V, ldv, j, 1, part, G, st);
-            }
-            hipLaunchKernelGGL(k_givens, dim3(1), dim3(64), 0, s, j, part, G, st);
+        if (o.graph) {
+            PHG_HIP(hipGraphLaunch(w->exec, s));
+        } else {
+            enqueue_cycle<VT, MT>(A, E, w, b, x, m, cgs, s);
+            PHG_HIP(hipGetLastError());
         }
-        PHG_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_solve_y, dim3(1), dim3(64), 0, s, st);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_xupdate<VT>), grid, block, 0, s, n, rpt, V, ldv, x,
-                           st);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_residual<VT>), grid, block, 0, s, n, rpt,
-                           A->ell64.width, A->ell64.col.p, A->ell64.val.p, b, x, W0, part, st);
-        hipLaunchKernelGGL(k_cycle_start, dim3(1), dim3(64), 0, s, st, part, G);
-        PHG_HIP(hipGetLastError());
     }
     const GmresState& h = *w->host;
     if (info) {

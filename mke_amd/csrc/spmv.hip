@@ -8,6 +8,7 @@
 // FP64 products are accumulated in CSR column order from 0.0 without fused
 // multiply-add (this file is compiled with -ffp-contract=off), so y = A x is
 // bit-identical to a plain CPU CSR loop.
+#include "ell.h"
 #include "phg_internal.h"
 
 namespace phg {
@@ -68,12 +69,7 @@ ell_spmv_f64(int n, int width, const int* __restrict__ col, const double* __rest
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double s = 0.0;
-    for (int k = 0; k < width; ++k) {
-        const size_t o = (size_t)k * n + i;
-        s += val[o] * x[col[o]];
-    }
-    y[i] = s;
+    y[i] = ell_row_f64(n, width, i, col, val, x, 1.0, false);
 }
 
 template <typename V>
@@ -83,12 +79,7 @@ ell_spmv_f32(int n, int width, const int* __restrict__ col, const V* __restrict_
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float s = 0.0f;
-    for (int k = 0; k < width; ++k) {
-        const size_t o = (size_t)k * n + i;
-        s = fmaf(val_f32(val[o]), x[col[o]], s);
-    }
-    y[i] = s;
+    y[i] = ell_row_f32<V>(n, width, i, col, val, x, 1.0f);
 }
 
 // Y = A X, X/Y row-major n x nrhs.  One wavefront per row: lane r owns

@@ -28,7 +28,7 @@ _PI, _PD = C.POINTER(C.c_int), C.POINTER(C.c_double)
 
 class GmresOpts(C.Structure):
     _fields_ = [("restart", _I), ("max_iters", _I), ("rtol", _D), ("ortho", _I),
-                ("precision", _I), ("inner_rtol", _D), ("use_x0", _I)]
+                ("precision", _I), ("inner_rtol", _D), ("use_x0", _I), ("graph", _I)]
 
 
 class GmresInfo(C.Structure):
@@ -257,12 +257,13 @@ class Matrix:
               "phg_spmm_f32")
 
     def solve(self, x, b, restart=30, rtol=1e-10, max_iters=200000, ortho=MGS, precision=F64,
-              inner_rtol=1e-5, use_x0=False, stream=None):
+              inner_rtol=1e-5, use_x0=False, graph=True, stream=None):
         """Solves A x = b (Solver::solve) with GMRES(restart); returns the info dict."""
         o = GmresOpts()
         lib().phg_gmres_default_opts(C.byref(o))
         o.restart, o.max_iters, o.rtol = restart, max_iters, rtol
         o.ortho, o.precision, o.inner_rtol, o.use_x0 = ortho, precision, inner_rtol, int(use_x0)
+        o.graph = int(graph)
         info = GmresInfo()
         check(lib().phg_gmres(self.h, _ptr(b), _ptr(x), C.byref(o), C.byref(info),
                               _stream(stream)), "phg_gmres")

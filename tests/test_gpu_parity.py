@@ -188,3 +188,19 @@ def test_solver_add_path_matches_assembly():
             S.add(i, int(ci[e]), float(v[e]))
     r2, c2, v2 = S.matrix().csr()
     assert np.array_equal(r2, rp) and np.array_equal(c2, ci) and np.array_equal(v2, v)
+
+
+@pytest.mark.parametrize("ortho", [P.MGS, P.CGS2])
+def test_gmres_graph_replay_matches_eager_and_repeats(ortho):
+    """The captured restart cycle gives bit-identical results to eager
+    launches, and replays are deterministic (reductions in fixed order)."""
+    pm, _ = _mesh(5)
+    A = P.Matrix.assemble(pm, P.HELMHOLTZ, CM, CL)
+    b = _dev(np.random.default_rng(11).standard_normal(A.n_rows))
+    xs = []
+    for graph in (False, True, True):
+        x = torch.zeros_like(b)
+        info = A.solve(x, b, ortho=ortho, graph=graph)
+        assert info["converged"]
+        xs.append(x.cpu().numpy())
+    assert np.array_equal(xs[0], xs[1]) and np.array_equal(xs[1], xs[2])
