@@ -39,8 +39,10 @@ enum { PHG_F64 = 0, PHG_F32 = 1, PHG_BF16 = 2 };
 /* element kinds (callbacks of src/slaplace.cpp:171-197 and
  * examples/barvortex_impl.h:57-80) */
 enum { PHG_MASS = 0, PHG_LAPLACE = 1, PHG_HELMHOLTZ = 2 };
-/* orthogonalisation of the Arnoldi step */
-enum { PHG_MGS = 0, PHG_CGS2 = 1 };
+/* orthogonalisation of the Arnoldi step: modified Gram-Schmidt, classical
+ * Gram-Schmidt with one re-orthogonalisation pass, or one classical pass
+ * (single pass: block solver only) */
+enum { PHG_MGS = 0, PHG_CGS2 = 1, PHG_CGS1 = 2 };
 
 /* ---- library ---------------------------------------------------------- */
 /* Version string and last error of the calling thread. */
@@ -161,6 +163,25 @@ void phg_gmres_default_opts(phg_gmres_opts *o);
 int phg_gmres(phg_matrix_t A, const double *d_b, double *d_x,
               const phg_gmres_opts *opts, phg_gmres_info *info,
               phg_stream_t s);
+
+/* ---- batched: 64 right-hand sides, one matrix (no single reference call:
+ *      it batches the Solver::solve of examples/barvortex_impl.h:296 over
+ *      ensemble members sharing one mesh) ------------------------------- */
+typedef struct {
+    int iterations;          /* block Arnoldi steps                        */
+    int cycles;              /* restart cycles (= refinement steps)        */
+    double max_rel_residual; /* max over columns of the true FP64 residual */
+    int converged;
+} phg_block_info;
+/* Block GMRES(restart <= 32) for A X = B with nrhs = 64 columns.  B and X
+ * are device arrays of n_rows x 64 FP64, row-major (entry (i, q) at
+ * i * 64 + q).  Krylov basis in FP32 (matrix values FP32, or BF16 with
+ * opts->precision = PHG_BF16), block Gram-Schmidt on MFMA, FP64 Gram
+ * matrices, Hessenberg reduction and refinement residual; stops when every
+ * column has ||b - A x|| / ||b|| <= rtol. */
+int phg_block_gmres(phg_matrix_t A, int nrhs, const double *d_B, double *d_X,
+                    const phg_gmres_opts *opts, phg_block_info *info,
+                    phg_stream_t s);
 
 #ifdef __cplusplus
 }

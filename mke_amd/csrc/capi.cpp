@@ -28,6 +28,8 @@ int p2u_f64(phg_mesh_s* m, const double* p, const double* bnd, double* u, hipStr
 int gmres_solve(phg_matrix_s* A, const double* b, double* x, const phg_gmres_opts* opts,
                 phg_gmres_info* info, hipStream_t s);
 void csr_structure(const SphereMesh& m, std::vector<int>& rp, std::vector<int>& ci);
+int block_gmres(phg_matrix_s* A, int nrhs, const double* B, double* X, const phg_gmres_opts* opts,
+                phg_block_info* info, hipStream_t s);
 
 int ensure_mesh_device(phg_mesh_s* m)
 {
@@ -61,6 +63,7 @@ int matrix_from_csr_host(int n_rows, int n_cols, const int* rp, const int* ci,
 phg_matrix_s::~phg_matrix_s()
 {
     for (auto& w : work) w.reset();
+    phg::block_destroy(bwork);
 }
 
 using namespace phg;
@@ -355,6 +358,23 @@ int phg_gmres(phg_matrix_t A, const double* d_b, double* d_x, const phg_gmres_op
         return PHG_ERR_ARG;
     }
     return gmres_solve(A, d_b, d_x, &o, info, (hipStream_t)s);
+}
+
+int phg_block_gmres(phg_matrix_t A, int nrhs, const double* d_B, double* d_X,
+                    const phg_gmres_opts* opts, phg_block_info* info, phg_stream_t s)
+{
+    if (!A || !d_B || !d_X) {
+        set_error("phg_block_gmres: null argument");
+        return PHG_ERR_ARG;
+    }
+    phg_gmres_opts o;
+    if (opts) o = *opts;
+    else phg_gmres_default_opts(&o);
+    if (o.max_iters < 0 || !(o.rtol >= 0)) {
+        set_error("phg_block_gmres: bad options");
+        return PHG_ERR_ARG;
+    }
+    return block_gmres(A, nrhs, d_B, d_X, &o, info, (hipStream_t)s);
 }
 
 }  // extern "C"

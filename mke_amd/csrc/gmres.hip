@@ -704,6 +704,10 @@ int gmres_solve(phg_matrix_s* A, const double* b, double* x, const phg_gmres_opt
         set_error("gmres: restart must be in 1..64");
         return PHG_ERR_ARG;
     }
+    if (o.ortho != PHG_MGS && o.ortho != PHG_CGS2) {
+        set_error("gmres: ortho must be PHG_MGS or PHG_CGS2");
+        return PHG_ERR_ARG;
+    }
     if (A->n_rows != A->n_cols) {
         set_error("gmres: matrix is not square");
         return PHG_ERR_ARG;

@@ -66,7 +66,9 @@ struct Ell {
 };
 
 struct GmresWork;  // gmres.hip
+struct BlockWork;  // block.hip
 void destroy_gmres_work(GmresWork* w);
+void block_destroy(BlockWork* w);
 struct GmresWorkDeleter {
     void operator()(GmresWork* w) const { destroy_gmres_work(w); }
 };
@@ -91,6 +93,7 @@ struct phg_matrix_s {
     phg::Ell<float> ell32;
     phg::Ell<uint16_t> ell16;      // bfloat16 bit patterns
     std::unique_ptr<phg::GmresWork, phg::GmresWorkDeleter> work[3];  // per precision
+    phg::BlockWork* bwork = nullptr;  // block GMRES workspace
     ~phg_matrix_s();
 };
 

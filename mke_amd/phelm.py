@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(_HERE, "libphelm_hip.so")
 
 F64, F32, BF16 = 0, 1, 2
 MASS, LAPLACE, HELMHOLTZ = 0, 1, 2
-MGS, CGS2 = 0, 1
+MGS, CGS2, CGS1 = 0, 1, 2
 OK, ERR_NOCONV = 0, 1
 
 _I, _D, _L, _P = C.c_int, C.c_double, C.c_long, C.c_void_p
@@ -34,6 +34,11 @@ class GmresOpts(C.Structure):
 class GmresInfo(C.Structure):
     _fields_ = [("iterations", _I), ("restarts", _I), ("refinements", _I),
                 ("rel_residual", _D), ("converged", _I)]
+
+
+class BlockInfo(C.Structure):
+    _fields_ = [("iterations", _I), ("cycles", _I), ("max_rel_residual", _D),
+                ("converged", _I)]
 
 
 # every symbol of include/phelm_gpu.h with (restype, argtypes)
@@ -66,6 +71,7 @@ SIGNATURES = {
     "phg_p2u": (_I, [_P, _P, _P, _P, _P]),
     "phg_gmres_default_opts": (None, [C.POINTER(GmresOpts)]),
     "phg_gmres": (_I, [_P, _P, _P, C.POINTER(GmresOpts), C.POINTER(GmresInfo), _P]),
+    "phg_block_gmres": (_I, [_P, _I, _P, _P, C.POINTER(GmresOpts), C.POINTER(BlockInfo), _P]),
 }
 
 _lib = None
@@ -270,6 +276,20 @@ class Matrix:
         return dict(iterations=info.iterations, restarts=info.restarts,
                     refinements=info.refinements, rel_residual=info.rel_residual,
                     converged=bool(info.converged))
+
+    def solve_block(self, X, B, restart=30, rtol=1e-10, max_iters=100000, precision=F32,
+                    inner_rtol=1e-5, use_x0=False, ortho=CGS2, stream=None):
+        """Block GMRES for 64 right-hand sides: X, B are n x 64 FP64 device
+        tensors (row-major).  Returns the info dict."""
+        o = GmresOpts()
+        lib().phg_gmres_default_opts(C.byref(o))
+        o.restart, o.max_iters, o.rtol, o.precision = restart, max_iters, rtol, precision
+        o.inner_rtol, o.use_x0, o.ortho = inner_rtol, int(use_x0), ortho
+        info = BlockInfo()
+        check(lib().phg_block_gmres(self.h, 64, _ptr(B), _ptr(X), C.byref(o), C.byref(info),
+                                    _stream(stream)), "phg_block_gmres")
+        return dict(iterations=info.iterations, cycles=info.cycles,
+                    max_rel_residual=info.max_rel_residual, converged=bool(info.converged))
 
     def close(self):
         if self.h:
