@@ -36,9 +36,16 @@ enum {
 
 /* value precisions of a matrix / solve */
 enum { PHG_F64 = 0, PHG_F32 = 1, PHG_BF16 = 2 };
-/* element kinds (callbacks of src/slaplace.cpp:171-197 and
- * examples/barvortex_impl.h:57-80) */
-enum { PHG_MASS = 0, PHG_LAPLACE = 1, PHG_HELMHOLTZ = 2 };
+/* element kinds: callbacks of src/slaplace.cpp:171-197 and
+ * examples/barvortex_impl.h:57-80 (GK15 trapezoid integrals), and the
+ * SphereJacobian callbacks of src/sjacobian.cpp:60-134 (7-point rule):
+ * JAC_MASS = id_cb (cos x phi_i phi_j), DIFF1/DIFF2 = diff_1_rp/diff_2_rp
+ * (d_x phi_j phi_i, d_y phi_j phi_i), DIFF1_COS/DIFF2_COS = the same times
+ * cos x (diff_1_cos_rp/diff_2_cos_rp) */
+enum {
+    PHG_MASS = 0, PHG_LAPLACE = 1, PHG_HELMHOLTZ = 2, PHG_JAC_MASS = 3,
+    PHG_DIFF1 = 4, PHG_DIFF2 = 5, PHG_DIFF1_COS = 6, PHG_DIFF2_COS = 7
+};
 /* orthogonalisation of the Arnoldi step: modified Gram-Schmidt, classical
  * Gram-Schmidt with one re-orthogonalisation pass, or one classical pass
  * (single pass: block solver only) */
@@ -182,6 +189,48 @@ typedef struct {
 int phg_block_gmres(phg_matrix_t A, int nrhs, const double *d_B, double *d_X,
                     const phg_gmres_opts *opts, phg_block_info *info,
                     phg_stream_t s);
+
+/* ---- batched independent solves: nrhs (1..64) systems A x_q = b_q of one
+ *      matrix, each with its own Krylov space and stopping test, sharing the
+ *      matrix reads (one SpMM per Arnoldi step).  Replaces repeated
+ *      Solver::solve calls on one matrix: the 16 idt_.solve calls (8 distinct
+ *      right-hand sides) of SphereJacobian::calc2 (src/sjacobian.cpp:136-362)
+ *      and the implicit-step solves of ensemble members on one mesh
+ *      (examples/barvortex_impl.h:296-298). --------------------------------- */
+/* d_B, d_X: n_rows x nrhs FP64, row-major (entry (i, q) at i * nrhs + q).
+ * opts->restart <= 32, opts->ortho PHG_CGS2 or PHG_CGS1, opts->precision
+ * PHG_F64 or mixed (PHG_F32 / PHG_BF16 inner Krylov, FP64 refinement).
+ * A column's result does not depend on the other columns of the call.
+ * infos: nrhs host entries (may be NULL).  Returns PHG_ERR_NOCONV unless
+ * every column converged. */
+int phg_gmres_batch(phg_matrix_t A, int nrhs, const double *d_B, double *d_X,
+                    const phg_gmres_opts *opts, phg_gmres_info *infos,
+                    phg_stream_t s);
+
+/* ---- SphereJacobian (src/sjacobian.h:65-149) ---------------------------- */
+typedef struct phg_jacobian_s *phg_jacobian_t;
+/* SphereJacobian::SphereJacobian (src/sjacobian.cpp:415-446): assembles
+ * idt_ (PHG_JAC_MASS), diff1_, diff2_, diff1_cos_, diff2_cos_ and their
+ * boundary matrices on the GPU.  The mesh must outlive the object. */
+int phg_jacobian_create(phg_mesh_t m, phg_stream_t s, phg_jacobian_t *out);
+void phg_jacobian_destroy(phg_jacobian_t J);
+/* The matrices (owned by J, do not destroy): which = 0 idt_, 1..4 diff1_,
+ * diff2_, diff1_cos_, diff2_cos_, 5..8 their boundary (_rp_) matrices. */
+int phg_jacobian_matrix(phg_jacobian_t J, int which, phg_matrix_t *out);
+/* SphereJacobian::calc2 (src/sjacobian.cpp:348-362) for npairs (1..8)
+ * pairs at once: u_k = d_u + k * n_points, v_k = d_v + k * n_points (full
+ * mesh vectors), J(u_k, v_k) at the inner points to d_ans + k * n_inner.
+ * The 8 * npairs mass solves run as one phg_gmres_batch with opts (NULL:
+ * CGS1, restart 32, rtol 1e-13); infos (8 * npairs entries) may be NULL. */
+int phg_jacobian_calc2(phg_jacobian_t J, int npairs, const double *d_u,
+                       const double *d_v, double *d_ans,
+                       const phg_gmres_opts *opts, phg_gmres_info *infos,
+                       phg_stream_t s);
+/* SphereJacobian::calc2t (src/sjacobian.cpp:404-408): -calc2. */
+int phg_jacobian_calc2t(phg_jacobian_t J, int npairs, const double *d_u,
+                        const double *d_v, double *d_ans,
+                        const phg_gmres_opts *opts, phg_gmres_info *infos,
+                        phg_stream_t s);
 
 #ifdef __cplusplus
 }

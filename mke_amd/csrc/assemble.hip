@@ -122,14 +122,14 @@ elem_contrib(int S, int kind, double c_mass, double c_lap, int n_zones,
         xs[a] = coords[((size_t)p * n_zones + z) * 2 + 0];
         ys[a] = coords[((size_t)p * n_zones + z) * 2 + 1];
     }
-    const TrapGeom g = trap_geom(xs[0], xs[1], xs[2], ys[0], ys[1], ys[2]);
     ElemTables tab;
-    elem_tables(g, tab);
+    if (elem_needs_tables(kind))
+        elem_tables(trap_geom(xs[0], xs[1], xs[2], ys[0], ys[1], ys[2]), tab);
     P1Poly phi[3];
     p1_basis(xs, ys, phi);
     const P1Poly& phi_i = phi[slot_local[s]];
     for (int a = 0; a < 3; ++a)
-        contrib[(size_t)s * 3 + a] = elem_value(kind, tab, phi_i, phi[a], c_mass, c_lap);
+        contrib[(size_t)s * 3 + a] = elem_value(kind, tab, xs, ys, phi_i, phi[a], c_mass, c_lap);
 }
 
 __global__ void __launch_bounds__(256)
@@ -230,15 +230,16 @@ __global__ void bnd_assemble(int os, int kind, double c_mass, double c_lap, int 
             xs[a] = coords[((size_t)p * n_zones + z) * 2 + 0];
             ys[a] = coords[((size_t)p * n_zones + z) * 2 + 1];
         }
-        const TrapGeom g = trap_geom(xs[0], xs[1], xs[2], ys[0], ys[1], ys[2]);
         ElemTables tab;
-        elem_tables(g, tab);
+        if (elem_needs_tables(kind))
+            elem_tables(trap_geom(xs[0], xs[1], xs[2], ys[0], ys[1], ys[2]), tab);
         P1Poly phi[3];
         p1_basis(xs, ys, phi);
         const P1Poly& phi_j = phi[oslot_local[s]];
         for (int a = 0; a < 3; ++a) {
             const int q = opos[(size_t)s * 3 + a];
-            if (q >= 0) vals[q] += elem_value(kind, tab, phi[a], phi_j, c_mass, c_lap);
+            if (q >= 0)
+                vals[q] += elem_value(kind, tab, xs, ys, phi[a], phi_j, c_mass, c_lap);
         }
     }
 }

@@ -30,11 +30,14 @@ int gmres_solve(phg_matrix_s* A, const double* b, double* x, const phg_gmres_opt
 void csr_structure(const SphereMesh& m, std::vector<int>& rp, std::vector<int>& ci);
 int block_gmres(phg_matrix_s* A, int nrhs, const double* B, double* X, const phg_gmres_opts* opts,
                 phg_block_info* info, hipStream_t s);
+int gmres_batch_solve(phg_matrix_s* A, int K, const double* B, double* X,
+                      const phg_gmres_opts* opts, phg_gmres_info* infos, hipStream_t s);
 
 int ensure_mesh_device(phg_mesh_s* m)
 {
     if (m->d_coords.p || m->h.n_points == 0) return PHG_OK;
     PHG_HIP(m->d_inner.upload(m->h.inner.data(), m->h.inner.size()));
+    PHG_HIP(m->d_outer.upload(m->h.outer.data(), m->h.outer.size()));
     PHG_HIP(m->d_p2io.upload(m->h.p2io.data(), m->h.p2io.size()));
     PHG_HIP(m->d_isb.upload(m->h.is_boundary.data(), m->h.is_boundary.size()));
     PHG_HIP(m->d_tri.upload(m->h.tri.data(), m->h.tri.size()));
@@ -64,6 +67,7 @@ phg_matrix_s::~phg_matrix_s()
 {
     for (auto& w : work) w.reset();
     phg::block_destroy(bwork);
+    for (auto* b : batch) phg::batch_destroy(b);
 }
 
 using namespace phg;
@@ -242,7 +246,7 @@ int phg_matrix_from_csr(int n_rows, int n_cols, const int* row_ptr, const int* c
 int phg_assemble(phg_mesh_t m, int kind, double c_mass, double c_lap, phg_stream_t stream,
                  phg_matrix_t* out)
 {
-    if (!m || !out || kind < 0 || kind > 2) {
+    if (!m || !out || kind < 0 || kind > PHG_DIFF2_COS) {
         set_error("phg_assemble: bad argument");
         return PHG_ERR_ARG;
     }
@@ -251,7 +255,7 @@ int phg_assemble(phg_mesh_t m, int kind, double c_mass, double c_lap, phg_stream
 
 int phg_assemble_boundary(phg_mesh_t m, int kind, double c_mass, double c_lap, phg_matrix_t* out)
 {
-    if (!m || !out || kind < 0 || kind > 2) {
+    if (!m || !out || kind < 0 || kind > PHG_DIFF2_COS) {
         set_error("phg_assemble_boundary: bad argument");
         return PHG_ERR_ARG;
     }
@@ -375,6 +379,26 @@ int phg_block_gmres(phg_matrix_t A, int nrhs, const double* d_B, double* d_X,
         return PHG_ERR_ARG;
     }
     return block_gmres(A, nrhs, d_B, d_X, &o, info, (hipStream_t)s);
+}
+
+int phg_gmres_batch(phg_matrix_t A, int nrhs, const double* d_B, double* d_X,
+                    const phg_gmres_opts* opts, phg_gmres_info* infos, phg_stream_t s)
+{
+    if (!A || !d_B || !d_X) {
+        set_error("phg_gmres_batch: null argument");
+        return PHG_ERR_ARG;
+    }
+    phg_gmres_opts o;
+    if (opts) o = *opts;
+    else {
+        phg_gmres_default_opts(&o);
+        o.ortho = PHG_CGS2;
+    }
+    if (o.max_iters < 0 || !(o.rtol >= 0)) {
+        set_error("phg_gmres_batch: bad options");
+        return PHG_ERR_ARG;
+    }
+    return gmres_batch_solve(A, nrhs, d_B, d_X, &o, infos, (hipStream_t)s);
 }
 
 }  // extern "C"

@@ -317,15 +317,124 @@ PHG_HD double integrate_stiff(const ElemTables& t, const P1Poly& a, const P1Poly
     return l1 + l2;
 }
 
-enum ElemKind : int { ELEM_MASS = 0, ELEM_LAPLACE = 1, ELEM_HELMHOLTZ = 2 };
+// ------------------------------------------------------- 7-point triangle rule --
+// src/quadrature.c:130-171 (degree-5 rule, 16-digit barycentric tables).
+#define PHG_C7_A1 0.7974269853530880
+#define PHG_C7_B1 0.1012865073234560
+#define PHG_C7_A2 0.0597158717897742
+#define PHG_C7_B2 0.4701420641051129
+
+// Polynom::apply (polynom.cpp:63-89) of an (XD, YD) polynomial, koef[i*(YD+1)+j].
+template <int XD, int YD>
+PHG_HD double poly_apply(const double* q, double x, double y)
+{
+    double xi[XD + 1], yj[YD + 1];
+    double r = 1.0;
+    for (int i = 0; i <= XD; ++i) {
+        xi[i] = r;
+        r *= x;
+    }
+    r = 1.0;
+    for (int j = 0; j <= YD; ++j) {
+        yj[j] = r;
+        r *= y;
+    }
+    double s = 0.0;
+    for (int i = 0; i <= XD; ++i)
+        for (int j = 0; j <= YD; ++j) s += q[i * (YD + 1) + j] * xi[i] * yj[j];
+    return s;
+}
+
+// cubature7 of q(x, y), or of cos(x) q(x, y) when wcos (the integrand
+// callbacks of src/sjacobian.cpp:50-58).
+template <int XD, int YD>
+PHG_HD double cubature7_poly(const double* q, const double xs[3], const double ys[3],
+                             bool wcos)
+{
+    const double w7[7] = {0.225,
+                          0.1259391805448265,
+                          0.1259391805448265,
+                          0.1259391805448265,
+                          0.1323941527885112,
+                          0.1323941527885112,
+                          0.1323941527885112};
+    const double c7[7][3] = {{1. / 3., 1. / 3., 1. / 3.},
+                             {PHG_C7_A1, PHG_C7_B1, PHG_C7_B1},
+                             {PHG_C7_B1, PHG_C7_A1, PHG_C7_B1},
+                             {PHG_C7_B1, PHG_C7_B1, PHG_C7_A1},
+                             {PHG_C7_A2, PHG_C7_B2, PHG_C7_B2},
+                             {PHG_C7_B2, PHG_C7_A2, PHG_C7_B2},
+                             {PHG_C7_B2, PHG_C7_B2, PHG_C7_A2}};
+    double s = 0.0;
+    for (int i = 0; i < 7; ++i) {
+        const double u = c7[i][0] * xs[0] + c7[i][1] * xs[1] + c7[i][2] * xs[2];
+        const double v = c7[i][0] * ys[0] + c7[i][1] * ys[1] + c7[i][2] * ys[2];
+        double f = poly_apply<XD, YD>(q, u, v);
+        if (wcos) f = fem_cos(u) * f;
+        s += w7[i] * f;
+    }
+    s *= 0.5 * fabs(((xs[0] - xs[2]) * (ys[1] - ys[2]) - (ys[0] - ys[2]) * (xs[1] - xs[2])));
+    return s;
+}
+
+enum ElemKind : int {
+    ELEM_MASS = 0,
+    ELEM_LAPLACE = 1,
+    ELEM_HELMHOLTZ = 2,
+    // SphereJacobian matrices (src/sjacobian.cpp:60-134), 7-point rule
+    ELEM_JAC_MASS = 3,   // id_cb: cos x phi_i phi_j
+    ELEM_DIFF1 = 4,      // diff_1_rp: d_x phi_j phi_i
+    ELEM_DIFF2 = 5,      // diff_2_rp: d_y phi_j phi_i
+    ELEM_DIFF1_COS = 6,  // diff_1_cos_rp: cos x d_x phi_j phi_i
+    ELEM_DIFF2_COS = 7,  // diff_2_cos_rp: cos x d_y phi_j phi_i
+    ELEM_NKINDS = 8
+};
+
+// Kinds integrated by integrate1 over trapezoids need the GK15 tables of
+// elem_tables; the Jacobian kinds use only the vertex coordinates.
+PHG_HD bool elem_needs_tables(int kind) { return kind < ELEM_JAC_MASS; }
+
+PHG_HD double elem_cubature(int kind, const double xs[3], const double ys[3],
+                            const P1Poly& phi_i, const P1Poly& phi_j)
+{
+    if (kind == ELEM_JAC_MASS) {
+        double q[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // phi_i * phi_j, (2, 2)
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+                for (int k = 0; k < 2; ++k)
+                    for (int m = 0; m < 2; ++m)
+                        q[(i + k) * 3 + (j + m)] += phi_i.k[i * 2 + j] * phi_j.k[k * 2 + m];
+        return cubature7_poly<2, 2>(q, xs, ys, true);
+    }
+    const bool wcos = kind >= ELEM_DIFF1_COS;
+    double q[6] = {0, 0, 0, 0, 0, 0};
+    double v = 1;  // the commented-out weight u[i] of the callbacks
+    if (kind == ELEM_DIFF1 || kind == ELEM_DIFF1_COS) {
+        // diff(phi_j, 0) = Polynom(0, 1) [k10, k11]; times phi_i -> (1, 2)
+        const double d[2] = {1.0 * phi_j.k[2], 1.0 * phi_j.k[3]};
+        for (int j = 0; j < 2; ++j)
+            for (int k = 0; k < 2; ++k)
+                for (int m = 0; m < 2; ++m) q[k * 3 + (j + m)] += d[j] * phi_i.k[k * 2 + m];
+        return v * cubature7_poly<1, 2>(q, xs, ys, wcos);
+    }
+    // diff(phi_j, 1) = Polynom(1, 0) [k01, k11]; times phi_i -> (2, 1)
+    const double d[2] = {1.0 * phi_j.k[1], 1.0 * phi_j.k[3]};
+    for (int i = 0; i < 2; ++i)
+        for (int k = 0; k < 2; ++k)
+            for (int m = 0; m < 2; ++m) q[(i + k) * 2 + m] += d[i] * phi_i.k[k * 2 + m];
+    return v * cubature7_poly<2, 1>(q, xs, ys, wcos);
+}
 
 // Value of one (phi_i, phi_j) pair over one triangle, as the reference's
-// callbacks compute it: id_cb, slaplace_integrate_cb, bv integrate_cb.
-PHG_HD double elem_value(int kind, const ElemTables& t, const P1Poly& phi_i,
-                         const P1Poly& phi_j, double c_mass, double c_lap)
+// callbacks compute it: id_cb, slaplace_integrate_cb, bv integrate_cb and the
+// SphereJacobian callbacks.  t is read only when elem_needs_tables(kind).
+PHG_HD double elem_value(int kind, const ElemTables& t, const double xs[3],
+                         const double ys[3], const P1Poly& phi_i, const P1Poly& phi_j,
+                         double c_mass, double c_lap)
 {
     if (kind == ELEM_MASS) return integrate_mass(t, phi_i, phi_j);
     if (kind == ELEM_LAPLACE) return integrate_stiff(t, phi_j, phi_i);
+    if (kind >= ELEM_JAC_MASS) return elem_cubature(kind, xs, ys, phi_i, phi_j);
     double pt1 = integrate_mass(t, phi_i, phi_j);
     pt1 *= c_mass;
     double pt2 = integrate_stiff(t, phi_j, phi_i);

@@ -1,0 +1,766 @@
+// Batched restarted GMRES(m): K <= 64 independent systems A x_q = b_q that
+// share one matrix, on gfx950.
+//
+// Each column keeps its own Krylov space, Hessenberg matrix, Givens
+// rotations and stopping test, exactly as K separate solves would (no block
+// Krylov coupling), while every matrix read of an Arnoldi step serves all K
+// columns (one SpMM instead of K SpMVs).  It is the form the callers with
+// several right-hand sides per operator need: the eight mass solves of one
+// SphereJacobian::calc2 (src/sjacobian.cpp:136-362: 16 idt_.solve calls with
+// 8 distinct right-hand sides) and the implicit-step solves of the ensemble
+// members that share one mesh level.
+//
+// Layout: vectors are n x KP row-major (entry (i, q) at i * KP + q), KP = K
+// rounded up to a power of two; a thread owns one (row, column) pair, so a
+// wavefront covers 64 / KP consecutive rows and the gathers of one ELL entry
+// read KP consecutive words.  The caller's B and X are n x K, row-major.
+//
+// Orthogonalisation: classical Gram-Schmidt with one re-orthogonalisation
+// pass (CGS2), or a single pass (CGS1) for well-conditioned operators such
+// as the mass matrix.  Precision: FP64 throughout, or the mixed form (FP32
+// basis, FP32/BF16 matrix values, FP64 dots, Hessenberg, Givens, update and
+// a true FP64 residual at every restart, i.e. iterative refinement).
+//
+// Reductions are deterministic and do not depend on the column's position
+// or on the other columns: a column's result is bit-identical whatever else
+// is in the batch, provided KP is the same.  A restart cycle is captured in
+// one hipGraph and replayed; the host reads a K-entry summary per cycle.
+#include <cmath>
+#include <cstring>
+
+#include "ell.h"
+#include "phg_internal.h"
+
+namespace phg {
+
+constexpr int BB_BS = 256;
+constexpr int BB_MAXM = 32;
+constexpr int BB_MAXK = 64;
+constexpr int BB_RPT = 8;  // rows per thread
+constexpr int BB_LB = 4;   // basis vectors per load batch
+
+struct BCol {
+    double H[(BB_MAXM + 1) * BB_MAXM];  // unrotated Hessenberg, H[i * BB_MAXM + j]
+    double R[BB_MAXM * BB_MAXM];        // rotated, upper triangular
+    double cs[BB_MAXM], sn[BB_MAXM], g[BB_MAXM], carry[BB_MAXM + 1], y[BB_MAXM];
+    double hc[BB_MAXM + 1];             // coefficients of the current CGS pass
+    double beta, bnorm, rel, hn;
+    int k, active, converged, iters, iters0, cycles, pad0, pad1;
+};
+
+struct BSum {
+    double rel;
+    int active, converged, iters, cycles;
+};
+
+struct BatchWork {
+    int n = 0, m = 0, kp = 0, G = 0, prec = 0;
+    DevBuf<char> V, W;    // (m+1) x n x kp and 2 x n x kp of the Krylov type
+    DevBuf<double> part;  // (m+2) x kp x G block partials
+    DevBuf<BCol> cols;    // kp
+    DevBuf<BSum> sum;     // kp
+    BSum* host = nullptr;
+    hipStream_t cap = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void* key[4] = {nullptr, nullptr, nullptr, nullptr};
+    int key_m = 0, key_ortho = -1, key_k = 0, key_maxit = 0;
+    double key_rtol = 0.0, key_inner = 0.0;
+    ~BatchWork()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (cap) (void)hipStreamDestroy(cap);
+        if (host) (void)hipHostFree(host);
+    }
+};
+
+void batch_destroy(BatchWork* w) { delete w; }
+
+namespace {
+
+__device__ __forceinline__ double wave_sum64(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum over the lanes of a wavefront that hold the same column (lane % KP).
+__device__ __forceinline__ double col_wave_sum(double v, int KP)
+{
+    for (int off = KP; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Fixed-order sum of np partials by one wavefront; every lane returns it.
+__device__ __forceinline__ double wave_sum_partials(const double* __restrict__ p, int np)
+{
+    const int lane = threadIdx.x & 63;
+    double s = 0.0;
+    int b = lane;
+    for (; b + 64 * 7 < np; b += 64 * 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = p[b + 64 * u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; b < np; b += 64) s += p[b];
+    return wave_sum64(s);
+}
+
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+    const unsigned long long u = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, lane);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+template <typename T>
+__device__ __forceinline__ double dd(T v) { return (double)v; }
+
+// Row of A times column q of X (n x KP), source scaled by s: FP64 without
+// fused multiply-add, or FP32 with fma over FP32/BF16 values.
+template <typename VT, typename MT>
+struct BRow;
+template <>
+struct BRow<double, double> {
+    __device__ __forceinline__ static double run(int n, int width, int row,
+                                                 const int* __restrict__ col,
+                                                 const double* __restrict__ val,
+                                                 const double* __restrict__ X, int KP, int q,
+                                                 double s)
+    {
+        int c[ELL_MAXW];
+        double v[ELL_MAXW], xv[ELL_MAXW];
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k) {
+            const size_t o = (size_t)k * n + row;
+            c[k] = k < width ? col[o] : 0;
+            v[k] = k < width ? val[o] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k) xv[k] = k < width ? X[(size_t)c[k] * KP + q] : 0.0;
+        double r = 0.0;
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k)
+            if (k < width) r += v[k] * (xv[k] * s);
+        return r;
+    }
+};
+template <typename MT>
+struct BRow<float, MT> {
+    __device__ __forceinline__ static float run(int n, int width, int row,
+                                                const int* __restrict__ col,
+                                                const MT* __restrict__ val,
+                                                const float* __restrict__ X, int KP, int q,
+                                                float s)
+    {
+        int c[ELL_MAXW];
+        MT v[ELL_MAXW];
+        float xv[ELL_MAXW];
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k) {
+            const size_t o = (size_t)k * n + row;
+            c[k] = k < width ? col[o] : 0;
+            v[k] = k < width ? val[o] : MT(0);
+        }
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k) xv[k] = k < width ? X[(size_t)c[k] * KP + q] : 0.0f;
+        float r = 0.0f;
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k)
+            if (k < width) r = fmaf(ell_val_f32(v[k]), xv[k] * s, r);
+        return r;
+    }
+};
+
+// Block partials of <V_i, w> for i < nd (i == own_i taken from `own`), or
+// of ||w||^2 when nd == 0, for this thread's rows; written to
+// part[((out0 + i) * KP + q) * G + block].  All threads of the block call it.
+template <typename VT>
+__device__ __forceinline__ void bdots(int nd, int own_i, const VT* own, const VT* __restrict__ V,
+                                      size_t bs, int n, int KP, int q, int base, int RB,
+                                      bool act, const VT* w, double* __restrict__ part, int G,
+                                      int out0)
+{
+    __shared__ double sh[BB_BS / 64][BB_LB][BB_MAXK];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nout = nd == 0 ? 1 : nd;
+    for (int i0 = 0; i0 < nout; i0 += BB_LB) {
+        double acc[BB_LB];
+        if (nd == 0) {
+            double a = 0.0;
+#pragma unroll
+            for (int t = 0; t < BB_RPT; ++t) {
+                const int row = base + t * RB;
+                if (act && row < n) a += dd(w[t]) * dd(w[t]);
+            }
+            acc[0] = a;
+#pragma unroll
+            for (int u = 1; u < BB_LB; ++u) acc[u] = 0.0;
+        } else {
+            VT vb[BB_LB][BB_RPT];
+#pragma unroll
+            for (int u = 0; u < BB_LB; ++u) {
+                const int i = i0 + u;
+#pragma unroll
+                for (int t = 0; t < BB_RPT; ++t) {
+                    const int row = base + t * RB;
+                    vb[u][t] = (act && i < nd && i != own_i && row < n)
+                                   ? V[(size_t)i * bs + (size_t)row * KP + q] : VT(0);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < BB_LB; ++u) {
+                const int i = i0 + u;
+                double a = 0.0;
+#pragma unroll
+                for (int t = 0; t < BB_RPT; ++t) {
+                    const int row = base + t * RB;
+                    if (act && i < nd && row < n) a += dd(w[t]) * dd(i == own_i ? own[t] : vb[u][t]);
+                }
+                acc[u] = a;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < BB_LB; ++u) {
+            const double s = col_wave_sum(acc[u], KP);
+            if (lane < KP) sh[wid][u][lane] = s;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < BB_LB * KP; e += BB_BS) {
+            const int u = e / KP, qq = e % KP;
+            if (i0 + u < nout) {
+                double s = 0.0;
+#pragma unroll
+                for (int ww = 0; ww < BB_BS / 64; ++ww) s += sh[ww][u][qq];
+                part[((size_t)(out0 + i0 + u) * KP + qq) * G + blockIdx.x] = s;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- kernels --
+// Arnoldi step j: v_j = W_src / h (stored), W_dst = A v_j, partial dots of
+// W_dst with v_0..v_j (first CGS pass).
+template <typename VT, typename MT>
+__global__ void __launch_bounds__(BB_BS)
+bk_step_spmm(int n, int KP, int K, int width, const int* __restrict__ col,
+             const MT* __restrict__ val, const VT* __restrict__ Wsrc, VT* __restrict__ V,
+             size_t bs, VT* __restrict__ Wdst, int j, double* __restrict__ part, int G,
+             const BCol* __restrict__ cols)
+{
+    const int q = threadIdx.x & (KP - 1);
+    const int RB = BB_BS / KP;
+    const int base = blockIdx.x * RB * BB_RPT + (int)threadIdx.x / KP;
+    const bool act = q < K && cols[q].active;
+    double h = 1.0;
+    if (act) h = j == 0 ? cols[q].beta : cols[q].hn;
+    const VT inv = (VT)(1.0 / h);
+    VT* Vj = V + (size_t)j * bs;
+    VT w[BB_RPT], vj[BB_RPT];
+#pragma unroll
+    for (int t = 0; t < BB_RPT; ++t) {
+        const int row = base + t * RB;
+        w[t] = VT(0);
+        vj[t] = VT(0);
+        if (act && row < n) {
+            const size_t o = (size_t)row * KP + q;
+            vj[t] = Wsrc[o] * inv;
+            Vj[o] = vj[t];
+            w[t] = BRow<VT, MT>::run(n, width, row, col, val, Wsrc, KP, q, inv);
+            Wdst[o] = w[t];
+        }
+    }
+    bdots<VT>(j + 1, j, vj, V, bs, n, KP, q, base, RB, act, w, part, G, 0);
+}
+
+// Sums the partials of the nd dots of a pass, one wavefront per (i, column):
+// pass 0 sets H(i, j), pass 1 adds the re-orthogonalisation correction.
+__global__ void __launch_bounds__(BB_BS)
+bk_reduce(int KP, int K, int nd, int j, int pass, const double* __restrict__ part, int G,
+          BCol* __restrict__ cols)
+{
+    const int wv = blockIdx.x * (BB_BS / 64) + ((int)threadIdx.x >> 6);
+    if (wv >= nd * KP) return;
+    const int i = wv / KP, q = wv % KP;
+    if (q >= K || !cols[q].active) return;
+    const double s = wave_sum_partials(part + ((size_t)i * KP + q) * G, G);
+    if ((threadIdx.x & 63) == 0) {
+        cols[q].hc[i] = s;
+        if (pass == 0) cols[q].H[i * BB_MAXM + j] = s;
+        else cols[q].H[i * BB_MAXM + j] += s;
+    }
+}
+
+// w -= sum_{i<=j} hc_i v_i, then the partial dots with v_0..v_j (next pass)
+// or the partial of ||w||^2 (final pass).
+template <typename VT>
+__global__ void __launch_bounds__(BB_BS)
+bk_update(int n, int KP, int K, VT* __restrict__ W, const VT* __restrict__ V, size_t bs, int j,
+          int final_pass, double* __restrict__ part, int G, const BCol* __restrict__ cols)
+{
+    const int q = threadIdx.x & (KP - 1);
+    const int RB = BB_BS / KP;
+    const int base = blockIdx.x * RB * BB_RPT + (int)threadIdx.x / KP;
+    const bool act = q < K && cols[q].active;
+    VT w[BB_RPT];
+#pragma unroll
+    for (int t = 0; t < BB_RPT; ++t) {
+        const int row = base + t * RB;
+        w[t] = (act && row < n) ? W[(size_t)row * KP + q] : VT(0);
+    }
+    for (int i0 = 0; i0 <= j; i0 += BB_LB) {
+        VT vb[BB_LB][BB_RPT];
+        VT hv[BB_LB];
+#pragma unroll
+        for (int u = 0; u < BB_LB; ++u) {
+            const int i = i0 + u;
+            hv[u] = (act && i <= j) ? (VT)cols[q].hc[i] : VT(0);
+#pragma unroll
+            for (int t = 0; t < BB_RPT; ++t) {
+                const int row = base + t * RB;
+                vb[u][t] = (act && i <= j && row < n) ? V[(size_t)i * bs + (size_t)row * KP + q]
+                                                      : VT(0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < BB_LB; ++u)
+#pragma unroll
+            for (int t = 0; t < BB_RPT; ++t) w[t] = w[t] - hv[u] * vb[u][t];
+    }
+#pragma unroll
+    for (int t = 0; t < BB_RPT; ++t) {
+        const int row = base + t * RB;
+        if (act && row < n) W[(size_t)row * KP + q] = w[t];
+    }
+    if (final_pass) bdots<VT>(0, -1, nullptr, V, bs, n, KP, q, base, RB, act, w, part, G, j + 1);
+    else bdots<VT>(j + 1, -1, nullptr, V, bs, n, KP, q, base, RB, act, w, part, G, 0);
+}
+
+// Givens rotation of step j for each column (one wavefront per column) from
+// H(0..j, j) and the norm partials; decides whether the column goes on.
+__global__ void __launch_bounds__(BB_BS)
+bk_givens(int KP, int K, int j, int m, const double* __restrict__ part, int G,
+          BCol* __restrict__ cols, double rtol, double inner_rtol, int max_iters)
+{
+    const int q = blockIdx.x * (BB_BS / 64) + ((int)threadIdx.x >> 6);
+    if (q >= K) return;
+    BCol* c = cols + q;
+    if (!c->active) return;
+    const int lane = threadIdx.x & 63;
+    const double s = wave_sum_partials(part + ((size_t)(j + 1) * KP + q) * G, G);
+    const double hreg = lane <= j ? c->H[lane * BB_MAXM + j] : 0.0;
+    const double creg = lane < j ? c->cs[lane] : 0.0;
+    const double sreg = lane < j ? c->sn[lane] : 0.0;
+    const double hn = sqrt(s);
+    double cur = readlane_d(hreg, 0);
+    for (int i = 0; i < j; ++i) {
+        const double cc = readlane_d(creg, i), sn = readlane_d(sreg, i);
+        const double nxt = readlane_d(hreg, i + 1);
+        const double t = cc * cur + sn * nxt;
+        cur = -sn * cur + cc * nxt;
+        if (lane == 0) c->R[i * BB_MAXM + j] = t;
+    }
+    const double r = hypot(cur, hn);
+    const double cj = cur / r, sj = hn / r;
+    const double carry = c->carry[j];
+    const double gj = cj * carry, cn = -sj * carry;
+    const int iters = c->iters0 + j + 1;
+    const double stop = fmax(rtol * c->bnorm, inner_rtol * c->beta);
+    const bool cont = !(fabs(cn) <= stop || !(hn > 0.0) || iters >= max_iters || j + 1 == m);
+    if (lane == 0) {
+        c->R[j * BB_MAXM + j] = r;
+        c->H[(j + 1) * BB_MAXM + j] = hn;
+        c->hn = hn;
+        c->cs[j] = cj;
+        c->sn[j] = sj;
+        c->g[j] = gj;
+        c->carry[j + 1] = cn;
+        c->k = j + 1;
+        c->iters = iters;
+        if (!cont) c->active = 0;
+    }
+}
+
+// y = R^{-1} g for each column that ran this cycle (one wavefront each:
+// R staged in LDS by all lanes, back substitution by lane 0).
+__global__ void __launch_bounds__(BB_BS)
+bk_solve_y(int K, BCol* __restrict__ cols)
+{
+    __shared__ double Rs[BB_BS / 64][BB_MAXM * BB_MAXM];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int q = blockIdx.x * (BB_BS / 64) + wid;
+    if (q >= K) return;
+    BCol* c = cols + q;
+    const int k = c->k;
+    if (c->converged || k == 0) return;
+    double* R = Rs[wid];
+    for (int e = lane; e < k * k; e += 64) R[e] = c->R[(e / k) * BB_MAXM + (e % k)];
+    const double gl = lane < k ? c->g[lane] : 0.0;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    double y[BB_MAXM];
+    for (int i = k - 1; i >= 0; --i) {
+        double s = readlane_d(gl, i);
+        for (int l = i + 1; l < k; ++l) s -= R[i * k + l] * y[l];
+        y[i] = s / R[i * k + i];
+    }
+    if (lane == 0)
+        for (int i = 0; i < k; ++i) c->y[i] = y[i];
+}
+
+// X += V y per column (FP64 accumulation); X is n x K (the caller's layout).
+template <typename VT>
+__global__ void __launch_bounds__(BB_BS)
+bk_xupdate(int n, int KP, int K, const VT* __restrict__ V, size_t bs, double* __restrict__ X,
+           const BCol* __restrict__ cols)
+{
+    const int q = threadIdx.x & (KP - 1);
+    const int RB = BB_BS / KP;
+    const int base = blockIdx.x * RB * BB_RPT + (int)threadIdx.x / KP;
+    if (q >= K) return;
+    const BCol* c = cols + q;
+    const int k = c->k;
+    if (c->converged || k == 0) return;
+    for (int t = 0; t < BB_RPT; ++t) {
+        const int row = base + t * RB;
+        if (row >= n) break;
+        double s = 0.0;
+        for (int i = 0; i < k; ++i) s += dd(V[(size_t)i * bs + (size_t)row * KP + q]) * c->y[i];
+        X[(size_t)row * K + q] += s;
+    }
+}
+
+// r = b - A x (FP64), W0 = r in the Krylov type, partial ||r||^2 per column.
+template <typename VT>
+__global__ void __launch_bounds__(BB_BS)
+bk_residual(int n, int KP, int K, int width, const int* __restrict__ col,
+            const double* __restrict__ val, const double* __restrict__ B,
+            const double* __restrict__ X, VT* __restrict__ W0, double* __restrict__ part, int G,
+            const BCol* __restrict__ cols)
+{
+    const int q = threadIdx.x & (KP - 1);
+    const int RB = BB_BS / KP;
+    const int base = blockIdx.x * RB * BB_RPT + (int)threadIdx.x / KP;
+    const bool act = q < K && !cols[q].converged;
+    VT w[BB_RPT];
+#pragma unroll
+    for (int t = 0; t < BB_RPT; ++t) {
+        const int row = base + t * RB;
+        w[t] = VT(0);
+        if (act && row < n) {
+            int c[ELL_MAXW];
+            double v[ELL_MAXW], xv[ELL_MAXW];
+#pragma unroll
+            for (int k = 0; k < ELL_MAXW; ++k) {
+                const size_t o = (size_t)k * n + row;
+                c[k] = k < width ? col[o] : 0;
+                v[k] = k < width ? val[o] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < ELL_MAXW; ++k) xv[k] = k < width ? X[(size_t)c[k] * K + q] : 0.0;
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < ELL_MAXW; ++k)
+                if (k < width) s += v[k] * xv[k];
+            const double r = B[(size_t)row * K + q] - s;
+            w[t] = (VT)r;
+            W0[(size_t)row * KP + q] = w[t];
+        }
+    }
+    bdots<VT>(0, -1, nullptr, nullptr, 0, n, KP, q, base, RB, act, w, part, G, 0);
+}
+
+// ||b_q||^2 partials: the same reduction with B read as the vector.
+__global__ void __launch_bounds__(BB_BS)
+bk_sumsq(int n, int KP, int K, const double* __restrict__ B, double* __restrict__ part, int G)
+{
+    const int q = threadIdx.x & (KP - 1);
+    const int RB = BB_BS / KP;
+    const int base = blockIdx.x * RB * BB_RPT + (int)threadIdx.x / KP;
+    const bool act = q < K;
+    double w[BB_RPT];
+#pragma unroll
+    for (int t = 0; t < BB_RPT; ++t) {
+        const int row = base + t * RB;
+        w[t] = (act && row < n) ? B[(size_t)row * K + q] : 0.0;
+    }
+    bdots<double>(0, -1, nullptr, nullptr, 0, n, KP, q, base, RB, act, w, part, G, 0);
+}
+
+__global__ void bk_set_bnorm(int KP, int K, const double* __restrict__ part, int G,
+                             BCol* __restrict__ cols)
+{
+    const int q = blockIdx.x * (BB_BS / 64) + ((int)threadIdx.x >> 6);
+    if (q >= K) return;
+    const double s = wave_sum_partials(part + (size_t)q * G, G);
+    if ((threadIdx.x & 63) == 0) {
+        BCol* c = cols + q;
+        const double bn = sqrt(s);
+        c->bnorm = bn > 0.0 ? bn : 1.0;
+        c->iters = 0;
+        c->cycles = 0;
+        c->converged = 0;
+        c->active = 0;
+        c->k = 0;
+    }
+}
+
+// Start of a restart cycle from the residual partials; writes the summary.
+__global__ void bk_cycle_start(int KP, int K, const double* __restrict__ part, int G,
+                               BCol* __restrict__ cols, BSum* __restrict__ sum, double rtol,
+                               int max_iters)
+{
+    const int q = blockIdx.x * (BB_BS / 64) + ((int)threadIdx.x >> 6);
+    if (q >= K) return;
+    BCol* c = cols + q;
+    const bool done = c->converged;
+    const double s = done ? 0.0 : wave_sum_partials(part + (size_t)q * G, G);
+    if ((threadIdx.x & 63) != 0) return;
+    if (!done) {
+        const double beta = sqrt(s);
+        c->beta = beta;
+        c->rel = beta / c->bnorm;
+        c->k = 0;
+        c->iters0 = c->iters;
+        if (c->rel <= rtol || beta == 0.0) {
+            c->converged = 1;
+            c->active = 0;
+        } else if (c->iters >= max_iters) {
+            c->active = 0;
+        } else {
+            c->active = 1;
+            c->carry[0] = beta;
+            c->cycles += 1;
+        }
+    }
+    BSum o;
+    o.rel = c->rel;
+    o.active = c->active;
+    o.converged = c->converged;
+    o.iters = c->iters;
+    o.cycles = c->cycles;
+    sum[q] = o;
+}
+
+__global__ void bk_fill0(size_t n, double* __restrict__ x)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.0;
+}
+
+// ---------------------------------------------------------------- driver --
+int next_pow2(int k)
+{
+    int p = 1;
+    while (p < k) p <<= 1;
+    return p;
+}
+
+int prepare_batch(phg_matrix_s* A, int slot, int prec, int m, int kp, BatchWork** out)
+{
+    BatchWork*& w = A->batch[slot];
+    const int n = A->n_rows;
+    if (!w || w->m < m || w->kp != kp || w->prec != prec) {
+        batch_destroy(w);
+        w = nullptr;
+        std::unique_ptr<BatchWork> b(new BatchWork());
+        b->n = n;
+        b->m = m;
+        b->kp = kp;
+        b->prec = prec;
+        const int rows_per_block = (BB_BS / kp) * BB_RPT;
+        b->G = std::max(1, (n + rows_per_block - 1) / rows_per_block);
+        const size_t es = prec == PHG_F64 ? 8 : 4;
+        const size_t bs = (size_t)n * kp;
+        PHG_HIP(b->V.alloc((size_t)(m + 1) * bs * es));
+        PHG_HIP(b->W.alloc(2 * bs * es));
+        PHG_HIP(b->part.alloc((size_t)(m + 2) * kp * b->G));
+        PHG_HIP(b->cols.alloc(kp));
+        PHG_HIP(b->sum.alloc(kp));
+        PHG_HIP(hipHostMalloc((void**)&b->host, sizeof(BSum) * kp, hipHostMallocDefault));
+        w = b.release();
+    }
+    *out = w;
+    return PHG_OK;
+}
+
+template <typename VT, typename MT>
+void enqueue_batch_cycle(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K,
+                         const double* B, double* X, int m, int passes, double rtol,
+                         double inner_rtol, int max_iters, hipStream_t s)
+{
+    const int n = w->n, KP = w->kp, G = w->G;
+    const size_t bs = (size_t)n * KP;
+    VT* V = (VT*)w->V.p;
+    VT* W0 = (VT*)w->W.p;
+    VT* W1 = W0 + bs;
+    double* part = w->part.p;
+    BCol* cols = w->cols.p;
+    const dim3 grid(G), block(BB_BS);
+    const int cb = (K + 3) / 4;  // blocks of one wavefront per column
+    for (int j = 0; j < m; ++j) {
+        VT* src = (j & 1) ? W1 : W0;
+        VT* dst = (j & 1) ? W0 : W1;
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_step_spmm<VT, MT>), grid, block, 0, s, n, KP, K,
+                           E.width, E.col.p, E.val.p, src, V, bs, dst, j, part, G, cols);
+        for (int p = 0; p < passes; ++p) {
+            const int nd = j + 1;
+            hipLaunchKernelGGL(bk_reduce, dim3((nd * KP + 3) / 4), block, 0, s, KP, K, nd, j, p,
+                               part, G, cols);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_update<VT>), grid, block, 0, s, n, KP, K, dst,
+                               V, bs, j, p == passes - 1 ? 1 : 0, part, G, cols);
+        }
+        hipLaunchKernelGGL(bk_givens, dim3(cb), block, 0, s, KP, K, j, m, part, G, cols, rtol,
+                           inner_rtol, max_iters);
+    }
+    hipLaunchKernelGGL(bk_solve_y, dim3(cb), block, 0, s, K, cols);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_xupdate<VT>), grid, block, 0, s, n, KP, K, V, bs, X,
+                       cols);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_residual<VT>), grid, block, 0, s, n, KP, K,
+                       A->ell64.width, A->ell64.col.p, A->ell64.val.p, B, X, W0, part, G, cols);
+    hipLaunchKernelGGL(bk_cycle_start, dim3(cb), block, 0, s, KP, K, part, G, cols, w->sum.p,
+                       rtol, max_iters);
+}
+
+template <typename VT, typename MT>
+int batch_run(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const double* B, double* X,
+              const phg_gmres_opts& o, phg_gmres_info* infos, hipStream_t s)
+{
+    const int n = w->n, KP = w->kp, G = w->G, m = o.restart;
+    const size_t bs = (size_t)n * KP;
+    VT* W0 = (VT*)w->W.p;
+    double* part = w->part.p;
+    BCol* cols = w->cols.p;
+    const dim3 grid(G), block(BB_BS);
+    const int cb = (K + 3) / 4;
+    const int passes = o.ortho == PHG_CGS1 ? 1 : 2;
+    const double inner = o.precision == PHG_F64 ? 0.0 : o.inner_rtol;
+    (void)bs;
+
+    if (!o.use_x0) {
+        const size_t tot = (size_t)n * K;
+        hipLaunchKernelGGL(bk_fill0, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, tot, X);
+    }
+    hipLaunchKernelGGL(bk_sumsq, grid, block, 0, s, n, KP, K, B, part, G);
+    hipLaunchKernelGGL(bk_set_bnorm, dim3(cb), block, 0, s, KP, K, part, G, cols);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_residual<VT>), grid, block, 0, s, n, KP, K,
+                       A->ell64.width, A->ell64.col.p, A->ell64.val.p, B, X, W0, part, G, cols);
+    hipLaunchKernelGGL(bk_cycle_start, dim3(cb), block, 0, s, KP, K, part, G, cols, w->sum.p,
+                       o.rtol, o.max_iters);
+    PHG_HIP(hipGetLastError());
+
+    if (o.graph) {
+        const void* key[4] = {B, X, E.val.p, A->ell64.val.p};
+        const bool same = w->exec && w->key_m == m && w->key_ortho == o.ortho && w->key_k == K &&
+                          w->key_rtol == o.rtol && w->key_inner == inner &&
+                          w->key_maxit == o.max_iters && !memcmp(key, w->key, sizeof key);
+        if (!same) {
+            if (w->exec) {
+                PHG_HIP(hipGraphExecDestroy(w->exec));
+                w->exec = nullptr;
+            }
+            if (!w->cap) PHG_HIP(hipStreamCreateWithFlags(&w->cap, hipStreamNonBlocking));
+            hipGraph_t g = nullptr;
+            PHG_HIP(hipStreamBeginCapture(w->cap, hipStreamCaptureModeThreadLocal));
+            enqueue_batch_cycle<VT, MT>(A, E, w, K, B, X, m, passes, o.rtol, inner, o.max_iters,
+                                        w->cap);
+            PHG_HIP(hipStreamEndCapture(w->cap, &g));
+            hipError_t e = hipGraphInstantiate(&w->exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            PHG_HIP(e);
+            memcpy(w->key, key, sizeof key);
+            w->key_m = m;
+            w->key_ortho = o.ortho;
+            w->key_k = K;
+            w->key_rtol = o.rtol;
+            w->key_inner = inner;
+            w->key_maxit = o.max_iters;
+        }
+    }
+    for (;;) {
+        PHG_HIP(hipMemcpyAsync(w->host, w->sum.p, sizeof(BSum) * K, hipMemcpyDeviceToHost, s));
+        PHG_HIP(hipStreamSynchronize(s));
+        bool any = false;
+        for (int q = 0; q < K; ++q) any = any || w->host[q].active;
+        if (!any) break;
+        if (o.graph) {
+            PHG_HIP(hipGraphLaunch(w->exec, s));
+        } else {
+            enqueue_batch_cycle<VT, MT>(A, E, w, K, B, X, m, passes, o.rtol, inner, o.max_iters,
+                                        s);
+            PHG_HIP(hipGetLastError());
+        }
+    }
+    bool all = true;
+    for (int q = 0; q < K; ++q) {
+        const BSum& h = w->host[q];
+        all = all && h.converged;
+        if (infos) {
+            infos[q].iterations = h.iters;
+            infos[q].restarts = h.cycles;
+            infos[q].refinements = o.precision == PHG_F64 ? 0 : h.cycles;
+            infos[q].rel_residual = h.rel;
+            infos[q].converged = h.converged;
+        }
+    }
+    return all ? PHG_OK : PHG_ERR_NOCONV;
+}
+
+}  // namespace
+
+int gmres_batch_solve(phg_matrix_s* A, int K, const double* B, double* X,
+                      const phg_gmres_opts* opts, phg_gmres_info* infos, hipStream_t s)
+{
+    phg_gmres_opts o = *opts;
+    if (K < 1 || K > BB_MAXK) {
+        set_error("gmres_batch: nrhs must be in 1..64");
+        return PHG_ERR_ARG;
+    }
+    if (o.restart < 1 || o.restart > BB_MAXM) {
+        set_error("gmres_batch: restart must be in 1..32");
+        return PHG_ERR_ARG;
+    }
+    if (o.ortho != PHG_CGS2 && o.ortho != PHG_CGS1) {
+        set_error("gmres_batch: ortho must be PHG_CGS2 or PHG_CGS1");
+        return PHG_ERR_ARG;
+    }
+    if (o.precision < PHG_F64 || o.precision > PHG_BF16) {
+        set_error("gmres_batch: bad precision");
+        return PHG_ERR_ARG;
+    }
+    if (A->n_rows != A->n_cols) {
+        set_error("gmres_batch: matrix is not square");
+        return PHG_ERR_ARG;
+    }
+    if (A->n_rows == 0) {
+        if (infos)
+            for (int q = 0; q < K; ++q) memset(&infos[q], 0, sizeof infos[q]), infos[q].converged = 1;
+        return PHG_OK;
+    }
+    int rc = ensure_ell64(A, s);
+    if (rc) return rc;
+    if (A->ell64.width > ELL_MAXW) {
+        set_error("gmres_batch: rows longer than 8 entries are not supported");
+        return PHG_ERR_ARG;
+    }
+    const int kp = next_pow2(K);
+    BatchWork* w = nullptr;
+    const int slot = o.precision == PHG_F64 ? 0 : 1;
+    rc = prepare_batch(A, slot, o.precision == PHG_F64 ? PHG_F64 : PHG_F32, o.restart, kp, &w);
+    if (rc) return rc;
+    if (o.precision == PHG_F64) return batch_run<double, double>(A, A->ell64, w, K, B, X, o, infos, s);
+    if (o.precision == PHG_F32) {
+        rc = ensure_ell32(A, s);
+        if (rc) return rc;
+        return batch_run<float, float>(A, A->ell32, w, K, B, X, o, infos, s);
+    }
+    rc = ensure_ell16(A, s);
+    if (rc) return rc;
+    return batch_run<float, uint16_t>(A, A->ell16, w, K, B, X, o, infos, s);
+}
+
+}  // namespace phg

@@ -67,8 +67,10 @@ struct Ell {
 
 struct GmresWork;  // gmres.hip
 struct BlockWork;  // block.hip
+struct BatchWork;  // gmres_batch.hip
 void destroy_gmres_work(GmresWork* w);
 void block_destroy(BlockWork* w);
+void batch_destroy(BatchWork* w);
 struct GmresWorkDeleter {
     void operator()(GmresWork* w) const { destroy_gmres_work(w); }
 };
@@ -78,7 +80,7 @@ struct GmresWorkDeleter {
 struct phg_mesh_s {
     phg::SphereMesh h;
     // device copies for gather/scatter and assembly (uploaded on first use)
-    phg::DevBuf<int> d_inner, d_p2io, d_isb;
+    phg::DevBuf<int> d_inner, d_outer, d_p2io, d_isb;
     phg::DevBuf<double> d_coords;
     phg::DevBuf<int> d_tri;
 };
@@ -94,6 +96,7 @@ struct phg_matrix_s {
     phg::Ell<uint16_t> ell16;      // bfloat16 bit patterns
     std::unique_ptr<phg::GmresWork, phg::GmresWorkDeleter> work[3];  // per precision
     phg::BlockWork* bwork = nullptr;  // block GMRES workspace
+    phg::BatchWork* batch[2] = {nullptr, nullptr};  // batched GMRES: FP64, mixed
     ~phg_matrix_s();
 };
 

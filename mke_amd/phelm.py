@@ -19,6 +19,8 @@ LIB_PATH = os.path.join(_HERE, "libphelm_hip.so")
 
 F64, F32, BF16 = 0, 1, 2
 MASS, LAPLACE, HELMHOLTZ = 0, 1, 2
+# SphereJacobian kinds (src/sjacobian.cpp:60-134)
+JAC_MASS, DIFF1, DIFF2, DIFF1_COS, DIFF2_COS = 3, 4, 5, 6, 7
 MGS, CGS2, CGS1 = 0, 1, 2
 OK, ERR_NOCONV = 0, 1
 
@@ -72,6 +74,12 @@ SIGNATURES = {
     "phg_gmres_default_opts": (None, [C.POINTER(GmresOpts)]),
     "phg_gmres": (_I, [_P, _P, _P, C.POINTER(GmresOpts), C.POINTER(GmresInfo), _P]),
     "phg_block_gmres": (_I, [_P, _I, _P, _P, C.POINTER(GmresOpts), C.POINTER(BlockInfo), _P]),
+    "phg_gmres_batch": (_I, [_P, _I, _P, _P, C.POINTER(GmresOpts), C.POINTER(GmresInfo), _P]),
+    "phg_jacobian_create": (_I, [_P, _P, C.POINTER(_P)]),
+    "phg_jacobian_destroy": (None, [_P]),
+    "phg_jacobian_matrix": (_I, [_P, _I, C.POINTER(_P)]),
+    "phg_jacobian_calc2": (_I, [_P, _I, _P, _P, _P, C.POINTER(GmresOpts), C.POINTER(GmresInfo), _P]),
+    "phg_jacobian_calc2t": (_I, [_P, _I, _P, _P, _P, C.POINTER(GmresOpts), C.POINTER(GmresInfo), _P]),
 }
 
 _lib = None
@@ -203,8 +211,9 @@ class Mesh:
 class Matrix:
     """A device CSR matrix (the storage behind Solver<T>)."""
 
-    def __init__(self, handle):
+    def __init__(self, handle, owned=True):
         self.h = handle
+        self.owned = owned
         r, c, nnz = C.c_int(), C.c_int(), C.c_long()
         check(lib().phg_matrix_info(self.h, C.byref(r), C.byref(c), C.byref(nnz)))
         self.n_rows, self.n_cols, self.nnz = r.value, c.value, nnz.value
@@ -291,9 +300,26 @@ class Matrix:
         return dict(iterations=info.iterations, cycles=info.cycles,
                     max_rel_residual=info.max_rel_residual, converged=bool(info.converged))
 
+    def solve_batch(self, X, B, restart=30, rtol=1e-10, max_iters=200000, ortho=CGS2,
+                    precision=F64, inner_rtol=1e-5, use_x0=False, graph=True, stream=None):
+        """K independent solves A x_q = b_q sharing the matrix (1 <= K <= 64):
+        X, B are n x K FP64 device tensors (row-major).  Returns one info
+        dict per column."""
+        K = int(B.shape[1]) if B.dim() == 2 else 1
+        o = GmresOpts()
+        lib().phg_gmres_default_opts(C.byref(o))
+        o.restart, o.max_iters, o.rtol, o.ortho = restart, max_iters, rtol, ortho
+        o.precision, o.inner_rtol, o.use_x0, o.graph = precision, inner_rtol, int(use_x0), int(graph)
+        infos = (GmresInfo * K)()
+        rc = lib().phg_gmres_batch(self.h, K, _ptr(B), _ptr(X), C.byref(o), infos, _stream(stream))
+        check(rc, "phg_gmres_batch")
+        return [dict(iterations=i.iterations, restarts=i.restarts, refinements=i.refinements,
+                     rel_residual=i.rel_residual, converged=bool(i.converged)) for i in infos]
+
     def close(self):
         if self.h:
-            lib().phg_matrix_destroy(self.h)
+            if self.owned:
+                lib().phg_matrix_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -330,6 +356,70 @@ class Solver:
 
     def solve(self, x, b, **kw):
         return self.matrix().solve(x, b, **kw)
+
+
+def _opts(restart, rtol, max_iters, ortho, precision, inner_rtol=1e-5, use_x0=False, graph=True):
+    o = GmresOpts()
+    lib().phg_gmres_default_opts(C.byref(o))
+    o.restart, o.max_iters, o.rtol, o.ortho = restart, max_iters, rtol, ortho
+    o.precision, o.inner_rtol, o.use_x0, o.graph = precision, inner_rtol, int(use_x0), int(graph)
+    return o
+
+
+class SphereJacobian:
+    """SphereJacobian (src/sjacobian.h:65-149): the nine cubature matrices
+    assembled on the GPU; calc2 evaluates up to 8 (u, v) pairs with one
+    batched mass solve of the 8 distinct right-hand sides per pair."""
+
+    MATRICES = ("idt_", "diff1_", "diff2_", "diff1_cos_", "diff2_cos_",
+                "diff1_rp_", "diff2_rp_", "diff1_cos_rp_", "diff2_cos_rp_")
+
+    def __init__(self, mesh, stream=None, restart=32, rtol=1e-13, ortho=CGS1,
+                 precision=F64):
+        self.m = mesh
+        self.h = C.c_void_p()
+        check(lib().phg_jacobian_create(mesh.h, _stream(stream), C.byref(self.h)),
+              "phg_jacobian_create")
+        self.opts = _opts(restart, rtol, 100000, ortho, precision)
+        for which, name in enumerate(self.MATRICES):
+            mh = C.c_void_p()
+            check(lib().phg_jacobian_matrix(self.h, which, C.byref(mh)))
+            setattr(self, name, Matrix(mh, owned=False))
+
+    def _run(self, fn, ans, u, v, stream):
+        npairs = 1 if u.dim() == 1 else int(u.shape[0])
+        infos = (GmresInfo * (8 * npairs))()
+        rc = fn(self.h, npairs, _ptr(u), _ptr(v), _ptr(ans), C.byref(self.opts), infos,
+                _stream(stream))
+        check(rc, "phg_jacobian_calc2")
+        return [dict(iterations=i.iterations, rel_residual=i.rel_residual,
+                     converged=bool(i.converged)) for i in infos]
+
+    def calc2(self, ans, u, v, stream=None):
+        """ans[k] = J(u[k], v[k]) at the inner points; u, v: (npairs, n_points)
+        or (n_points,) FP64 device tensors."""
+        return self._run(lib().phg_jacobian_calc2, ans, u, v, stream)
+
+    def calc2t(self, ans, u, v, stream=None):
+        return self._run(lib().phg_jacobian_calc2t, ans, u, v, stream)
+
+    def calc1(self, ans, u, v, bnd, p1, stream=None):
+        """calc2 then p2u with the boundary values bnd (src/sjacobian.cpp:448-453);
+        p1 is an inner-sized work tensor."""
+        infos = self.calc2(p1, u, v, stream)
+        self.m.p2u(p1, bnd, ans, stream)
+        return infos
+
+    def close(self):
+        if self.h:
+            lib().phg_jacobian_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def helmholtz_coeffs(tau=1.0, sigma=0.0, theta=1.0, mu=1e-2):

@@ -447,9 +447,14 @@ static double slaplace(const or_poly *a, const or_poly *b, const double xs[3],
     return l1 + l2;
 }
 
-static double jac_cos_func(double x, double y, void *d)  /* sjacobian.cpp */
+static double jac_cos_func(double x, double y, void *d)  /* sjacobian.cpp:50-53 */
 {
     return or_cos(x) * or_poly_apply((const or_poly *)d, x, y);
+}
+
+static double jac_func(double x, double y, void *d)      /* sjacobian.cpp:55-58 */
+{
+    return or_poly_apply((const or_poly *)d, x, y);
 }
 
 double or_element(int kind, const or_poly *phi_i, const or_poly *phi_j,
@@ -471,10 +476,23 @@ double or_element(int kind, const or_poly *phi_i, const or_poly *phi_j,
         pt2 *= c_lap;
         return pt1 + pt2;
     }
-    default:  /* SphereJacobian id_cb: sjacobian.cpp:60-74 */
+    case 3:   /* SphereJacobian id_cb: sjacobian.cpp:60-74 */
         or_poly_mul(phi_i, phi_j, &poly);
         return or_cubature7(xs[0], ys[0], xs[1], ys[1], xs[2], ys[2],
                             jac_cos_func, &poly);
+    default: {
+        /* diff_1_rp, diff_2_rp, diff_1_cos_rp, diff_2_cos_rp
+         * (sjacobian.cpp:76-134): poly = diff(phi_j, d) * phi_i, times
+         * v = 1, integrated by cubature7 with or without cos x */
+        or_poly dj;
+        const int d = (kind == 4 || kind == 6) ? 0 : 1;
+        const int wcos = (kind >= 6);
+        or_poly_diff(phi_j, d, &dj);
+        or_poly_mul(&dj, phi_i, &poly);
+        double v = 1;
+        return v * or_cubature7(xs[0], ys[0], xs[1], ys[1], xs[2], ys[2],
+                                wcos ? jac_cos_func : jac_func, &poly);
+    }
     }
 }
 

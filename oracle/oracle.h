@@ -80,7 +80,9 @@ double or_poly_apply(const or_poly *a, double x, double y);
  *       1 = stiffness (slaplace_integrate_cb, src/slaplace.cpp:171-183)
  *       2 = Helmholtz c_mass*mass + c_lap*stiffness per triangle
  *                                         (examples/barvortex_impl.h:57-80)
- *       3 = Jacobian mass, cubature7 (src/sjacobian.cpp:60-74)            */
+ *       3 = Jacobian mass, cubature7 (src/sjacobian.cpp:60-74)
+ *       4 = diff_1_rp, 5 = diff_2_rp, 6 = diff_1_cos_rp, 7 = diff_2_cos_rp
+ *           (src/sjacobian.cpp:76-134)                                    */
 double or_element(int kind, const or_poly *phi_i, const or_poly *phi_j,
                   const double xs[3], const double ys[3], double c_mass,
                   double c_lap);

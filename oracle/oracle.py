@@ -20,6 +20,8 @@ FX = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
 FXY = C.CFUNCTYPE(C.c_double, C.c_double, C.c_double, C.c_void_p)
 
 KIND_MASS, KIND_LAPLACE, KIND_HELMHOLTZ, KIND_JAC_MASS = 0, 1, 2, 3
+# SphereJacobian operators (src/sjacobian.cpp:76-134)
+KIND_DIFF1, KIND_DIFF2, KIND_DIFF1_COS, KIND_DIFF2_COS = 4, 5, 6, 7
 
 
 class OrPoly(C.Structure):

@@ -58,6 +58,26 @@ def test_assembly_bitwise_level7():
     assert np.array_equal(v, ov)
 
 
+JAC_KINDS = [P.JAC_MASS, P.DIFF1, P.DIFF2, P.DIFF1_COS, P.DIFF2_COS]
+
+
+@pytest.mark.parametrize("level", [1, 3, 5])
+@pytest.mark.parametrize("kind", JAC_KINDS)
+def test_jacobian_assembly_bitwise(level, kind):
+    print("jacobian assembly level=%d kind=%d" % (level, kind), flush=True)
+    pm, om = _mesh(level)
+    A = P.Matrix.assemble(pm, kind)
+    rp, ci, v = A.csr()
+    orp, oci, ov = oracle_matrix(level, kind)
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+    bad = np.nonzero(v != ov)[0]
+    assert len(bad) == 0, "first mismatch at %d: %r vs %r" % (bad[0], v[bad[0]], ov[bad[0]])
+    B = P.Matrix.assemble_boundary(pm, kind)
+    brp, bci, bv = B.csr()
+    orp, oci, ov = om.assemble_boundary(kind)
+    assert np.array_equal(brp, orp) and np.array_equal(bci, oci) and np.array_equal(bv, ov)
+
+
 @pytest.mark.parametrize("level", [2, 4])
 @pytest.mark.parametrize("kind,cm,cl", [(P.MASS, 1.0, 0.0), (P.LAPLACE, 1.0, 0.0),
                                         (P.HELMHOLTZ, CM, CL)])
