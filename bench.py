@@ -63,7 +63,8 @@ def time_solves(A, b, x, steps, warmup, stream, **kw):
 
 
 def kernel_avg_ms(fn, launches, stream):
-    """Average duration of one launch of fn, HIP events on its stream."""
+    """Average duration of one launch of fn: `launches` back-to-back launches
+    on `stream` between two HIP events on that stream."""
     import torch
     for _ in range(10):
         fn()
@@ -75,6 +76,18 @@ def kernel_avg_ms(fn, launches, stream):
     e.record(stream)
     torch.cuda.synchronize()
     return s.elapsed_time(e) / launches
+
+
+def traffic_from_profiles(kernel_file):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the factor a
+    plain copy of known size gives), or None if it was not recorded."""
+    path = os.path.join(ROOT, "profiles", kernel_file)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("traffic_bytes_per_launch")
 
 
 def cpu_baseline(rp, ci, v, b, budget_s):
@@ -102,6 +115,10 @@ def main():
                     help="seconds of oracle CPU solves for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--cold-spmv", action="store_true",
+                    help="also time the SpMV round-robin over 40 matrix copies (> 512 MB)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run a 256 MiB device copy (calibrates rocprof byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (single-GPU boxes); "
@@ -202,29 +219,38 @@ def main():
             (batched["solves_per_s"], batched["batch_ms"], binfo["iterations"],
              binfo["max_rel_residual"]))
 
-    # ---- roofline of the dominant kernel: the FP64 SpMV ----
+    # ---- roofline of the FP64 SpMV (Solver::mult_vector) ----
     y = torch.zeros_like(b)
     spmv_ms = kernel_avg_ms(lambda: A.mult_vector(y, b, stream=stream), 400, stream)
     bspmv = spmv_bytes(n, nnz, 8, 8)
     achieved = bspmv / (spmv_ms * 1e-3) / 1e9
-    # cold-cache variant: round-robin over 40 copies (> 512 MB working set)
-    copies = [P.Matrix.from_csr(n, n, rp, ci, v) for _ in range(40)]
-    xs = [torch.randn(n, dtype=torch.float64, device=dev) for _ in range(40)]
-    ys = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(40)]
-    for c, xx, yy in zip(copies, xs, ys):
-        c.mult_vector(yy, xx, stream=stream)
-    torch.cuda.synchronize()
-    it = [0]
+    log("spmv fp64: %.2f us per launch (%.0f GB/s)" % (spmv_ms * 1e3, achieved))
+    cold_ms = cold_gbs = None
+    if args.cold_spmv:
+        # cold-cache variant: round-robin over 40 copies (> 512 MB working set)
+        copies = [P.Matrix.from_csr(n, n, rp, ci, v) for _ in range(40)]
+        xs = [torch.randn(n, dtype=torch.float64, device=dev) for _ in range(40)]
+        ys = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(40)]
+        for c, xx, yy in zip(copies, xs, ys):  # builds every copy's ELL before timing
+            c.mult_vector(yy, xx, stream=stream)
+        torch.cuda.synchronize()
+        it = [0]
 
-    def cold():
-        k = it[0] % 40
-        it[0] += 1
-        copies[k].mult_vector(ys[k], xs[k], stream=stream)
+        def cold():
+            k = it[0] % 40
+            it[0] += 1
+            copies[k].mult_vector(ys[k], xs[k], stream=stream)
 
-    cold_ms = kernel_avg_ms(cold, 400, stream)
-    cold_gbs = bspmv / (cold_ms * 1e-3) / 1e9
-    log("spmv fp64: hot %.2f us (%.0f GB/s), cold %.2f us (%.0f GB/s)" %
-        (spmv_ms * 1e3, achieved, cold_ms * 1e3, cold_gbs))
+        cold_ms = kernel_avg_ms(cold, 400, stream)
+        cold_gbs = bspmv / (cold_ms * 1e-3) / 1e9
+        log("spmv fp64 cold (40 copies): %.2f us (%.0f GB/s)" % (cold_ms * 1e3, cold_gbs))
+    if args.calib_copy:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 5 x 256 MiB")
 
     # ---- forward error vs the FP64 oracle + CPU baseline ----
     cpu = None
@@ -278,13 +304,15 @@ def main():
         "variants": variants,
         "roofline": {"bound": "hbm", "kernel": "ell_spmv_f64 (Solver::mult_vector)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch": bspmv, "avg_launch_us": round(spmv_ms * 1e3, 3),
-                     "cold_achieved": round(cold_gbs, 1),
-                     "cold_frac": round(cold_gbs / HBM_PEAK_GBS, 4),
-                     "cold_avg_launch_us": round(cold_ms * 1e3, 3)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles("ell_spmv_f64_L7_traffic.json"),
+                     "bytes_per_launch": bspmv, "avg_launch_us": round(spmv_ms * 1e3, 3)},
         "cpu_baseline": cpu,
     }
+    if cold_ms is not None:
+        out["roofline"].update(cold_achieved=round(cold_gbs, 1),
+                               cold_frac=round(cold_gbs / HBM_PEAK_GBS, 4),
+                               cold_avg_launch_us=round(cold_ms * 1e3, 3))
     print(json.dumps(out))
 
 

@@ -200,7 +200,7 @@ __device__ __forceinline__ void block_dots(int nd, int skip, const VT* own, cons
     if ((int)threadIdx.x < outs) {
         double s = 0.0;
         for (int q = 0; q < GM_BS / 64; ++q) s += shd[q][threadIdx.x];
-        part[(size_t)(out0 + threadIdx.x) * G + blockIdx.x] = s;
+        part[(size_t)(out0 + threadIdx.x) * G + lblock()] = s;
     }
 }
 
@@ -274,7 +274,7 @@ k_residual(int n, int rpt, int width, const int* __restrict__ col,
     __shared__ double sh[GM_BS / 64];
     if (st->converged) return;
     double acc = 0.0;
-    const int base = blockIdx.x * GM_BS * rpt + threadIdx.x;
+    const int base = lblock() * GM_BS * rpt + threadIdx.x;
     for (int r = 0; r < rpt; ++r) {
         const int row = base + r * GM_BS;
         if (row >= n) break;
@@ -284,7 +284,7 @@ k_residual(int n, int rpt, int width, const int* __restrict__ col,
         acc += rr * rr;
     }
     acc = block_sum(acc, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+    if (threadIdx.x == 0) part[lblock()] = acc;
 }
 
 __global__ void __launch_bounds__(GM_BS)
@@ -292,14 +292,14 @@ k_sumsq(int n, int rpt, const double* __restrict__ b, double* __restrict__ part)
 {
     __shared__ double sh[GM_BS / 64];
     double acc = 0.0;
-    const int base = blockIdx.x * GM_BS * rpt + threadIdx.x;
+    const int base = lblock() * GM_BS * rpt + threadIdx.x;
     for (int r = 0; r < rpt; ++r) {
         const int row = base + r * GM_BS;
         if (row >= n) break;
         acc += b[row] * b[row];
     }
     acc = block_sum(acc, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+    if (threadIdx.x == 0) part[lblock()] = acc;
 }
 
 __global__ void k_set_bnorm(GmresState* st, const double* __restrict__ part, int G)
@@ -360,7 +360,7 @@ k_step_spmv(int n, int rpt, int width, const int* __restrict__ col, const MT* __
             sh_go = 1;
         }
     } else if (threadIdx.x < 64) {
-        const GivensOut g = givens_step(st, j - 1, part, G, blockIdx.x == 0);
+        const GivensOut g = givens_step(st, j - 1, part, G, lblock() == 0);
         if (threadIdx.x == 0) {
             sh_h = g.hn;
             sh_go = g.cont;
@@ -371,7 +371,7 @@ k_step_spmv(int n, int rpt, int width, const int* __restrict__ col, const MT* __
     const VT inv = (VT)(1.0 / sh_h);
     VT* Vj = V + (size_t)j * ldv;
     VT w[GM_RPT], vj[GM_RPT];
-    const int base = blockIdx.x * GM_BS * rpt + threadIdx.x;
+    const int base = lblock() * GM_BS * rpt + threadIdx.x;
 #pragma unroll
     for (int r = 0; r < GM_RPT; ++r) {
         const int row = base + r * GM_BS;
@@ -402,7 +402,7 @@ k_mgs(int n, int rpt, VT* __restrict__ w, const VT* __restrict__ V, size_t ldv, 
     const VT* Vp = V + (size_t)(i - 1) * ldv;
     const VT* Vi = V + (size_t)i * ldv;
     const bool last = (i == j + 1);
-    const int base = blockIdx.x * GM_BS * rpt + threadIdx.x;
+    const int base = lblock() * GM_BS * rpt + threadIdx.x;
     VT wv[GM_RPT], pv[GM_RPT], iv[GM_RPT];
 #pragma unroll
     for (int r = 0; r < GM_RPT; ++r) {
@@ -413,7 +413,7 @@ k_mgs(int n, int rpt, VT* __restrict__ w, const VT* __restrict__ V, size_t ldv, 
         iv[r] = (in && !last) ? Vi[row] : VT(0);
     }
     if (!active) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->H[(i - 1) * st->m + j] = h;
+    if (lblock() == 0 && threadIdx.x == 0) st->H[(i - 1) * st->m + j] = h;
     const VT hv = (VT)h;
     double acc = 0.0;
 #pragma unroll
@@ -427,7 +427,7 @@ k_mgs(int n, int rpt, VT* __restrict__ w, const VT* __restrict__ V, size_t ldv, 
         }
     }
     acc = block_sum(acc, sh);
-    if (threadIdx.x == 0) part[(size_t)i * G + blockIdx.x] = acc;
+    if (threadIdx.x == 0) part[(size_t)i * G + lblock()] = acc;
 }
 
 // CGS2: adds the partials of the j+1 dots of a pass, one wavefront per dot;
@@ -460,7 +460,7 @@ k_cgs_update(int n, int rpt, VT* __restrict__ w, const VT* __restrict__ V, size_
     if ((int)threadIdx.x <= j) hs[threadIdx.x] = st->hc[threadIdx.x];
     __syncthreads();
     VT wv[GM_RPT];
-    const int base = blockIdx.x * GM_BS * rpt + threadIdx.x;
+    const int base = lblock() * GM_BS * rpt + threadIdx.x;
 #pragma unroll
     for (int r = 0; r < GM_RPT; ++r) {
         const int row = base + r * GM_BS;
@@ -532,7 +532,7 @@ k_xupdate(int n, int rpt, const VT* __restrict__ V, size_t ldv, double* __restri
     if (k == 0) return;
     if ((int)threadIdx.x < k) ys[threadIdx.x] = st->y[threadIdx.x];
     __syncthreads();
-    const int base = blockIdx.x * GM_BS * rpt + threadIdx.x;
+    const int base = lblock() * GM_BS * rpt + threadIdx.x;
     for (int r = 0; r < rpt; ++r) {
         const int row = base + r * GM_BS;
         if (row >= n) break;
@@ -544,7 +544,7 @@ k_xupdate(int n, int rpt, const VT* __restrict__ V, size_t ldv, double* __restri
 
 __global__ void k_fill0(int n, double* __restrict__ x)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = lblock() * blockDim.x + threadIdx.x;
     if (i < n) x[i] = 0.0;
 }
 

@@ -234,7 +234,7 @@ __device__ __forceinline__ void bdots(int nd, int own_i, const VT* own, const VT
                 double s = 0.0;
 #pragma unroll
                 for (int ww = 0; ww < BB_BS / 64; ++ww) s += sh[ww][u][qq];
-                part[((size_t)(out0 + i0 + u) * KP + qq) * G + blockIdx.x] = s;
+                part[((size_t)(out0 + i0 + u) * KP + qq) * G + lblock()] = s;
             }
         }
         __syncthreads();
@@ -253,7 +253,7 @@ bk_step_spmm(int n, int KP, int K, int width, const int* __restrict__ col,
 {
     const int q = threadIdx.x & (KP - 1);
     const int RB = BB_BS / KP;
-    const int base = blockIdx.x * RB * BB_RPT + (int)threadIdx.x / KP;
+    const int base = lblock() * RB * BB_RPT + (int)threadIdx.x / KP;
     const bool act = q < K && cols[q].active;
     double h = 1.0;
     if (act) h = j == 0 ? cols[q].beta : cols[q].hn;
@@ -282,7 +282,7 @@ __global__ void __launch_bounds__(BB_BS)
 bk_reduce(int KP, int K, int nd, int j, int pass, const double* __restrict__ part, int G,
           BCol* __restrict__ cols)
 {
-    const int wv = blockIdx.x * (BB_BS / 64) + ((int)threadIdx.x >> 6);
+    const int wv = lblock() * (BB_BS / 64) + ((int)threadIdx.x >> 6);
     if (wv >= nd * KP) return;
     const int i = wv / KP, q = wv % KP;
     if (q >= K || !cols[q].active) return;
@@ -303,7 +303,7 @@ bk_update(int n, int KP, int K, VT* __restrict__ W, const VT* __restrict__ V, si
 {
     const int q = threadIdx.x & (KP - 1);
     const int RB = BB_BS / KP;
-    const int base = blockIdx.x * RB * BB_RPT + (int)threadIdx.x / KP;
+    const int base = lblock() * RB * BB_RPT + (int)threadIdx.x / KP;
     const bool act = q < K && cols[q].active;
     VT w[BB_RPT];
 #pragma unroll
@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(BB_BS)
 bk_givens(int KP, int K, int j, int m, const double* __restrict__ part, int G,
           BCol* __restrict__ cols, double rtol, double inner_rtol, int max_iters)
 {
-    const int q = blockIdx.x * (BB_BS / 64) + ((int)threadIdx.x >> 6);
+    const int q = lblock() * (BB_BS / 64) + ((int)threadIdx.x >> 6);
     if (q >= K) return;
     BCol* c = cols + q;
     if (!c->active) return;
@@ -391,7 +391,7 @@ bk_solve_y(int K, BCol* __restrict__ cols)
 {
     __shared__ double Rs[BB_BS / 64][BB_MAXM * BB_MAXM];
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int q = blockIdx.x * (BB_BS / 64) + wid;
+    const int q = lblock() * (BB_BS / 64) + wid;
     if (q >= K) return;
     BCol* c = cols + q;
     const int k = c->k;
@@ -420,7 +420,7 @@ bk_xupdate(int n, int KP, int K, const VT* __restrict__ V, size_t bs, double* __
 {
     const int q = threadIdx.x & (KP - 1);
     const int RB = BB_BS / KP;
-    const int base = blockIdx.x * RB * BB_RPT + (int)threadIdx.x / KP;
+    const int base = lblock() * RB * BB_RPT + (int)threadIdx.x / KP;
     if (q >= K) return;
     const BCol* c = cols + q;
     const int k = c->k;
@@ -444,7 +444,7 @@ bk_residual(int n, int KP, int K, int width, const int* __restrict__ col,
 {
     const int q = threadIdx.x & (KP - 1);
     const int RB = BB_BS / KP;
-    const int base = blockIdx.x * RB * BB_RPT + (int)threadIdx.x / KP;
+    const int base = lblock() * RB * BB_RPT + (int)threadIdx.x / KP;
     const bool act = q < K && !cols[q].converged;
     VT w[BB_RPT];
 #pragma unroll
@@ -480,7 +480,7 @@ bk_sumsq(int n, int KP, int K, const double* __restrict__ B, double* __restrict_
 {
     const int q = threadIdx.x & (KP - 1);
     const int RB = BB_BS / KP;
-    const int base = blockIdx.x * RB * BB_RPT + (int)threadIdx.x / KP;
+    const int base = lblock() * RB * BB_RPT + (int)threadIdx.x / KP;
     const bool act = q < K;
     double w[BB_RPT];
 #pragma unroll
@@ -494,7 +494,7 @@ bk_sumsq(int n, int KP, int K, const double* __restrict__ B, double* __restrict_
 __global__ void bk_set_bnorm(int KP, int K, const double* __restrict__ part, int G,
                              BCol* __restrict__ cols)
 {
-    const int q = blockIdx.x * (BB_BS / 64) + ((int)threadIdx.x >> 6);
+    const int q = lblock() * (BB_BS / 64) + ((int)threadIdx.x >> 6);
     if (q >= K) return;
     const double s = wave_sum_partials(part + (size_t)q * G, G);
     if ((threadIdx.x & 63) == 0) {
@@ -514,7 +514,7 @@ __global__ void bk_cycle_start(int KP, int K, const double* __restrict__ part, i
                                BCol* __restrict__ cols, BSum* __restrict__ sum, double rtol,
                                int max_iters)
 {
-    const int q = blockIdx.x * (BB_BS / 64) + ((int)threadIdx.x >> 6);
+    const int q = lblock() * (BB_BS / 64) + ((int)threadIdx.x >> 6);
     if (q >= K) return;
     BCol* c = cols + q;
     const bool done = c->converged;
@@ -548,7 +548,7 @@ __global__ void bk_cycle_start(int KP, int K, const double* __restrict__ part, i
 
 __global__ void bk_fill0(size_t n, double* __restrict__ x)
 {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t i = (size_t)lblock() * blockDim.x + threadIdx.x;
     if (i < n) x[i] = 0.0;
 }
 

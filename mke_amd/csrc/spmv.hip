@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(256)
 ell_spmv_f64(int n, int width, const int* __restrict__ col, const double* __restrict__ val,
              const double* __restrict__ x, double* __restrict__ y)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = lblock() * blockDim.x + threadIdx.x;
     if (i >= n) return;
     y[i] = ell_row_f64(n, width, i, col, val, x, 1.0, false);
 }
@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(256)
 ell_spmv_f32(int n, int width, const int* __restrict__ col, const V* __restrict__ val,
              const float* __restrict__ x, float* __restrict__ y)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = lblock() * blockDim.x + threadIdx.x;
     if (i >= n) return;
     y[i] = ell_row_f32<V>(n, width, i, col, val, x, 1.0f);
 }
@@ -90,7 +90,7 @@ ell_spmm_f32(int n, int width, int nrhs, const int* __restrict__ col,
              const V* __restrict__ val, const float* __restrict__ X, float* __restrict__ Y)
 {
     const int lane = threadIdx.x & 63;
-    const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int row = lblock() * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (row >= n) return;
     for (int r = lane; r < nrhs; r += 64) {
         float s = 0.0f;
