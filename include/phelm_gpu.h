@@ -207,6 +207,31 @@ int phg_gmres_batch(phg_matrix_t A, int nrhs, const double *d_B, double *d_X,
                     const phg_gmres_opts *opts, phg_gmres_info *infos,
                     phg_stream_t s);
 
+/* ---- multigrid-preconditioned solves on the icosahedral sphere ----------
+ * No single reference call: it makes the solves of SphereLaplace::solve
+ * (src/impl/slaplace_impl.h:84-124, the pole-pinned Laplace stiffness, as
+ * called once per fixed-point iteration by BarVortex::calc,
+ * examples/barvortex_impl.h:298) converge in a level-independent number of
+ * GMRES steps.  V-cycles over the nested meshes of levels 2..L of
+ * mke_mesh_sphere; every level's operator is assembled on its own mesh. */
+typedef struct phg_mg_s *phg_mg_t;
+/* fine: a full sphere mesh of mke_mesh_sphere (any level); kind/c_mass/
+ * c_lap as phg_assemble (PHG_MASS, PHG_LAPLACE or PHG_HELMHOLTZ).  The mesh
+ * must outlive the object. */
+int phg_mg_create(phg_mesh_t fine, int kind, double c_mass, double c_lap,
+                  phg_stream_t s, phg_mg_t *out);
+void phg_mg_destroy(phg_mg_t M);
+/* The fine-level operator (owned by M; bit-identical to phg_assemble). */
+int phg_mg_matrix(phg_mg_t M, phg_matrix_t *out);
+int phg_mg_levels(phg_mg_t M);
+/* phg_gmres_batch with M's fine operator, right-preconditioned by one
+ * V-cycle per Arnoldi step (flexible form: the preconditioned vectors are
+ * kept for the update).  opts->precision must be PHG_F64.  Calls sharing
+ * one M must not overlap in time (its level vectors are shared). */
+int phg_gmres_batch_mg(phg_mg_t M, int nrhs, const double *d_B, double *d_X,
+                       const phg_gmres_opts *opts, phg_gmres_info *infos,
+                       phg_stream_t s);
+
 /* ---- SphereJacobian (src/sjacobian.h:65-149) ---------------------------- */
 typedef struct phg_jacobian_s *phg_jacobian_t;
 /* SphereJacobian::SphereJacobian (src/sjacobian.cpp:415-446): assembles

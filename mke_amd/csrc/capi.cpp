@@ -30,8 +30,6 @@ int gmres_solve(phg_matrix_s* A, const double* b, double* x, const phg_gmres_opt
 void csr_structure(const SphereMesh& m, std::vector<int>& rp, std::vector<int>& ci);
 int block_gmres(phg_matrix_s* A, int nrhs, const double* B, double* X, const phg_gmres_opts* opts,
                 phg_block_info* info, hipStream_t s);
-int gmres_batch_solve(phg_matrix_s* A, int K, const double* B, double* X,
-                      const phg_gmres_opts* opts, phg_gmres_info* infos, hipStream_t s);
 
 int ensure_mesh_device(phg_mesh_s* m)
 {
@@ -379,6 +377,29 @@ int phg_block_gmres(phg_matrix_t A, int nrhs, const double* d_B, double* d_X,
         return PHG_ERR_ARG;
     }
     return block_gmres(A, nrhs, d_B, d_X, &o, info, (hipStream_t)s);
+}
+
+int phg_gmres_batch_mg(phg_mg_t M, int nrhs, const double* d_B, double* d_X,
+                       const phg_gmres_opts* opts, phg_gmres_info* infos, phg_stream_t s)
+{
+    if (!M || !d_B || !d_X) {
+        set_error("phg_gmres_batch_mg: null argument");
+        return PHG_ERR_ARG;
+    }
+    phg_gmres_opts o;
+    if (opts) o = *opts;
+    else {
+        phg_gmres_default_opts(&o);
+        o.ortho = PHG_CGS2;
+    }
+    if (o.max_iters < 0 || !(o.rtol >= 0)) {
+        set_error("phg_gmres_batch_mg: bad options");
+        return PHG_ERR_ARG;
+    }
+    phg_matrix_t A = nullptr;
+    int rc = phg_mg_matrix(M, &A);
+    if (rc) return rc;
+    return gmres_batch_solve(A, nrhs, d_B, d_X, &o, infos, (hipStream_t)s, M);
 }
 
 int phg_gmres_batch(phg_matrix_t A, int nrhs, const double* d_B, double* d_X,

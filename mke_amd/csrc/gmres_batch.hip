@@ -56,6 +56,7 @@ struct BSum {
 struct BatchWork {
     int n = 0, m = 0, kp = 0, G = 0, prec = 0;
     DevBuf<char> V, W;    // (m+1) x n x kp and 2 x n x kp of the Krylov type
+    DevBuf<char> Z;       // m x n x kp preconditioned vectors (with a preconditioner)
     DevBuf<double> part;  // (m+2) x kp x G block partials
     DevBuf<BCol> cols;    // kp
     DevBuf<BSum> sum;     // kp
@@ -64,6 +65,7 @@ struct BatchWork {
     hipGraphExec_t exec = nullptr;
     const void* key[4] = {nullptr, nullptr, nullptr, nullptr};
     int key_m = 0, key_ortho = -1, key_k = 0, key_maxit = 0;
+    const void* key_pc = nullptr;
     double key_rtol = 0.0, key_inner = 0.0;
     ~BatchWork()
     {
@@ -274,6 +276,46 @@ bk_step_spmm(int n, int KP, int K, int width, const int* __restrict__ col,
         }
     }
     bdots<VT>(j + 1, j, vj, V, bs, n, KP, q, base, RB, act, w, part, G, 0);
+}
+
+// Right-preconditioned step: v_j = W_src / h (stored) for the preconditioner.
+template <typename VT>
+__global__ void __launch_bounds__(BB_BS)
+bk_scale(int n, int KP, int K, const VT* __restrict__ Wsrc, VT* __restrict__ Vj, int j,
+         const BCol* __restrict__ cols)
+{
+    const size_t e = (size_t)lblock() * BB_BS + threadIdx.x;
+    if (e >= (size_t)n * KP) return;
+    const int q = (int)(e % KP);
+    if (q >= K || !cols[q].active) return;
+    const double h = j == 0 ? cols[q].beta : cols[q].hn;
+    Vj[e] = Wsrc[e] * (VT)(1.0 / h);
+}
+
+// W_dst = A z_j (z_j = M^-1 v_j, the preconditioned vector) and the partial
+// dots of W_dst with v_0..v_j.
+template <typename VT, typename MT>
+__global__ void __launch_bounds__(BB_BS)
+bk_step_spmm_pc(int n, int KP, int K, int width, const int* __restrict__ col,
+                const MT* __restrict__ val, const VT* __restrict__ Zj, const VT* __restrict__ V,
+                size_t bs, VT* __restrict__ Wdst, int j, double* __restrict__ part, int G,
+                const BCol* __restrict__ cols)
+{
+    const int q = threadIdx.x & (KP - 1);
+    const int RB = BB_BS / KP;
+    const int base = lblock() * RB * BB_RPT + (int)threadIdx.x / KP;
+    const bool act = q < K && cols[q].active;
+    VT w[BB_RPT];
+#pragma unroll
+    for (int t = 0; t < BB_RPT; ++t) {
+        const int row = base + t * RB;
+        w[t] = VT(0);
+        if (act && row < n) {
+            w[t] = BRow<VT, MT>::run(n, width, row, col, val, Zj, KP, q, VT(1));
+            Wdst[(size_t)row * KP + q] = w[t];
+        }
+    }
+    bdots<VT>(j + 1, -1, nullptr, V, bs, n, KP, q, base, RB, act, w, part, G, 0);
 }
 
 // Sums the partials of the nd dots of a pass, one wavefront per (i, column):
@@ -553,6 +595,19 @@ __global__ void bk_fill0(size_t n, double* __restrict__ x)
 }
 
 // ---------------------------------------------------------------- driver --
+// The preconditioner works on FP64 vectors (the form it is used with).
+template <typename VT>
+void precond_apply(phg_mg_s* pc, int KP, const VT* v, VT* z, hipStream_t s);
+template <>
+void precond_apply<double>(phg_mg_s* pc, int KP, const double* v, double* z, hipStream_t s)
+{
+    mg_apply(pc, KP, v, z, s);
+}
+template <>
+void precond_apply<float>(phg_mg_s*, int, const float*, float*, hipStream_t)
+{
+}
+
 int next_pow2(int k)
 {
     int p = 1;
@@ -560,7 +615,7 @@ int next_pow2(int k)
     return p;
 }
 
-int prepare_batch(phg_matrix_s* A, int slot, int prec, int m, int kp, BatchWork** out)
+int prepare_batch(phg_matrix_s* A, int slot, int prec, int m, int kp, bool pc, BatchWork** out)
 {
     BatchWork*& w = A->batch[slot];
     const int n = A->n_rows;
@@ -584,6 +639,7 @@ int prepare_batch(phg_matrix_s* A, int slot, int prec, int m, int kp, BatchWork*
         PHG_HIP(hipHostMalloc((void**)&b->host, sizeof(BSum) * kp, hipHostMallocDefault));
         w = b.release();
     }
+    if (pc && !w->Z.p) PHG_HIP(w->Z.alloc((size_t)w->m * w->n * w->kp * (prec == PHG_F64 ? 8 : 4)));
     *out = w;
     return PHG_OK;
 }
@@ -591,7 +647,7 @@ int prepare_batch(phg_matrix_s* A, int slot, int prec, int m, int kp, BatchWork*
 template <typename VT, typename MT>
 void enqueue_batch_cycle(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K,
                          const double* B, double* X, int m, int passes, double rtol,
-                         double inner_rtol, int max_iters, hipStream_t s)
+                         double inner_rtol, int max_iters, phg_mg_s* pc, hipStream_t s)
 {
     const int n = w->n, KP = w->kp, G = w->G;
     const size_t bs = (size_t)n * KP;
@@ -605,8 +661,19 @@ void enqueue_batch_cycle(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K,
     for (int j = 0; j < m; ++j) {
         VT* src = (j & 1) ? W1 : W0;
         VT* dst = (j & 1) ? W0 : W1;
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_step_spmm<VT, MT>), grid, block, 0, s, n, KP, K,
-                           E.width, E.col.p, E.val.p, src, V, bs, dst, j, part, G, cols);
+        if (pc) {
+            // right preconditioning (flexible form): z_j = M^-1 v_j kept for the update
+            VT* Vj = V + (size_t)j * bs;
+            VT* Zj = (VT*)w->Z.p + (size_t)j * bs;
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_scale<VT>), dim3((unsigned)((bs + BB_BS - 1) / BB_BS)),
+                               block, 0, s, n, KP, K, src, Vj, j, cols);
+            precond_apply<VT>(pc, KP, Vj, Zj, s);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_step_spmm_pc<VT, MT>), grid, block, 0, s, n, KP,
+                               K, E.width, E.col.p, E.val.p, Zj, V, bs, dst, j, part, G, cols);
+        } else {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_step_spmm<VT, MT>), grid, block, 0, s, n, KP, K,
+                               E.width, E.col.p, E.val.p, src, V, bs, dst, j, part, G, cols);
+        }
         for (int p = 0; p < passes; ++p) {
             const int nd = j + 1;
             hipLaunchKernelGGL(bk_reduce, dim3((nd * KP + 3) / 4), block, 0, s, KP, K, nd, j, p,
@@ -618,8 +685,8 @@ void enqueue_batch_cycle(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K,
                            inner_rtol, max_iters);
     }
     hipLaunchKernelGGL(bk_solve_y, dim3(cb), block, 0, s, K, cols);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_xupdate<VT>), grid, block, 0, s, n, KP, K, V, bs, X,
-                       cols);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_xupdate<VT>), grid, block, 0, s, n, KP, K,
+                       pc ? (const VT*)w->Z.p : (const VT*)V, bs, X, cols);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_residual<VT>), grid, block, 0, s, n, KP, K,
                        A->ell64.width, A->ell64.col.p, A->ell64.val.p, B, X, W0, part, G, cols);
     hipLaunchKernelGGL(bk_cycle_start, dim3(cb), block, 0, s, KP, K, part, G, cols, w->sum.p,
@@ -628,7 +695,7 @@ void enqueue_batch_cycle(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K,
 
 template <typename VT, typename MT>
 int batch_run(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const double* B, double* X,
-              const phg_gmres_opts& o, phg_gmres_info* infos, hipStream_t s)
+              const phg_gmres_opts& o, phg_gmres_info* infos, phg_mg_s* pc, hipStream_t s)
 {
     const int n = w->n, KP = w->kp, G = w->G, m = o.restart;
     const size_t bs = (size_t)n * KP;
@@ -657,7 +724,8 @@ int batch_run(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const doub
         const void* key[4] = {B, X, E.val.p, A->ell64.val.p};
         const bool same = w->exec && w->key_m == m && w->key_ortho == o.ortho && w->key_k == K &&
                           w->key_rtol == o.rtol && w->key_inner == inner &&
-                          w->key_maxit == o.max_iters && !memcmp(key, w->key, sizeof key);
+                          w->key_maxit == o.max_iters && w->key_pc == (const void*)pc &&
+                          !memcmp(key, w->key, sizeof key);
         if (!same) {
             if (w->exec) {
                 PHG_HIP(hipGraphExecDestroy(w->exec));
@@ -667,7 +735,7 @@ int batch_run(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const doub
             hipGraph_t g = nullptr;
             PHG_HIP(hipStreamBeginCapture(w->cap, hipStreamCaptureModeThreadLocal));
             enqueue_batch_cycle<VT, MT>(A, E, w, K, B, X, m, passes, o.rtol, inner, o.max_iters,
-                                        w->cap);
+                                        pc, w->cap);
             PHG_HIP(hipStreamEndCapture(w->cap, &g));
             hipError_t e = hipGraphInstantiate(&w->exec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
@@ -679,6 +747,7 @@ int batch_run(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const doub
             w->key_rtol = o.rtol;
             w->key_inner = inner;
             w->key_maxit = o.max_iters;
+            w->key_pc = pc;
         }
     }
     for (;;) {
@@ -691,7 +760,7 @@ int batch_run(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const doub
             PHG_HIP(hipGraphLaunch(w->exec, s));
         } else {
             enqueue_batch_cycle<VT, MT>(A, E, w, K, B, X, m, passes, o.rtol, inner, o.max_iters,
-                                        s);
+                                        pc, s);
             PHG_HIP(hipGetLastError());
         }
     }
@@ -713,7 +782,8 @@ int batch_run(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const doub
 }  // namespace
 
 int gmres_batch_solve(phg_matrix_s* A, int K, const double* B, double* X,
-                      const phg_gmres_opts* opts, phg_gmres_info* infos, hipStream_t s)
+                      const phg_gmres_opts* opts, phg_gmres_info* infos, hipStream_t s,
+                      phg_mg_s* pc)
 {
     phg_gmres_opts o = *opts;
     if (K < 1 || K > BB_MAXK) {
@@ -748,19 +818,29 @@ int gmres_batch_solve(phg_matrix_s* A, int K, const double* B, double* X,
         return PHG_ERR_ARG;
     }
     const int kp = next_pow2(K);
+    if (pc) {
+        if (o.precision != PHG_F64 || mg_rows(pc) != A->n_rows) {
+            set_error("gmres_batch: the multigrid preconditioner needs PHG_F64 and its own matrix");
+            return PHG_ERR_ARG;
+        }
+        rc = mg_reserve(pc, kp);
+        if (rc) return rc;
+    }
     BatchWork* w = nullptr;
     const int slot = o.precision == PHG_F64 ? 0 : 1;
-    rc = prepare_batch(A, slot, o.precision == PHG_F64 ? PHG_F64 : PHG_F32, o.restart, kp, &w);
+    rc = prepare_batch(A, slot, o.precision == PHG_F64 ? PHG_F64 : PHG_F32, o.restart, kp,
+                       pc != nullptr, &w);
     if (rc) return rc;
-    if (o.precision == PHG_F64) return batch_run<double, double>(A, A->ell64, w, K, B, X, o, infos, s);
+    if (o.precision == PHG_F64)
+        return batch_run<double, double>(A, A->ell64, w, K, B, X, o, infos, pc, s);
     if (o.precision == PHG_F32) {
         rc = ensure_ell32(A, s);
         if (rc) return rc;
-        return batch_run<float, float>(A, A->ell32, w, K, B, X, o, infos, s);
+        return batch_run<float, float>(A, A->ell32, w, K, B, X, o, infos, nullptr, s);
     }
     rc = ensure_ell16(A, s);
     if (rc) return rc;
-    return batch_run<float, uint16_t>(A, A->ell16, w, K, B, X, o, infos, s);
+    return batch_run<float, uint16_t>(A, A->ell16, w, K, B, X, o, infos, nullptr, s);
 }
 
 }  // namespace phg

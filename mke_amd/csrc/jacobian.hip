@@ -22,8 +22,6 @@
 #include "phg_internal.h"
 
 namespace phg {
-int gmres_batch_solve(phg_matrix_s* A, int K, const double* B, double* X,
-                      const phg_gmres_opts* opts, phg_gmres_info* infos, hipStream_t s);
 int assemble_device(phg_mesh_s* M, int kind, double c_mass, double c_lap, hipStream_t stream,
                     phg_matrix_s** out);
 int assemble_boundary_device(phg_mesh_s* M, int kind, double c_mass, double c_lap,

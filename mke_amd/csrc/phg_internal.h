@@ -101,6 +101,13 @@ struct phg_matrix_s {
 };
 
 namespace phg {
+// multigrid preconditioner (mg.hip)
+int mg_reserve(phg_mg_s* M, int kp);
+void mg_apply(phg_mg_s* M, int KP, const double* v, double* z, hipStream_t s);
+int mg_rows(const phg_mg_s* M);
+int gmres_batch_solve(phg_matrix_s* A, int K, const double* B, double* X,
+                      const phg_gmres_opts* opts, phg_gmres_info* infos, hipStream_t s,
+                      phg_mg_s* pc = nullptr);
 int ensure_mesh_device(phg_mesh_s* m);
 int ensure_ell64(phg_matrix_s* A, hipStream_t s);
 int ensure_ell32(phg_matrix_s* A, hipStream_t s);

@@ -75,6 +75,11 @@ SIGNATURES = {
     "phg_gmres": (_I, [_P, _P, _P, C.POINTER(GmresOpts), C.POINTER(GmresInfo), _P]),
     "phg_block_gmres": (_I, [_P, _I, _P, _P, C.POINTER(GmresOpts), C.POINTER(BlockInfo), _P]),
     "phg_gmres_batch": (_I, [_P, _I, _P, _P, C.POINTER(GmresOpts), C.POINTER(GmresInfo), _P]),
+    "phg_mg_create": (_I, [_P, _I, _D, _D, _P, C.POINTER(_P)]),
+    "phg_mg_destroy": (None, [_P]),
+    "phg_mg_matrix": (_I, [_P, C.POINTER(_P)]),
+    "phg_mg_levels": (_I, [_P]),
+    "phg_gmres_batch_mg": (_I, [_P, _I, _P, _P, C.POINTER(GmresOpts), C.POINTER(GmresInfo), _P]),
     "phg_jacobian_create": (_I, [_P, _P, C.POINTER(_P)]),
     "phg_jacobian_destroy": (None, [_P]),
     "phg_jacobian_matrix": (_I, [_P, _I, C.POINTER(_P)]),
@@ -364,6 +369,43 @@ def _opts(restart, rtol, max_iters, ortho, precision, inner_rtol=1e-5, use_x0=Fa
     o.restart, o.max_iters, o.rtol, o.ortho = restart, max_iters, rtol, ortho
     o.precision, o.inner_rtol, o.use_x0, o.graph = precision, inner_rtol, int(use_x0), int(graph)
     return o
+
+
+class Multigrid:
+    """V-cycle preconditioner over the nested sphere meshes of levels 2..L and
+    its fine operator (phg_mg_*); solve_batch runs the right-preconditioned
+    batched GMRES (FP64)."""
+
+    def __init__(self, mesh, kind=LAPLACE, c_mass=1.0, c_lap=0.0, stream=None):
+        self.m = mesh
+        self.h = C.c_void_p()
+        check(lib().phg_mg_create(mesh.h, kind, c_mass, c_lap, _stream(stream), C.byref(self.h)),
+              "phg_mg_create")
+        mh = C.c_void_p()
+        check(lib().phg_mg_matrix(self.h, C.byref(mh)))
+        self.A = Matrix(mh, owned=False)
+        self.levels = lib().phg_mg_levels(self.h)
+
+    def solve_batch(self, X, B, restart=30, rtol=1e-10, max_iters=10000, ortho=CGS2,
+                    use_x0=False, graph=True, stream=None):
+        K = int(B.shape[1]) if B.dim() == 2 else 1
+        o = _opts(restart, rtol, max_iters, ortho, F64, use_x0=use_x0, graph=graph)
+        infos = (GmresInfo * K)()
+        check(lib().phg_gmres_batch_mg(self.h, K, _ptr(B), _ptr(X), C.byref(o), infos,
+                                       _stream(stream)), "phg_gmres_batch_mg")
+        return [dict(iterations=i.iterations, restarts=i.restarts, rel_residual=i.rel_residual,
+                     converged=bool(i.converged)) for i in infos]
+
+    def close(self):
+        if self.h:
+            lib().phg_mg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class SphereJacobian:
