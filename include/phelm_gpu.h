@@ -111,6 +111,12 @@ int phg_assemble(phg_mesh_t m, int kind, double c_mass, double c_lap,
  * inner x outer matrix, same kinds. */
 int phg_assemble_boundary(phg_mesh_t m, int kind, double c_mass, double c_lap,
                           phg_matrix_t *out);
+/* generate_full_matrix(A, m, cb) (src/generators.h:200-234): every point x
+ * every point, rows in the zone of their first triangle, nothing skipped;
+ * with PHG_MASS it is SphereNorm's matrix (src/norm.h:71-74,
+ * sphere_scalar_cb src/phelm.cpp:330-335). */
+int phg_assemble_full(phg_mesh_t m, int kind, double c_mass, double c_lap,
+                      phg_stream_t stream, phg_matrix_t *out);
 /* Shape and host copies of the CSR (pointers may be NULL). */
 int phg_matrix_info(phg_matrix_t A, int *n_rows, int *n_cols, long *nnz);
 int phg_matrix_csr(phg_matrix_t A, int *row_ptr, int *col_idx, double *vals);
@@ -256,6 +262,60 @@ int phg_jacobian_calc2t(phg_jacobian_t J, int npairs, const double *d_u,
                         const double *d_v, double *d_ans,
                         const phg_gmres_opts *opts, phg_gmres_info *infos,
                         phg_stream_t s);
+
+/* ---- BarVortex implicit step (examples/barvortex.h:62-155,
+ *      examples/barvortex_impl.h:108-310) --------------------------------- */
+/* BarVortexConf without the callbacks: the Coriolis/orography term lh is
+ * passed as a mesh function (proj(lh, m, conf.coriolis),
+ * barvortex_impl.h:121-122); the right-hand-side callback conf.rp is not
+ * supported (no forcing). */
+typedef struct {
+    double tau, sigma, mu, k1, k2, theta;
+} phg_bv_conf;
+typedef struct {
+    int max_iters;         /* fixed-point passes, reference: 1000              */
+    double tol;            /* stop at ||u_n1 - u_n|| / ||u|| < tol, ref. 1e-14  */
+    double rtol_mass;      /* mass solves of L() and of the Jacobian           */
+    double rtol_helmholtz; /* A_ solve                                        */
+    double rtol_poisson;   /* laplace_ solve (multigrid-preconditioned)       */
+    int precision;         /* PHG_F64, or PHG_F32 / PHG_BF16: mass and Helmholtz
+                              solves in FP32 Krylov with FP64 refinement      */
+} phg_bv_opts;
+typedef struct {
+    int iterations;        /* fixed-point passes of the last step             */
+    double rel_change;     /* ||u_n1 - u_n|| / ||u|| of the last pass         */
+    int converged;         /* rel_change < tol                                */
+    int solver_iterations; /* GMRES steps of the member's solves (all kinds)  */
+} phg_bv_info;
+typedef struct phg_barvortex_s *phg_barvortex_t;
+
+void phg_bv_default_opts(phg_bv_opts *o);
+/* Device bytes an object with `slots` slots needs on a mesh of n_points
+ * (estimate used for admission). */
+size_t phg_barvortex_estimate_bytes(int n_points, int slots);
+/* BarVortex(m, conf) for up to `slots` (1, 2, 4 or 8) members at once:
+ * assembles A_, SphereLaplace's and SphereJacobian's matrices, SphereNorm's
+ * matrix and the Poisson V-cycle hierarchy; d_lh: n_points values (device).
+ * The boundary (pole) values of u and w are zero, as the reference drivers
+ * pass them (examples/test_barvortex.cpp:445). */
+int phg_barvortex_create(phg_mesh_t m, const phg_bv_conf *conf, const double *d_lh,
+                         int slots, phg_stream_t s, phg_barvortex_t *out);
+void phg_barvortex_destroy(phg_barvortex_t B);
+int phg_barvortex_slots(phg_barvortex_t B);
+/* BarVortex::calc(u1, u, 0, 0, t) for nmembers (<= slots) members: d_u and
+ * d_u1 hold nmembers mesh functions of n_points values each, one after the
+ * other.  infos: nmembers host entries (may be NULL). */
+int phg_barvortex_calc(phg_barvortex_t B, int nmembers, const double *d_u, double *d_u1,
+                       const phg_bv_opts *opts, phg_bv_info *infos, phg_stream_t s);
+/* The same step, slot by slot, for a caller that admits and retires members
+ * while others run (continuous batching): load a free slot with u, then call
+ * iterate (starts loaded slots, runs one fixed-point pass of the running
+ * ones; state[k] = 0 free, 1 running, 2 done), and store a done slot's u1,
+ * which frees it.  A member's result does not depend on the other slots. */
+int phg_barvortex_load(phg_barvortex_t B, int slot, const double *d_u, phg_stream_t s);
+int phg_barvortex_iterate(phg_barvortex_t B, const phg_bv_opts *opts, int *state,
+                          phg_bv_info *infos, phg_stream_t s);
+int phg_barvortex_store(phg_barvortex_t B, int slot, double *d_u1, phg_stream_t s);
 
 #ifdef __cplusplus
 }

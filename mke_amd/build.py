@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libphelm_hip.so")
 
-HIP_SOURCES = ["assemble.hip", "spmv.hip", "gmres.hip", "gmres_batch.hip", "mg.hip", "jacobian.hip", "block.hip", "serve.hip"]
+HIP_SOURCES = ["assemble.hip", "spmv.hip", "gmres.hip", "gmres_batch.hip", "mg.hip", "jacobian.hip", "barvortex.hip", "block.hip", "serve.hip"]
 CPP_SOURCES = ["mesh.cpp", "capi.cpp"]
 
 # -ffp-contract=off everywhere: the assembly must match the CPU restatement

@@ -105,6 +105,52 @@ void build_plan(const SphereMesh& m, AsmPlan& P)
     }
 }
 
+// generate_full_matrix (src/generators.h:200-234): every point is a row, in
+// the zone of its first triangle; columns are point ids; nothing is skipped.
+void build_plan_full(const SphereMesh& m, AsmPlan& P)
+{
+    const int V = m.n_points;
+    P.rp.assign(V + 1, 0);
+    P.ci.clear();
+    P.slot_ptr.assign(V + 1, 0);
+    P.row_zone.resize(V);
+    std::vector<int> tmp;
+    for (int p = 0; p < V; ++p) {
+        tmp.clear();
+        for (int e = m.adj_ptr[p]; e < m.adj_ptr[p + 1]; ++e)
+            for (int a = 0; a < 3; ++a) tmp.push_back(m.tri[3 * m.adj[e] + a]);
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+        P.ci.insert(P.ci.end(), tmp.begin(), tmp.end());
+        P.rp[p + 1] = (int)P.ci.size();
+        P.slot_ptr[p + 1] = P.slot_ptr[p] + (m.adj_ptr[p + 1] - m.adj_ptr[p]);
+        P.row_zone[p] = m.tri_zone[m.adj[m.adj_ptr[p]]];
+    }
+    const int S = P.slot_ptr[V];
+    P.slot_row.resize(S);
+    P.slot_tri.resize(S);
+    P.slot_local.resize(S);
+    P.pos.resize((size_t)S * 3);
+    for (int p = 0; p < V; ++p) {
+        int s = P.slot_ptr[p];
+        for (int e = m.adj_ptr[p]; e < m.adj_ptr[p + 1]; ++e, ++s) {
+            const int t = m.adj[e];
+            P.slot_row[s] = p;
+            P.slot_tri[s] = t;
+            int loc = 0;
+            while (m.tri[3 * t + loc] != p) ++loc;
+            P.slot_local[s] = loc;
+            for (int a = 0; a < 3; ++a) {
+                const int q = m.tri[3 * t + a];
+                int where = -1;
+                for (int c = P.rp[p]; c < P.rp[p + 1]; ++c)
+                    if (P.ci[c] == q) where = c;
+                P.pos[(size_t)s * 3 + a] = where;
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256)
 elem_contrib(int S, int kind, double c_mass, double c_lap, int n_zones,
              const double* __restrict__ coords, const int* __restrict__ tri,
@@ -163,12 +209,13 @@ row_reduce(int rs, const int* __restrict__ rp, const int* __restrict__ slot_ptr,
 }  // namespace
 
 int assemble_device(phg_mesh_s* M, int kind, double c_mass, double c_lap,
-                    hipStream_t stream, phg_matrix_s** out)
+                    hipStream_t stream, phg_matrix_s** out, bool full)
 {
     const SphereMesh& m = M->h;
     AsmPlan P;
-    build_plan(m, P);
-    const int rs = (int)m.inner.size();
+    if (full) build_plan_full(m, P);
+    else build_plan(m, P);
+    const int rs = full ? m.n_points : (int)m.inner.size();
     const int S = P.slot_ptr[rs];
     int rc = ensure_mesh_device(M);
     if (rc) return rc;

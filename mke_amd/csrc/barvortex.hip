@@ -1,0 +1,654 @@
+// BarVortex::calc (examples/barvortex_impl.h:171-310) device resident, for S
+// slots (ensemble members on one mesh) at once.
+//
+// One implicit step of the barotropic vorticity equation: with w = L(u)
+// (SphereLaplace::calc1) and FC = w/tau + mu(1-theta) L(w) - sigma(1-theta) w,
+// iterate from u_n = w_n = 0
+//     tmp1 = k1 theta w_n + k1 (1-theta) w + k2 lh
+//     tmp2 = theta u_n + (1-theta) u
+//     F    = FC_inner - J(tmp2, tmp1)                     (SphereJacobian::calc2)
+//     w_n  = p2u(A^-1 (idt F))                            (phelm::solve, Helmholtz A)
+//     u_n1 = p2u(laplace^-1 (idt u2p(w_n)))               (SphereLaplace::solve)
+//     stop when ||u_n1 - u_n|| / ||u|| < tol (SphereNorm, src/norm.h:58-120)
+// as the reference does, with the boundary (pole) values zero as its drivers
+// pass them (examples/test_barvortex.cpp:445).
+//
+// Layout: every per-member vector is V x S (or n x S) row-major, slot fastest,
+// so each solve is one batched GMRES over the S slots, each slot with its own
+// Krylov space and stopping test.  A slot that is empty, not yet started or
+// finished has zero columns in every solve, which the batched solver skips;
+// a slot's arithmetic never depends on the other slots, so a member gives the
+// same bits alone or in any mix.  The Poisson solve is preconditioned by the
+// multigrid V-cycle; the Helmholtz, Jacobian and L() mass solves are plain
+// batched GMRES (mass-dominated, well conditioned).  The Helmholtz and
+// Poisson solves start from the current iterate (w_n, u_n): once the iterate
+// stops moving they return it unchanged and the step ends.
+#include <cmath>
+#include <cstring>
+
+#include "ell.h"
+#include "phg_internal.h"
+
+namespace phg {
+int jacobian_calc2_strided(phg_jacobian_s* J, int np, const double* d_u, const double* d_v,
+                           size_t su, size_t sk, double* d_ans, size_t ai, size_t ak,
+                           double sign, const phg_gmres_opts* opts, phg_gmres_info* infos,
+                           hipStream_t s);
+}  // namespace phg
+
+using namespace phg;
+
+namespace {
+constexpr int BV_MAXS = 8;
+constexpr int BV_BS = 256;
+}  // namespace
+
+struct phg_barvortex_s {
+    phg_mesh_s* mesh = nullptr;
+    int V = 0, n = 0, os = 0, S = 0;
+    phg_bv_conf conf{};
+    phg_matrix_s* idt = nullptr;    // SphereLaplace::idt_ (MASS)
+    phg_matrix_s* bnd3 = nullptr;   // SphereLaplace::bnd3_ (slaplace_integrate_cb, boundary)
+    phg_matrix_s* A = nullptr;      // BarVortex::A_ (Helmholtz)
+    phg_matrix_s* norm = nullptr;   // SphereNorm::NORM_ (full mass, V x V)
+    phg_jacobian_t J = nullptr;
+    phg_mg_t mg = nullptr;          // Poisson: laplace_ and its V-cycle
+    DevBuf<double> lh;              // V
+    DevBuf<double> U, W, FC, Un, Wn, Un1, T1, T2;  // V x S
+    DevBuf<double> JAC, F, RP, X, Y, Z, Q;          // n x S
+    DevBuf<double> Ub;                              // outer x S
+    DevBuf<double> part, nrm;       // reductions
+    DevBuf<int> dmask;
+    int G = 0;
+    std::vector<int> status, iters, solver_iters;  // per slot
+    std::vector<double> nr0, rel;
+    ~phg_barvortex_s()
+    {
+        delete idt;
+        delete bnd3;
+        delete A;
+        delete norm;
+        if (J) phg_jacobian_destroy(J);
+        if (mg) phg_mg_destroy(mg);
+    }
+};
+
+namespace {
+
+inline dim3 g256(size_t work) { return dim3((unsigned)((work + BV_BS - 1) / BV_BS)); }
+
+// Y[r][s] = beta * Y[r][s] + sum_c A[r][c] X[c][s] (CSR order, no fma); only
+// columns with mask[s] != 0.
+__global__ void bv_spmm(int rows, int S, const int* __restrict__ rp, const int* __restrict__ ci,
+                        const double* __restrict__ val, const double* __restrict__ X,
+                        double* __restrict__ Y, int accumulate, const int* __restrict__ mask)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)rows * S) return;
+    const int r = (int)(e / S), s = (int)(e % S);
+    if (!mask[s]) return;
+    double t = 0.0;
+    for (int k = rp[r]; k < rp[r + 1]; ++k) t += val[k] * X[(size_t)ci[k] * S + s];
+    Y[e] = accumulate ? Y[e] + t : t;
+}
+
+// p[i][s] = u[inner[i]][s]
+__global__ void bv_u2p(int n, int S, const int* __restrict__ inner, const double* __restrict__ u,
+                       double* __restrict__ p, const int* __restrict__ mask)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * S) return;
+    const int i = (int)(e / S), s = (int)(e % S);
+    p[e] = mask[s] ? u[(size_t)inner[i] * S + s] : 0.0;
+}
+
+// u[k][s] = boundary ? 0 : p[p2io[k]][s]  (p2u with zero boundary values)
+__global__ void bv_p2u(int V, int S, const int* __restrict__ isb, const int* __restrict__ p2io,
+                       const double* __restrict__ p, double* __restrict__ u,
+                       const int* __restrict__ mask)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)V * S) return;
+    const int k = (int)(e / S), s = (int)(e % S);
+    if (!mask[s]) return;
+    u[e] = isb[k] ? 0.0 : p[(size_t)p2io[k] * S + s];
+}
+
+// r = k1 a + k2 b (vec_sum1), masked columns; zero elsewhere when zero_rest.
+__global__ void bv_axpby(size_t rows, int S, double k1, const double* __restrict__ a, double k2,
+                         const double* __restrict__ b, double* __restrict__ r,
+                         const int* __restrict__ mask, int zero_rest)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * S) return;
+    const int s = (int)(e % S);
+    if (mask[s]) r[e] = k1 * a[e] + k2 * b[e];
+    else if (zero_rest) r[e] = 0.0;
+}
+
+// r = a + k * lh (vec_sum2 with a mesh function shared by all slots)
+__global__ void bv_add_shared(int V, int S, const double* __restrict__ a, double k,
+                              const double* __restrict__ lh, double* __restrict__ r,
+                              const int* __restrict__ mask)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)V * S) return;
+    const int p = (int)(e / S), s = (int)(e % S);
+    if (mask[s]) r[e] = a[e] + k * lh[p];
+}
+
+// F[i][s] = FC[inner[i]][s] - jac[i][s]
+__global__ void bv_rhs(int n, int S, const int* __restrict__ inner, const double* __restrict__ FC,
+                       const double* __restrict__ jac, double* __restrict__ F,
+                       const int* __restrict__ mask)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * S) return;
+    const int i = (int)(e / S), s = (int)(e % S);
+    F[e] = mask[s] ? FC[(size_t)inner[i] * S + s] - jac[e] : 0.0;
+}
+
+__global__ void bv_zero_cols(size_t rows, int S, double* __restrict__ x,
+                             const int* __restrict__ mask)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * S) return;
+    if (!mask[e % S]) x[e] = 0.0;
+}
+
+__global__ void bv_copy_cols(size_t rows, int S, const double* __restrict__ src,
+                             double* __restrict__ dst, const int* __restrict__ mask)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * S) return;
+    if (mask[e % S]) dst[e] = src[e];
+}
+
+// column s of a V x S array <-> a contiguous vector
+__global__ void bv_col_in(int V, int S, int s, const double* __restrict__ v, double* __restrict__ M)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < V) M[(size_t)k * S + s] = v[k];
+}
+__global__ void bv_col_out(int V, int S, int s, const double* __restrict__ M, double* __restrict__ v)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < V) v[k] = M[(size_t)k * S + s];
+}
+
+// Block partials of d^T N d per column, d = a - b (b may be null): N is the
+// full V x V mass matrix (CSR).  Thread per (row, column), 256/S rows per
+// block; part[s * G + block].
+__global__ void __launch_bounds__(BV_BS)
+bv_normsq(int V, int S, const int* __restrict__ rp, const int* __restrict__ ci,
+          const double* __restrict__ val, const double* __restrict__ a,
+          const double* __restrict__ b, double* __restrict__ part, int G,
+          const int* __restrict__ mask)
+{
+    __shared__ double sh[BV_BS / 64][BV_MAXS];
+    const int s = threadIdx.x % S;
+    const int r = blockIdx.x * (BV_BS / S) + (int)threadIdx.x / S;
+    double acc = 0.0;
+    if (r < V && mask[s]) {
+        double t = 0.0;
+        for (int k = rp[r]; k < rp[r + 1]; ++k) {
+            const size_t o = (size_t)ci[k] * S + s;
+            t += val[k] * (b ? a[o] - b[o] : a[o]);
+        }
+        const size_t o = (size_t)r * S + s;
+        acc = (b ? a[o] - b[o] : a[o]) * t;
+    }
+    for (int off = S; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane < S) sh[wid][lane] = acc;
+    __syncthreads();
+    if ((int)threadIdx.x < S) {
+        double t = 0.0;
+        for (int w = 0; w < BV_BS / 64; ++w) t += sh[w][threadIdx.x];
+        part[(size_t)threadIdx.x * G + blockIdx.x] = t;
+    }
+}
+
+// out[s] = sqrt(sum of the G partials of column s), fixed order, one wavefront per column.
+__global__ void bv_sum_cols(int S, const double* __restrict__ part, int G, double* __restrict__ out)
+{
+    const int s = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (s >= S) return;
+    double t = 0.0;
+    for (int b = lane; b < G; b += 64) t += part[(size_t)s * G + b];
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    if (lane == 0) out[s] = sqrt(t);
+}
+
+int upload_mask(phg_barvortex_s* B, const std::vector<int>& m, hipStream_t s)
+{
+    PHG_HIP(hipMemcpyAsync(B->dmask.p, m.data(), sizeof(int) * B->S, hipMemcpyHostToDevice, s));
+    return PHG_OK;
+}
+
+void spmm(phg_barvortex_s* B, const phg_matrix_s* M, const double* X, double* Y, int accumulate,
+          hipStream_t s)
+{
+    hipLaunchKernelGGL(bv_spmm, g256((size_t)M->n_rows * B->S), dim3(BV_BS), 0, s, M->n_rows,
+                       B->S, M->rp.p, M->ci.p, M->val.p, X, Y, accumulate, B->dmask.p);
+}
+
+// out = sqrt(d^T NORM d) per slot, d = a - b; copied to host (synchronises).
+int norms(phg_barvortex_s* B, const double* a, const double* b, std::vector<double>& out,
+          hipStream_t s)
+{
+    const int S = B->S;
+    hipLaunchKernelGGL(bv_normsq, dim3(B->G), dim3(BV_BS), 0, s, B->V, S, B->norm->rp.p,
+                       B->norm->ci.p, B->norm->val.p, a, b, B->part.p, B->G, B->dmask.p);
+    hipLaunchKernelGGL(bv_sum_cols, dim3(1), dim3(64 * BV_MAXS), 0, s, S, B->part.p, B->G,
+                       B->nrm.p);
+    PHG_HIP(hipGetLastError());
+    out.assign(S, 0.0);
+    PHG_HIP(hipMemcpyAsync(out.data(), B->nrm.p, sizeof(double) * S, hipMemcpyDeviceToHost, s));
+    PHG_HIP(hipStreamSynchronize(s));
+    return PHG_OK;
+}
+
+phg_gmres_opts solve_opts(double rtol, int precision, int ortho, int use_x0)
+{
+    phg_gmres_opts o;
+    phg_gmres_default_opts(&o);
+    o.restart = 30;
+    o.rtol = rtol;
+    o.ortho = ortho;
+    o.precision = precision;
+    o.inner_rtol = 1e-4;
+    o.use_x0 = use_x0;
+    o.max_iters = 20000;
+    return o;
+}
+
+int count_iters(const std::vector<phg_gmres_info>& inf, int S, std::vector<int>& acc, int stride)
+{
+    for (int s = 0; s < S; ++s)
+        for (int k = 0; k < stride; ++k) acc[s] += inf[(size_t)s * stride + k].iterations;
+    return 0;
+}
+
+
+// Ub[o][s] = u[outer[o]][s]
+__global__ void bv_outer(int os, int S, const int* __restrict__ outer, const double* __restrict__ u,
+                         double* __restrict__ Ub, const int* __restrict__ mask)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)os * S) return;
+    const int o = (int)(e / S), s = (int)(e % S);
+    Ub[e] = mask[s] ? u[(size_t)outer[o] * S + s] : 0.0;
+}
+
+__global__ void bv_set_cols(size_t rows, int S, double v, double* __restrict__ x,
+                            const int* __restrict__ mask)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * S) return;
+    if (mask[e % S]) x[e] = v;
+}
+
+enum { SLOT_FREE = 0, SLOT_NEW = 1, SLOT_RUN = 2, SLOT_DONE = 3 };
+
+void zero_cols(phg_barvortex_s* B, size_t rows, double* x, hipStream_t s)
+{
+    hipLaunchKernelGGL(bv_zero_cols, g256(rows * B->S), dim3(BV_BS), 0, s, rows, B->S, x,
+                       B->dmask.p);
+}
+
+int batched(phg_barvortex_s* B, phg_matrix_s* A, const double* rhs, double* x, double rtol,
+            int precision, int ortho, int use_x0, phg_mg_s* pc, std::vector<int>& acc,
+            hipStream_t s)
+{
+    phg_gmres_opts o = solve_opts(rtol, precision, ortho, use_x0);
+    std::vector<phg_gmres_info> inf(B->S);
+    const int rc = gmres_batch_solve(A, B->S, rhs, x, &o, inf.data(), s, pc);
+    if (rc < 0) return rc;
+    count_iters(inf, B->S, acc, 1);
+    return PHG_OK;
+}
+
+// SphereLaplace::calc1 (src/impl/slaplace_impl.h:153-192) for the masked
+// slots: out = p2u(idt^-1 (laplace u_in + bnd3 u_bnd), 0).
+int calc1(phg_barvortex_s* B, const double* u, double* out, const phg_bv_opts& o,
+          std::vector<int>& acc, hipStream_t s)
+{
+    const int S = B->S, n = B->n;
+    phg_matrix_t lap = nullptr;
+    phg_mg_matrix(B->mg, &lap);
+    hipLaunchKernelGGL(bv_u2p, g256((size_t)n * S), dim3(BV_BS), 0, s, n, S, B->mesh->d_inner.p,
+                       u, B->Y.p, B->dmask.p);
+    if (B->os)
+        hipLaunchKernelGGL(bv_outer, g256((size_t)B->os * S), dim3(BV_BS), 0, s, B->os, S,
+                           B->mesh->d_outer.p, u, B->Ub.p, B->dmask.p);
+    spmm(B, lap, B->Y.p, B->Q.p, 0, s);
+    if (B->os) spmm(B, B->bnd3, B->Ub.p, B->Q.p, 1, s);
+    zero_cols(B, n, B->Q.p, s);
+    PHG_HIP(hipGetLastError());
+    int rc = batched(B, B->idt, B->Q.p, B->X.p, o.rtol_mass, o.precision, PHG_CGS1, 0, nullptr,
+                     acc, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(bv_p2u, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
+                       B->mesh->d_isb.p, B->mesh->d_p2io.p, B->X.p, out, B->dmask.p);
+    PHG_HIP(hipGetLastError());
+    return PHG_OK;
+}
+
+// Start of BarVortex::calc for the slots in `mask` (barvortex_impl.h:175-245).
+int init_slots(phg_barvortex_s* B, const std::vector<int>& mask, const phg_bv_opts& o,
+               hipStream_t s)
+{
+    const int S = B->S;
+    const size_t V = B->V;
+    const phg_bv_conf& c = B->conf;
+    int rc = upload_mask(B, mask, s);
+    if (rc) return rc;
+    rc = calc1(B, B->U.p, B->W.p, o, B->solver_iters, s);       // w = L(u)
+    if (rc) return rc;
+    rc = calc1(B, B->W.p, B->T1.p, o, B->solver_iters, s);      // dw = L(w)
+    if (rc) return rc;
+    // FC = w / tau + mu (1 - theta) dw;  FC = FC - sigma (1 - theta) w
+    hipLaunchKernelGGL(bv_axpby, g256(V * S), dim3(BV_BS), 0, s, V, S, 1.0 / c.tau, B->W.p,
+                       c.mu * (1.0 - c.theta), B->T1.p, B->FC.p, B->dmask.p, 0);
+    hipLaunchKernelGGL(bv_axpby, g256(V * S), dim3(BV_BS), 0, s, V, S, 1.0, B->FC.p,
+                       -c.sigma * (1.0 - c.theta), B->W.p, B->FC.p, B->dmask.p, 0);
+    hipLaunchKernelGGL(bv_set_cols, g256(V * S), dim3(BV_BS), 0, s, V, S, 0.0, B->Un.p,
+                       B->dmask.p);
+    hipLaunchKernelGGL(bv_set_cols, g256(V * S), dim3(BV_BS), 0, s, V, S, 0.0, B->Wn.p,
+                       B->dmask.p);
+    PHG_HIP(hipGetLastError());
+    std::vector<double> nr;
+    rc = norms(B, B->U.p, nullptr, nr, s);                       // nr0 = ||u||
+    if (rc) return rc;
+    for (int k = 0; k < S; ++k)
+        if (mask[k]) {
+            B->nr0[k] = nr[k];
+            B->iters[k] = 0;
+            B->rel[k] = 1.0;
+        }
+    return PHG_OK;
+}
+
+// One pass of the fixed point (barvortex_impl.h:250-305) for the slots in mask.
+int iterate_slots(phg_barvortex_s* B, const std::vector<int>& mask, const phg_bv_opts& o,
+                  std::vector<double>& nr, hipStream_t s)
+{
+    const int S = B->S, n = B->n;
+    const size_t V = B->V;
+    const phg_bv_conf& c = B->conf;
+    int rc = upload_mask(B, mask, s);
+    if (rc) return rc;
+    // tmp1 = k1 theta w_n + k1 (1 - theta) w + k2 lh;  tmp2 = theta u_n + (1 - theta) u
+    hipLaunchKernelGGL(bv_axpby, g256(V * S), dim3(BV_BS), 0, s, V, S, c.k1 * c.theta, B->Wn.p,
+                       c.k1 * (1.0 - c.theta), B->W.p, B->T1.p, B->dmask.p, 1);
+    hipLaunchKernelGGL(bv_add_shared, g256(V * S), dim3(BV_BS), 0, s, B->V, S, B->T1.p, c.k2,
+                       B->lh.p, B->T1.p, B->dmask.p);
+    hipLaunchKernelGGL(bv_axpby, g256(V * S), dim3(BV_BS), 0, s, V, S, c.theta, B->Un.p,
+                       1.0 - c.theta, B->U.p, B->T2.p, B->dmask.p, 1);
+    PHG_HIP(hipGetLastError());
+    // jac = J(tmp2, tmp1): 8 mass solves per slot, one batched solve
+    {
+        phg_gmres_opts jo = solve_opts(o.rtol_mass, o.precision, PHG_CGS1, 0);
+        jo.restart = 32;
+        std::vector<phg_gmres_info> inf((size_t)8 * S);
+        rc = jacobian_calc2_strided(B->J, S, B->T2.p, B->T1.p, S, 1, B->JAC.p, S, 1, 1.0, &jo,
+                                    inf.data(), s);
+        if (rc < 0) return rc;
+        count_iters(inf, S, B->solver_iters, 8);
+    }
+    // F = FC_inner - jac;  rp = idt F
+    hipLaunchKernelGGL(bv_rhs, g256((size_t)n * S), dim3(BV_BS), 0, s, n, S, B->mesh->d_inner.p,
+                       B->FC.p, B->JAC.p, B->F.p, B->dmask.p);
+    spmm(B, B->idt, B->F.p, B->RP.p, 0, s);
+    zero_cols(B, n, B->RP.p, s);
+    // w_n = p2u(A^-1 rp), starting from the current w_n
+    hipLaunchKernelGGL(bv_u2p, g256((size_t)n * S), dim3(BV_BS), 0, s, n, S, B->mesh->d_inner.p,
+                       B->Wn.p, B->X.p, B->dmask.p);
+    PHG_HIP(hipGetLastError());
+    rc = batched(B, B->A, B->RP.p, B->X.p, o.rtol_helmholtz, o.precision, PHG_CGS2, 1, nullptr,
+                 B->solver_iters, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(bv_p2u, g256(V * S), dim3(BV_BS), 0, s, B->V, S, B->mesh->d_isb.p,
+                       B->mesh->d_p2io.p, B->X.p, B->Wn.p, B->dmask.p);
+    // u_n1 = p2u(laplace^-1 (idt u2p(w_n))), starting from the current u_n
+    hipLaunchKernelGGL(bv_u2p, g256((size_t)n * S), dim3(BV_BS), 0, s, n, S, B->mesh->d_inner.p,
+                       B->Wn.p, B->Y.p, B->dmask.p);
+    spmm(B, B->idt, B->Y.p, B->Q.p, 0, s);
+    zero_cols(B, n, B->Q.p, s);
+    hipLaunchKernelGGL(bv_u2p, g256((size_t)n * S), dim3(BV_BS), 0, s, n, S, B->mesh->d_inner.p,
+                       B->Un.p, B->Z.p, B->dmask.p);
+    PHG_HIP(hipGetLastError());
+    phg_matrix_t lap = nullptr;
+    phg_mg_matrix(B->mg, &lap);
+    rc = batched(B, lap, B->Q.p, B->Z.p, o.rtol_poisson, PHG_F64, PHG_CGS2, 1, B->mg,
+                 B->solver_iters, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(bv_p2u, g256(V * S), dim3(BV_BS), 0, s, B->V, S, B->mesh->d_isb.p,
+                       B->mesh->d_p2io.p, B->Z.p, B->Un1.p, B->dmask.p);
+    PHG_HIP(hipGetLastError());
+    // nr = ||u_n1 - u_n||, then u_n = u_n1
+    rc = norms(B, B->Un1.p, B->Un.p, nr, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(bv_copy_cols, g256(V * S), dim3(BV_BS), 0, s, V, S, B->Un1.p, B->Un.p,
+                       B->dmask.p);
+    PHG_HIP(hipGetLastError());
+    return PHG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void phg_bv_default_opts(phg_bv_opts* o)
+{
+    o->max_iters = 1000;
+    o->tol = 1e-14;
+    o->rtol_mass = 1e-13;
+    o->rtol_helmholtz = 1e-12;
+    o->rtol_poisson = 1e-12;
+    o->precision = PHG_F64;
+}
+
+size_t phg_barvortex_estimate_bytes(int n_points, int slots)
+{
+    // matrices: ~7 entries per row, 12 B each (value + column), for idt, A,
+    // laplace + V-cycle levels (4/3), 5 Jacobian matrices, the full mass
+    // and the ELL copies; Krylov bases: Jacobian (33 x 8S), Helmholtz (31 x S),
+    // Poisson (31 x S + 30 x S preconditioned), L() mass (31 x S); slot
+    // vectors: 8 full + 7 inner + V-cycle levels.
+    const double n = (double)n_points, S = (double)(slots < 1 ? 1 : slots);
+    const int kp8 = 8 * slots <= 8 ? 8 : 8 * slots <= 16 ? 16 : 8 * slots <= 32 ? 32 : 64;
+    double kp = 1;
+    while (kp < S) kp *= 2;
+    const double mats = n * 7 * 12 * (2 + 4.0 / 3 + 5 + 1) * 2;
+    const double bases = n * 8 * (33.0 * kp8 + 31 * kp + 61 * kp + 31 * kp + 8 * kp8);
+    const double vecs = n * 8 * S * (8 + 7 + 4 * 4.0 / 3);
+    return (size_t)(mats + bases + vecs);
+}
+
+int phg_barvortex_create(phg_mesh_t m, const phg_bv_conf* conf, const double* d_lh, int slots,
+                         phg_stream_t stream, phg_barvortex_t* out)
+{
+    if (!m || !conf || !d_lh || !out || slots < 1 || slots > BV_MAXS ||
+        (slots & (slots - 1)) != 0) {
+        set_error("phg_barvortex_create: bad argument (slots must be 1, 2, 4 or 8)");
+        return PHG_ERR_ARG;
+    }
+    if (!(conf->tau > 0.0)) {
+        set_error("phg_barvortex_create: tau must be positive");
+        return PHG_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    std::unique_ptr<phg_barvortex_s> B(new phg_barvortex_s());
+    B->mesh = m;
+    B->conf = *conf;
+    B->S = slots;
+    B->V = m->h.n_points;
+    B->n = (int)m->h.inner.size();
+    B->os = (int)m->h.outer.size();
+    const phg_bv_conf& c = *conf;
+    int rc = assemble_device(m, PHG_MASS, 1.0, 0.0, s, &B->idt);
+    if (rc) return rc;
+    rc = assemble_boundary_device(m, PHG_LAPLACE, 1.0, 0.0, s, &B->bnd3);
+    if (rc) return rc;
+    // A_ = (1/tau + sigma theta) M - theta mu L  (bv_private::integrate_cb)
+    rc = assemble_device(m, PHG_HELMHOLTZ, 1.0 / c.tau + c.sigma * c.theta, -c.theta * c.mu, s,
+                         &B->A);
+    if (rc) return rc;
+    rc = assemble_device(m, PHG_MASS, 1.0, 0.0, s, &B->norm, true);
+    if (rc) return rc;
+    rc = phg_jacobian_create(m, stream, &B->J);
+    if (rc) return rc;
+    rc = phg_mg_create(m, PHG_LAPLACE, 1.0, 0.0, stream, &B->mg);
+    if (rc) return rc;
+    const size_t V = B->V, n = B->n, S = slots;
+    PHG_HIP(B->lh.alloc(V));
+    PHG_HIP(hipMemcpyAsync(B->lh.p, d_lh, V * sizeof(double), hipMemcpyDeviceToDevice, s));
+    DevBuf<double>* full[] = {&B->U, &B->W, &B->FC, &B->Un, &B->Wn, &B->Un1, &B->T1, &B->T2};
+    for (auto* b : full) {
+        PHG_HIP(b->alloc(V * S));
+        PHG_HIP(hipMemsetAsync(b->p, 0, V * S * sizeof(double), s));
+    }
+    DevBuf<double>* inner[] = {&B->JAC, &B->F, &B->RP, &B->X, &B->Y, &B->Z, &B->Q};
+    for (auto* b : inner) {
+        PHG_HIP(b->alloc(n * S));
+        PHG_HIP(hipMemsetAsync(b->p, 0, n * S * sizeof(double), s));
+    }
+    PHG_HIP(B->Ub.alloc(std::max<size_t>(B->os, 1) * S));
+    B->G = (int)((V + (BV_BS / S) - 1) / (BV_BS / S));
+    PHG_HIP(B->part.alloc((size_t)B->G * S));
+    PHG_HIP(B->nrm.alloc(S));
+    PHG_HIP(B->dmask.alloc(S));
+    B->status.assign(S, SLOT_FREE);
+    B->iters.assign(S, 0);
+    B->solver_iters.assign(S, 0);
+    B->nr0.assign(S, 0.0);
+    B->rel.assign(S, 0.0);
+    PHG_HIP(hipStreamSynchronize(s));
+    *out = B.release();
+    return PHG_OK;
+}
+
+void phg_barvortex_destroy(phg_barvortex_t B) { delete B; }
+
+int phg_barvortex_slots(phg_barvortex_t B) { return B ? B->S : 0; }
+
+int phg_barvortex_load(phg_barvortex_t B, int slot, const double* d_u, phg_stream_t stream)
+{
+    if (!B || !d_u || slot < 0 || slot >= B->S || B->status[slot] != SLOT_FREE) {
+        set_error("phg_barvortex_load: bad argument or slot not free");
+        return PHG_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(bv_col_in, g256(B->V), dim3(BV_BS), 0, s, B->V, B->S, slot, d_u, B->U.p);
+    PHG_HIP(hipGetLastError());
+    B->status[slot] = SLOT_NEW;
+    B->solver_iters[slot] = 0;
+    return PHG_OK;
+}
+
+int phg_barvortex_iterate(phg_barvortex_t B, const phg_bv_opts* opts, int* state,
+                          phg_bv_info* infos, phg_stream_t stream)
+{
+    if (!B) {
+        set_error("phg_barvortex_iterate: null argument");
+        return PHG_ERR_ARG;
+    }
+    phg_bv_opts o;
+    if (opts) o = *opts;
+    else phg_bv_default_opts(&o);
+    hipStream_t s = (hipStream_t)stream;
+    const int S = B->S;
+    std::vector<int> fresh(S, 0), run(S, 0);
+    bool any_new = false, any_run = false;
+    for (int k = 0; k < S; ++k) {
+        fresh[k] = B->status[k] == SLOT_NEW;
+        any_new = any_new || fresh[k];
+    }
+    int rc;
+    if (any_new) {
+        rc = init_slots(B, fresh, o, s);
+        if (rc) return rc;
+        for (int k = 0; k < S; ++k)
+            if (fresh[k]) B->status[k] = SLOT_RUN;
+    }
+    for (int k = 0; k < S; ++k) {
+        run[k] = B->status[k] == SLOT_RUN;
+        any_run = any_run || run[k];
+    }
+    if (any_run) {
+        std::vector<double> nr;
+        rc = iterate_slots(B, run, o, nr, s);
+        if (rc) return rc;
+        for (int k = 0; k < S; ++k) {
+            if (!run[k]) continue;
+            B->iters[k] += 1;
+            B->rel[k] = B->nr0[k] > 0.0 ? nr[k] / B->nr0[k] : nr[k];
+            if (B->rel[k] < o.tol || std::isnan(nr[k]) || B->iters[k] >= o.max_iters)
+                B->status[k] = SLOT_DONE;
+        }
+    }
+    for (int k = 0; k < S; ++k) {
+        if (state) state[k] = B->status[k] == SLOT_FREE ? 0 : B->status[k] == SLOT_DONE ? 2 : 1;
+        if (infos) {
+            infos[k].iterations = B->iters[k];
+            infos[k].rel_change = B->rel[k];
+            infos[k].converged = B->status[k] == SLOT_DONE && B->rel[k] < o.tol;
+            infos[k].solver_iterations = B->solver_iters[k];
+        }
+    }
+    return PHG_OK;
+}
+
+int phg_barvortex_store(phg_barvortex_t B, int slot, double* d_u1, phg_stream_t stream)
+{
+    if (!B || !d_u1 || slot < 0 || slot >= B->S || B->status[slot] != SLOT_DONE) {
+        set_error("phg_barvortex_store: bad argument or slot not done");
+        return PHG_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(bv_col_out, g256(B->V), dim3(BV_BS), 0, s, B->V, B->S, slot, B->Un.p, d_u1);
+    PHG_HIP(hipGetLastError());
+    B->status[slot] = SLOT_FREE;
+    return PHG_OK;
+}
+
+int phg_barvortex_calc(phg_barvortex_t B, int nmembers, const double* d_u, double* d_u1,
+                       const phg_bv_opts* opts, phg_bv_info* infos, phg_stream_t stream)
+{
+    if (!B || !d_u || !d_u1 || nmembers < 1 || nmembers > B->S) {
+        set_error("phg_barvortex_calc: bad argument (1 <= nmembers <= slots)");
+        return PHG_ERR_ARG;
+    }
+    for (int k = 0; k < B->S; ++k)
+        if (B->status[k] != SLOT_FREE) {
+            set_error("phg_barvortex_calc: slots in use");
+            return PHG_ERR_ARG;
+        }
+    int rc;
+    for (int k = 0; k < nmembers; ++k) {
+        rc = phg_barvortex_load(B, k, d_u + (size_t)k * B->V, stream);
+        if (rc) return rc;
+    }
+    std::vector<int> state(B->S);
+    std::vector<phg_bv_info> inf(B->S);
+    for (;;) {
+        rc = phg_barvortex_iterate(B, opts, state.data(), inf.data(), stream);
+        if (rc) return rc;
+        bool busy = false;
+        for (int k = 0; k < nmembers; ++k) busy = busy || state[k] == 1;
+        if (!busy) break;
+    }
+    bool all = true;
+    for (int k = 0; k < nmembers; ++k) {
+        rc = phg_barvortex_store(B, k, d_u1 + (size_t)k * B->V, stream);
+        if (rc) return rc;
+        if (infos) infos[k] = inf[k];
+        all = all && inf[k].converged;
+    }
+    PHG_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return all ? PHG_OK : PHG_ERR_NOCONV;
+}
+
+}  // extern "C"

@@ -13,10 +13,6 @@ namespace phg {
 static thread_local std::string g_error;
 void set_error(const std::string& msg) { g_error = msg; }
 
-int assemble_device(phg_mesh_s* M, int kind, double c_mass, double c_lap, hipStream_t stream,
-                    phg_matrix_s** out);
-int assemble_boundary_device(phg_mesh_s* M, int kind, double c_mass, double c_lap,
-                             hipStream_t stream, phg_matrix_s** out);
 int spmv_f64(phg_matrix_s* A, const double* x, double* y, hipStream_t s);
 int spmv_f32(phg_matrix_s* A, int prec, const float* x, float* y, hipStream_t s);
 int spmm_f32(phg_matrix_s* A, int prec, int nrhs, const float* X, float* Y, hipStream_t s);
@@ -258,6 +254,16 @@ int phg_assemble_boundary(phg_mesh_t m, int kind, double c_mass, double c_lap, p
         return PHG_ERR_ARG;
     }
     return assemble_boundary_device(m, kind, c_mass, c_lap, nullptr, out);
+}
+
+int phg_assemble_full(phg_mesh_t m, int kind, double c_mass, double c_lap, phg_stream_t stream,
+                      phg_matrix_t* out)
+{
+    if (!m || !out || kind < 0 || kind > PHG_DIFF2_COS) {
+        set_error("phg_assemble_full: bad argument");
+        return PHG_ERR_ARG;
+    }
+    return assemble_device(m, kind, c_mass, c_lap, (hipStream_t)stream, out, true);
 }
 
 int phg_matrix_info(phg_matrix_t A, int* n_rows, int* n_cols, long* nnz)

@@ -21,13 +21,6 @@
 #include "ell.h"
 #include "phg_internal.h"
 
-namespace phg {
-int assemble_device(phg_mesh_s* M, int kind, double c_mass, double c_lap, hipStream_t stream,
-                    phg_matrix_s** out);
-int assemble_boundary_device(phg_mesh_s* M, int kind, double c_mass, double c_lap,
-                             hipStream_t stream, phg_matrix_s** out);
-}  // namespace phg
-
 using namespace phg;
 
 constexpr int JK_MAXPAIRS = 8;
@@ -65,11 +58,12 @@ __global__ void jk_interleave(int n, int width, const double* __restrict__ v0,
     out[e * 4 + 3] = v3[e];
 }
 
-// Uin[j][2k + c] = (c ? v_k : u_k)[inner[j]], Ubnd[o][2k + c] likewise at outer[o].
-__global__ void jk_gather(int n, int os, int V, int np, const int* __restrict__ inner,
-                          const int* __restrict__ outer, const double* __restrict__ u,
-                          const double* __restrict__ v, double* __restrict__ Uin,
-                          double* __restrict__ Ubnd)
+// Uin[j][2k + c] = (c ? v_k : u_k)[inner[j]], Ubnd[o][2k + c] likewise at
+// outer[o]; point p of pair k is read at p * su + k * sk.
+__global__ void jk_gather(int n, int os, int np, size_t su, size_t sk,
+                          const int* __restrict__ inner, const int* __restrict__ outer,
+                          const double* __restrict__ u, const double* __restrict__ v,
+                          double* __restrict__ Uin, double* __restrict__ Ubnd)
 {
     const int P = 2 * np;
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -77,12 +71,12 @@ __global__ void jk_gather(int n, int os, int V, int np, const int* __restrict__ 
     if (e < tot_in) {
         const int j = (int)(e / P), p = (int)(e % P);
         const double* src = (p & 1) ? v : u;
-        Uin[e] = src[(size_t)(p >> 1) * V + inner[j]];
+        Uin[e] = src[(size_t)inner[j] * su + (size_t)(p >> 1) * sk];
     } else if (e < tot_in + (size_t)os * P) {
         const size_t f = e - tot_in;
         const int o = (int)(f / P), p = (int)(f % P);
         const double* src = (p & 1) ? v : u;
-        Ubnd[f] = src[(size_t)(p >> 1) * V + outer[o]];
+        Ubnd[f] = src[(size_t)outer[o] * su + (size_t)(p >> 1) * sk];
     }
 }
 
@@ -125,9 +119,10 @@ jk_rhs(int n, int width, int np, const int* __restrict__ col, const double* __re
     for (int op = 0; op < 4; ++op) out[op] = b[op] + s[op];
 }
 
-// ans_k[i] = 0.25 * (((c1 + c2) + c3) + c4) * sign (sjacobian.cpp:136-362).
+// ans_k[i] = 0.25 * (((c1 + c2) + c3) + c4) * sign (sjacobian.cpp:136-362),
+// stored at i * ai + k * ak.
 __global__ void jk_combine(int n, int np, const double* __restrict__ X, double sign,
-                           double* __restrict__ ans)
+                           double* __restrict__ ans, size_t ai, size_t ak)
 {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)n * np) return;
@@ -145,7 +140,7 @@ __global__ void jk_combine(int n, int np, const double* __restrict__ X, double s
     a = a + c4;
     a = a * 0.25;
     if (sign < 0.0) a = a * -1.0;
-    ans[(size_t)k * n + i] = a;
+    ans[(size_t)i * ai + (size_t)k * ak] = a;
 }
 
 bool same_structure(const phg_matrix_s* a, const phg_matrix_s* b)
@@ -155,6 +150,49 @@ bool same_structure(const phg_matrix_s* a, const phg_matrix_s* b)
 }
 
 }  // namespace
+
+namespace phg {
+
+// calc2 (sign 1) or calc2t (sign -1) of np pairs; point p of pair k is read
+// at p * su + k * sk of u and v, J(u_k, v_k) at inner point i is stored at
+// i * ai + k * ak of ans.
+int jacobian_calc2_strided(phg_jacobian_s* J, int np, const double* d_u, const double* d_v,
+                           size_t su, size_t sk, double* d_ans, size_t ai, size_t ak,
+                           double sign, const phg_gmres_opts* opts, phg_gmres_info* infos,
+                           hipStream_t s)
+{
+    if (!J || !d_u || !d_v || !d_ans || np < 1 || np > JK_MAXPAIRS) {
+        set_error("phg_jacobian_calc2: bad argument (1 <= npairs <= 8)");
+        return PHG_ERR_ARG;
+    }
+    const int n = J->n, os = J->os, P = 2 * np;
+    phg_gmres_opts o;
+    if (opts) o = *opts;
+    else {
+        phg_gmres_default_opts(&o);
+        o.ortho = PHG_CGS1;
+        o.restart = 32;
+        o.rtol = 1e-13;
+    }
+    const size_t gat = (size_t)(n + os) * P;
+    hipLaunchKernelGGL(jk_gather, dim3((unsigned)((gat + 255) / 256)), dim3(256), 0, s, n, os, np,
+                       su, sk, J->mesh->d_inner.p, J->mesh->d_outer.p, d_u, d_v, J->Uin.p,
+                       J->Ubnd.p);
+    const size_t tr = (size_t)n * P;
+    hipLaunchKernelGGL(jk_rhs, dim3((unsigned)((tr + 255) / 256)), dim3(256), 0, s, n, J->width,
+                       np, J->col.p, J->val4.p, J->brp.p, J->bci.p, J->bval4.p, J->Uin.p,
+                       J->Ubnd.p, J->B.p);
+    PHG_HIP(hipGetLastError());
+    int rc = gmres_batch_solve(J->idt, 8 * np, J->B.p, J->X.p, &o, infos, s);
+    if (rc < 0) return rc;
+    const size_t tc = (size_t)n * np;
+    hipLaunchKernelGGL(jk_combine, dim3((unsigned)((tc + 255) / 256)), dim3(256), 0, s, n, np,
+                       J->X.p, sign, d_ans, ai, ak);
+    PHG_HIP(hipGetLastError());
+    return rc;
+}
+
+}  // namespace phg
 
 extern "C" {
 
@@ -231,53 +269,22 @@ int phg_jacobian_matrix(phg_jacobian_t J, int which, phg_matrix_t* out)
     return PHG_OK;
 }
 
-static int jacobian_run(phg_jacobian_t J, int np, const double* d_u, const double* d_v,
-                        double* d_ans, double sign, const phg_gmres_opts* opts,
-                        phg_gmres_info* infos, hipStream_t s)
-{
-    if (!J || !d_u || !d_v || !d_ans || np < 1 || np > JK_MAXPAIRS) {
-        set_error("phg_jacobian_calc2: bad argument (1 <= npairs <= 8)");
-        return PHG_ERR_ARG;
-    }
-    const int n = J->n, os = J->os, P = 2 * np;
-    const int V = (int)J->mesh->h.is_boundary.size();
-    phg_gmres_opts o;
-    if (opts) o = *opts;
-    else {
-        phg_gmres_default_opts(&o);
-        o.ortho = PHG_CGS1;
-        o.restart = 32;
-        o.rtol = 1e-13;
-    }
-    const size_t gat = (size_t)(n + os) * P;
-    hipLaunchKernelGGL(jk_gather, dim3((unsigned)((gat + 255) / 256)), dim3(256), 0, s, n, os, V,
-                       np, J->mesh->d_inner.p, J->mesh->d_outer.p, d_u, d_v, J->Uin.p, J->Ubnd.p);
-    const size_t tr = (size_t)n * P;
-    hipLaunchKernelGGL(jk_rhs, dim3((unsigned)((tr + 255) / 256)), dim3(256), 0, s, n, J->width,
-                       np, J->col.p, J->val4.p, J->brp.p, J->bci.p, J->bval4.p, J->Uin.p,
-                       J->Ubnd.p, J->B.p);
-    PHG_HIP(hipGetLastError());
-    int rc = gmres_batch_solve(J->idt, 8 * np, J->B.p, J->X.p, &o, infos, s);
-    if (rc < 0) return rc;
-    const size_t tc = (size_t)n * np;
-    hipLaunchKernelGGL(jk_combine, dim3((unsigned)((tc + 255) / 256)), dim3(256), 0, s, n, np,
-                       J->X.p, sign, d_ans);
-    PHG_HIP(hipGetLastError());
-    return rc;
-}
-
 int phg_jacobian_calc2(phg_jacobian_t J, int npairs, const double* d_u, const double* d_v,
                        double* d_ans, const phg_gmres_opts* opts, phg_gmres_info* infos,
                        phg_stream_t s)
 {
-    return jacobian_run(J, npairs, d_u, d_v, d_ans, 1.0, opts, infos, (hipStream_t)s);
+    const size_t V = J ? J->mesh->h.is_boundary.size() : 0, n = J ? J->n : 0;
+    return jacobian_calc2_strided(J, npairs, d_u, d_v, 1, V, d_ans, 1, n, 1.0, opts, infos,
+                                  (hipStream_t)s);
 }
 
 int phg_jacobian_calc2t(phg_jacobian_t J, int npairs, const double* d_u, const double* d_v,
                         double* d_ans, const phg_gmres_opts* opts, phg_gmres_info* infos,
                         phg_stream_t s)
 {
-    return jacobian_run(J, npairs, d_u, d_v, d_ans, -1.0, opts, infos, (hipStream_t)s);
+    const size_t V = J ? J->mesh->h.is_boundary.size() : 0, n = J ? J->n : 0;
+    return jacobian_calc2_strided(J, npairs, d_u, d_v, 1, V, d_ans, 1, n, -1.0, opts, infos,
+                                  (hipStream_t)s);
 }
 
 }  // extern "C"

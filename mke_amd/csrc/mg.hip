@@ -30,11 +30,6 @@
 #include "ell.h"
 #include "phg_internal.h"
 
-namespace phg {
-int assemble_device(phg_mesh_s* M, int kind, double c_mass, double c_lap, hipStream_t stream,
-                    phg_matrix_s** out);
-}  // namespace phg
-
 using namespace phg;
 
 namespace {

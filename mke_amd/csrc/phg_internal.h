@@ -101,6 +101,12 @@ struct phg_matrix_s {
 };
 
 namespace phg {
+// assembly (assemble.hip): inner x inner operator, or every point x every
+// point (generate_full_matrix) when full; the inner x outer boundary matrix
+int assemble_device(phg_mesh_s* M, int kind, double c_mass, double c_lap, hipStream_t stream,
+                    phg_matrix_s** out, bool full = false);
+int assemble_boundary_device(phg_mesh_s* M, int kind, double c_mass, double c_lap,
+                             hipStream_t stream, phg_matrix_s** out);
 // multigrid preconditioner (mg.hip)
 int mg_reserve(phg_mg_s* M, int kp);
 void mg_apply(phg_mg_s* M, int KP, const double* v, double* z, hipStream_t s);

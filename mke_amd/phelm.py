@@ -80,6 +80,16 @@ SIGNATURES = {
     "phg_mg_matrix": (_I, [_P, C.POINTER(_P)]),
     "phg_mg_levels": (_I, [_P]),
     "phg_gmres_batch_mg": (_I, [_P, _I, _P, _P, C.POINTER(GmresOpts), C.POINTER(GmresInfo), _P]),
+    "phg_assemble_full": (_I, [_P, _I, _D, _D, _P, C.POINTER(_P)]),
+    "phg_bv_default_opts": (None, [_P]),
+    "phg_barvortex_estimate_bytes": (C.c_size_t, [_I, _I]),
+    "phg_barvortex_create": (_I, [_P, _P, _P, _I, _P, C.POINTER(_P)]),
+    "phg_barvortex_destroy": (None, [_P]),
+    "phg_barvortex_slots": (_I, [_P]),
+    "phg_barvortex_calc": (_I, [_P, _I, _P, _P, _P, _P, _P]),
+    "phg_barvortex_load": (_I, [_P, _I, _P, _P]),
+    "phg_barvortex_iterate": (_I, [_P, _P, _P, _P, _P]),
+    "phg_barvortex_store": (_I, [_P, _I, _P, _P]),
     "phg_jacobian_create": (_I, [_P, _P, C.POINTER(_P)]),
     "phg_jacobian_destroy": (None, [_P]),
     "phg_jacobian_matrix": (_I, [_P, _I, C.POINTER(_P)]),
@@ -228,6 +238,15 @@ class Matrix:
         h = C.c_void_p()
         check(lib().phg_assemble(mesh.h, kind, c_mass, c_lap, _stream(stream), C.byref(h)),
               "phg_assemble")
+        return cls(h)
+
+    @classmethod
+    def assemble_full(cls, mesh, kind, c_mass=1.0, c_lap=0.0, stream=None):
+        """generate_full_matrix: every point x every point (SphereNorm's matrix
+        with MASS)."""
+        h = C.c_void_p()
+        check(lib().phg_assemble_full(mesh.h, kind, c_mass, c_lap, _stream(stream), C.byref(h)),
+              "phg_assemble_full")
         return cls(h)
 
     @classmethod
@@ -399,6 +418,82 @@ class Multigrid:
     def close(self):
         if self.h:
             lib().phg_mg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BvConf(C.Structure):
+    _fields_ = [("tau", _D), ("sigma", _D), ("mu", _D), ("k1", _D), ("k2", _D),
+                ("theta", _D)]
+
+
+class BvOpts(C.Structure):
+    _fields_ = [("max_iters", _I), ("tol", _D), ("rtol_mass", _D), ("rtol_helmholtz", _D),
+                ("rtol_poisson", _D), ("precision", _I)]
+
+
+class BvInfo(C.Structure):
+    _fields_ = [("iterations", _I), ("rel_change", _D), ("converged", _I),
+                ("solver_iterations", _I)]
+
+
+def bv_opts(**kw):
+    o = BvOpts()
+    lib().phg_bv_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class BarVortex:
+    """BarVortex (examples/barvortex.h:62-155): the implicit step
+    BarVortex::calc on the GPU for up to `slots` members at once; lh is the
+    Coriolis + orography mesh function (device tensor of n_points values)."""
+
+    def __init__(self, mesh, lh, tau, sigma=0.0, mu=0.0, k1=1.0, k2=1.0, theta=0.5, slots=1,
+                 stream=None):
+        self.m = mesh
+        self.conf = BvConf(tau, sigma, mu, k1, k2, theta)
+        self.h = C.c_void_p()
+        check(lib().phg_barvortex_create(mesh.h, C.byref(self.conf), _ptr(lh), slots,
+                                         _stream(stream), C.byref(self.h)), "phg_barvortex_create")
+        self.slots = slots
+
+    def calc(self, u1, u, opts=None, stream=None):
+        """u, u1: (nmembers, n_points) FP64 device tensors (or (n_points,))."""
+        k = 1 if u.dim() == 1 else int(u.shape[0])
+        infos = (BvInfo * k)()
+        rc = lib().phg_barvortex_calc(self.h, k, _ptr(u), _ptr(u1),
+                                      C.byref(opts) if opts is not None else None, infos,
+                                      _stream(stream))
+        check(rc, "phg_barvortex_calc")
+        return [dict(iterations=i.iterations, rel_change=i.rel_change,
+                     converged=bool(i.converged), solver_iterations=i.solver_iterations)
+                for i in infos]
+
+    def load(self, slot, u, stream=None):
+        check(lib().phg_barvortex_load(self.h, slot, _ptr(u), _stream(stream)), "phg_barvortex_load")
+
+    def iterate(self, opts=None, stream=None):
+        state = (C.c_int * self.slots)()
+        infos = (BvInfo * self.slots)()
+        check(lib().phg_barvortex_iterate(self.h, C.byref(opts) if opts is not None else None,
+                                          state, infos, _stream(stream)), "phg_barvortex_iterate")
+        return list(state), [dict(iterations=i.iterations, rel_change=i.rel_change,
+                                  converged=bool(i.converged)) for i in infos]
+
+    def store(self, slot, u1, stream=None):
+        check(lib().phg_barvortex_store(self.h, slot, _ptr(u1), _stream(stream)),
+              "phg_barvortex_store")
+
+    def close(self):
+        if self.h:
+            lib().phg_barvortex_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -591,6 +591,56 @@ int or_assemble(const or_mesh *m, int kind, double c_mass, double c_lap,
     return 0;
 }
 
+/* generate_full_matrix (src/generators.h:200-234): all points x all points,
+ * row p in the zone of its first triangle, no boundary skipped.  Structure:
+ * row p holds every point of every triangle around p, ascending. */
+long or_full_structure(const or_mesh *m, int *row_ptr, int *col_idx)
+{
+    long nnz = 0;
+    row_ptr[0] = 0;
+    for (int p = 0; p < m->n_points; ++p) {
+        int tmp[64], nt = 0;
+        for (int e = m->adj_ptr[p]; e < m->adj_ptr[p + 1]; ++e)
+            for (int a = 0; a < 3; ++a) {
+                int q = m->tri[3 * m->adj[e] + a], dup = 0;
+                for (int c = 0; c < nt; ++c) dup |= tmp[c] == q;
+                if (!dup && nt < 64) tmp[nt++] = q;
+            }
+        for (int a = 1; a < nt; ++a)          /* insertion sort */
+            for (int b = a; b > 0 && tmp[b - 1] > tmp[b]; --b) {
+                int t = tmp[b]; tmp[b] = tmp[b - 1]; tmp[b - 1] = t;
+            }
+        if (col_idx)
+            for (int c = 0; c < nt; ++c) col_idx[nnz + c] = tmp[c];
+        nnz += nt;
+        row_ptr[p + 1] = (int)nnz;
+    }
+    return nnz;
+}
+
+int or_assemble_full(const or_mesh *m, int kind, double c_mass, double c_lap,
+                     const int *row_ptr, const int *col_idx, double *vals)
+{
+    for (int p = 0; p < m->n_points; ++p) {
+        for (int e = row_ptr[p]; e < row_ptr[p + 1]; ++e) vals[e] = 0.0;
+        int zone = m->tri_zone[m->adj[m->adj_ptr[p]]];   /* generators.h:213 */
+        for (int e = m->adj_ptr[p]; e < m->adj_ptr[p + 1]; ++e) {
+            int t = m->adj[e];
+            double xs[3], ys[3];
+            or_poly phik[3];
+            tri_coords(m, t, zone, xs, ys);
+            or_basis_p1(xs, ys, phik);
+            const or_poly *phi_i = &phik[local_index(m, t, p)];
+            for (int i0 = 0; i0 < 3; ++i0) {
+                int p2 = m->tri[3 * t + i0];
+                double a = or_element(kind, phi_i, &phik[i0], xs, ys, c_mass, c_lap);
+                vals[find_pos(row_ptr, col_idx, p, p2)] += a;
+            }
+        }
+    }
+    return 0;
+}
+
 /* boundary matrix rows = inner points touching an outer point */
 long or_boundary_structure(const or_mesh *m, int *row_ptr, int *col_idx)
 {

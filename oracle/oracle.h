@@ -110,6 +110,11 @@ long or_csr_structure(const or_mesh *m, int *row_ptr, int *col_idx);
  * adjacency order (src/generators.cpp:57-114).  Uses OpenMP over rows. */
 int or_assemble(const or_mesh *m, int kind, double c_mass, double c_lap,
                 const int *row_ptr, const int *col_idx, double *vals);
+/* generate_full_matrix (src/generators.h:200-234): every point x every
+ * point, no boundary skipped (SphereNorm's matrix, src/norm.h:71-74) */
+long or_full_structure(const or_mesh *m, int *row_ptr, int *col_idx);
+int or_assemble_full(const or_mesh *m, int kind, double c_mass, double c_lap,
+                     const int *row_ptr, const int *col_idx, double *vals);
 /* inner x outer boundary matrix (src/generators.cpp:141-186) as CSR */
 long or_boundary_structure(const or_mesh *m, int *row_ptr, int *col_idx);
 int or_assemble_boundary(const or_mesh *m, int kind, double c_mass,

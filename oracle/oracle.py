@@ -77,6 +77,9 @@ def lib():
         L.or_boundary_structure.restype = C.c_long
         L.or_boundary_structure.argtypes = [C.POINTER(OrMesh), _PI, _PI]
         L.or_assemble.argtypes = [C.POINTER(OrMesh), _I, _D, _D, _PI, _PI, _PD]
+        L.or_full_structure.restype = C.c_long
+        L.or_full_structure.argtypes = [C.POINTER(OrMesh), _PI, _PI]
+        L.or_assemble_full.argtypes = [C.POINTER(OrMesh), _I, _D, _D, _PI, _PI, _PD]
         L.or_assemble_boundary.argtypes = [C.POINTER(OrMesh), _I, _D, _D, _PI,
                                            _PI, _PD]
         L.or_spmv.argtypes = [_I, _PI, _PI, _PD, _PD, _PD]
@@ -170,6 +173,21 @@ class Mesh:
         vals = np.zeros(len(ci))
         lib().or_assemble(C.byref(self.c), kind, c_mass, c_lap, _pi(rp), _pi(ci),
                           _pd(vals))
+        return rp, ci, vals
+
+    def full_structure(self):
+        V = self.coords.shape[0]
+        rp = np.zeros(V + 1, dtype=np.int32)
+        nnz = lib().or_full_structure(C.byref(self.c), _pi(rp), None)
+        ci = np.zeros(nnz, dtype=np.int32)
+        lib().or_full_structure(C.byref(self.c), _pi(rp), _pi(ci))
+        return rp, ci
+
+    def assemble_full(self, kind, c_mass=1.0, c_lap=0.0):
+        """generate_full_matrix (src/generators.h:200-234), V x V."""
+        rp, ci = self.full_structure()
+        vals = np.zeros(len(ci))
+        lib().or_assemble_full(C.byref(self.c), kind, c_mass, c_lap, _pi(rp), _pi(ci), _pd(vals))
         return rp, ci, vals
 
     def boundary_structure(self):
