@@ -317,6 +317,51 @@ int phg_barvortex_iterate(phg_barvortex_t B, const phg_bv_opts *opts, int *state
                           phg_bv_info *infos, phg_stream_t s);
 int phg_barvortex_store(phg_barvortex_t B, int slot, double *d_u1, phg_stream_t s);
 
+/* ---- serving: implicit-step requests of ensemble members on several mesh
+ *      levels, continuously batched (no counterpart in the reference, which
+ *      steps one BarVortex at a time, examples/test_barvortex.cpp:441-470;
+ *      every request is one BarVortex::calc of its member) --------------- */
+#define PHG_SERVE_MAXLEVELS 8
+typedef struct {
+    int n_levels;
+    int levels[PHG_SERVE_MAXLEVELS];  /* mke_mesh_sphere levels served        */
+    int slots[PHG_SERVE_MAXLEVELS];   /* slots per level (1, 2, 4, 8); 0: the
+                                         largest that fits the budget        */
+    size_t hbm_budget;                /* device bytes for engines + members   */
+    double coriolis;                  /* lh = coriolis * sin(latitude)        */
+    phg_bv_conf conf;
+    phg_bv_opts opts;
+} phg_serve_config;
+typedef struct {
+    int completed;
+    long passes;            /* fixed-point passes run, all levels           */
+    double wall_s, throughput;            /* steps/s over the run           */
+    double p50_latency_s, p99_latency_s, mean_latency_s;  /* finish - arrival */
+    double p50_service_s;   /* finish - admission                           */
+    double mean_occupancy;  /* members per fixed-point pass                 */
+    size_t device_bytes;    /* estimated engine + member-state bytes        */
+} phg_serve_stats;
+typedef struct phg_server_s *phg_server_t;
+
+/* Levels 5-8, budget 200 GiB, BarVortexConf of the ensemble workload in units
+ * of 1/Earth's angular velocity: tau = 600 s, sigma 1.6e-2, mu 8e-5,
+ * k1 = k2 = 1, theta 1/2, lh = 2 sin(latitude); default phg_bv_opts. */
+void phg_serve_default_config(phg_serve_config *c);
+int phg_serve_create(const phg_serve_config *cfg, phg_server_t *out);
+void phg_serve_destroy(phg_server_t S);
+int phg_serve_slots(phg_server_t S, int level);
+/* A member on `level` with initial stream function h_u (host, n_points
+ * values); its state stays on the device.  PHG_ERR_NOMEM over budget. */
+int phg_serve_add_member(phg_server_t S, int level, const double *h_u, int *member_id);
+int phg_serve_member_state(phg_server_t S, int member, double *h_u);
+/* Serves n requests (member[r], released at arrival_s[r] seconds after the
+ * call starts, wall clock) and returns when all are done; start_s/finish_s
+ * (host, n entries) receive admission and completion times, infos (may be
+ * NULL) each step's fixed-point info. */
+int phg_serve_run(phg_server_t S, int n, const int *member, const double *arrival_s,
+                  double *start_s, double *finish_s, phg_bv_info *infos,
+                  phg_serve_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,6 +90,13 @@ SIGNATURES = {
     "phg_barvortex_load": (_I, [_P, _I, _P, _P]),
     "phg_barvortex_iterate": (_I, [_P, _P, _P, _P, _P]),
     "phg_barvortex_store": (_I, [_P, _I, _P, _P]),
+    "phg_serve_default_config": (None, [_P]),
+    "phg_serve_create": (_I, [_P, C.POINTER(_P)]),
+    "phg_serve_destroy": (None, [_P]),
+    "phg_serve_slots": (_I, [_P, _I]),
+    "phg_serve_add_member": (_I, [_P, _I, _PD, _PI]),
+    "phg_serve_member_state": (_I, [_P, _I, _PD]),
+    "phg_serve_run": (_I, [_P, _I, _PI, _PD, _PD, _PD, _P, _P]),
     "phg_jacobian_create": (_I, [_P, _P, C.POINTER(_P)]),
     "phg_jacobian_destroy": (None, [_P]),
     "phg_jacobian_matrix": (_I, [_P, _I, C.POINTER(_P)]),
@@ -494,6 +501,82 @@ class BarVortex:
     def close(self):
         if self.h:
             lib().phg_barvortex_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("n_levels", _I), ("levels", _I * 8), ("slots", _I * 8),
+                ("hbm_budget", C.c_size_t), ("coriolis", _D), ("conf", BvConf),
+                ("opts", BvOpts)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("completed", _I), ("passes", _L), ("wall_s", _D), ("throughput", _D),
+                ("p50_latency_s", _D), ("p99_latency_s", _D), ("mean_latency_s", _D),
+                ("p50_service_s", _D), ("mean_occupancy", _D), ("device_bytes", C.c_size_t)]
+
+
+class Server:
+    """Continuous-batching server of BarVortex implicit steps (phg_serve_*):
+    one engine per mesh level, slots sized by the HBM budget, member states
+    resident on the device."""
+
+    def __init__(self, levels=(5, 6, 7, 8), slots=None, hbm_budget=None, precision=F64,
+                 **conf):
+        c = ServeConfig()
+        lib().phg_serve_default_config(C.byref(c))
+        c.n_levels = len(levels)
+        for i, lv in enumerate(levels):
+            c.levels[i] = lv
+            c.slots[i] = 0 if slots is None else slots[i]
+        if hbm_budget is not None:
+            c.hbm_budget = int(hbm_budget)
+        for k, v in conf.items():
+            setattr(c.conf, k, v)
+        c.opts.precision = precision
+        self.cfg = c
+        self.h = C.c_void_p()
+        check(lib().phg_serve_create(C.byref(c), C.byref(self.h)), "phg_serve_create")
+        self.levels = list(levels)
+
+    def slots(self, level):
+        return lib().phg_serve_slots(self.h, level)
+
+    def add_member(self, level, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        mid = C.c_int()
+        check(lib().phg_serve_add_member(self.h, level, _pd(u), C.byref(mid)),
+              "phg_serve_add_member")
+        return mid.value
+
+    def member_state(self, member, n_points):
+        out = np.zeros(n_points)
+        check(lib().phg_serve_member_state(self.h, member, _pd(out)), "phg_serve_member_state")
+        return out
+
+    def run(self, members, arrivals):
+        n = len(members)
+        m = np.ascontiguousarray(members, dtype=np.int32)
+        a = np.ascontiguousarray(arrivals, dtype=np.float64)
+        start, finish = np.zeros(n), np.zeros(n)
+        infos = (BvInfo * max(n, 1))()
+        st = ServeStats()
+        check(lib().phg_serve_run(self.h, n, _pi(m), _pd(a), _pd(start), _pd(finish), infos,
+                                  C.byref(st)), "phg_serve_run")
+        its = np.array([infos[r].iterations for r in range(n)])
+        conv = np.array([bool(infos[r].converged) for r in range(n)])
+        stats = {f: getattr(st, f) for f, _ in ServeStats._fields_}
+        return start, finish, its, conv, stats
+
+    def close(self):
+        if self.h:
+            lib().phg_serve_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -55,3 +55,20 @@ def oracle_mesh(level):
 def oracle_matrix(level, kind, c_mass=1.0, c_lap=0.0):
     _, M = oracle_mesh(level)
     return M.assemble(kind, c_mass, c_lap)
+
+
+def rh_state(pr, seed=None, noise=1e-3, omega=7.292e-5):
+    """Stream function of the Rossby-Haurwitz wave 4 (Williamson et al. 1992,
+    case 6: omega = K = 7.848e-6 s^-1) in units of Earth's angular velocity,
+    plus, with a seed, a smooth N(0, noise * max|psi|)-scaled perturbation;
+    shifted so that the south pole (the Dirichlet point) is 0."""
+    z = np.clip(pr[:, 2], -1.0, 1.0)
+    phi = np.arcsin(z)
+    lam = np.arctan2(pr[:, 1], pr[:, 0])
+    w = 7.848e-6 / omega
+    psi = -w * np.sin(phi) + w * np.cos(phi) ** 4 * np.sin(phi) * np.cos(4 * lam)
+    if seed is not None:
+        f = real_sph_harm_field(pr, 8, seed)
+        psi = psi + noise * np.abs(psi).max() * f / np.abs(f).max()
+    pole = int(np.argmin(z))
+    return psi - psi[pole]
