@@ -90,6 +90,105 @@ def traffic_from_profiles(kernel_file):
     return d.get("traffic_bytes_per_launch")
 
 
+def serve_leg(n_requests, rate, seed=20251202, verbose=False):
+    """Ensemble serving: 256 members on levels 5/6/7/8 (40/30/20/10 %), each
+    request one BarVortex implicit step (dt = 600 s, Earth-scaled units) of
+    its member; Poisson arrivals at `rate` req/s.  The initial state of a
+    member is the Rossby-Haurwitz wave-4 vorticity plus N(0, 1e-3 max|w|) per
+    vertex, inverted to a stream function by the multigrid Poisson solve.
+    Runs the FP64 and the mixed form; returns one dict per form."""
+    import numpy as np
+    import torch
+
+    import mke_amd as P
+
+    levels, weights, n_members = (5, 6, 7, 8), (0.4, 0.3, 0.2, 0.1), 256
+    # at L8 the reference fixed point amplifies a mode at tau = 600 s (the
+    # change grows x1.7 per two passes from ~1e-6): each request there runs
+    # two BarVortex::calc steps of 300 s
+    substeps = (1, 1, 1, 2)
+    counts = [int(round(w * n_members)) for w in weights]
+    counts[0] += n_members - sum(counts)
+    omega = 7.292e-5
+    wrh = 7.848e-6 / omega
+    rng = np.random.default_rng(seed)
+    init = {}
+    for lv, cnt in zip(levels, counts):
+        mesh = P.Mesh.sphere(lv)
+        a = mesh.arrays()
+        pr, inner = a["pr"], a["inner"]
+        phi = np.arcsin(np.clip(pr[:, 2], -1, 1))
+        lam = np.arctan2(pr[:, 1], pr[:, 0])
+        w0 = 2 * wrh * np.sin(phi) - 30 * wrh * np.sin(phi) * np.cos(phi) ** 4 * np.cos(4 * lam)
+        W = w0[None, :] + rng.standard_normal((cnt, len(w0))) * 1e-3 * np.abs(w0).max()
+        mg = P.Multigrid(mesh, P.LAPLACE)
+        M = P.Matrix.assemble(mesh, P.MASS)
+        psis = []
+        for c0 in range(0, cnt, 64):
+            k = min(64, cnt - c0)
+            Bm = torch.zeros((len(inner), k), dtype=torch.float64, device="cuda")
+            for q in range(k):
+                col = torch.zeros(len(inner), dtype=torch.float64, device="cuda")
+                M.mult_vector(col, torch.as_tensor(W[c0 + q][inner], device="cuda"))
+                Bm[:, q] = col
+            Xm = torch.zeros_like(Bm)
+            mg.solve_batch(Xm, Bm, rtol=1e-10, max_iters=500)
+            for q in range(k):
+                psi = np.zeros(len(w0))
+                psi[inner] = Xm[:, q].cpu().numpy()
+                psis.append(psi)
+        init[lv] = psis
+        log("serve: %d initial states at L%d" % (cnt, lv))
+    members_level = [lv for lv, cnt in zip(levels, counts) for _ in range(cnt)]
+    req_members = rng.integers(0, n_members, n_requests)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    out = {}
+    for name, prec in (("serve_ensemble_f64", P.F64), ("serve_ensemble_mixed_f32", P.F32)):
+        t_srv = time.perf_counter()
+        srv = P.Server(levels=levels, precision=prec, verbose=verbose, substeps=substeps)
+        log("serve: %s engines ready in %.1fs, slots %s" %
+            (name, time.perf_counter() - t_srv, [srv.slots(lv) for lv in levels]))
+        ids, k_at = [], {lv: 0 for lv in levels}
+        for lv in members_level:
+            ids.append(srv.add_member(lv, init[lv][k_at[lv]]))
+            k_at[lv] += 1
+        # warm-up: one request per slot of every level (graphs, workspaces)
+        warm = [ids[members_level.index(lv) + s] for lv in levels for s in range(srv.slots(lv))]
+        srv.run(warm, np.zeros(len(warm)))
+        log("serve: warm-up done (%d steps)" % len(warm))
+        start, finish, infos, st = srv.run(req_members, arrivals)
+        lat = finish - arrivals
+        its = np.array([i["iterations"] for i in infos])
+        conv = np.array([i["converged"] for i in infos])
+        div = np.array([i["diverged"] for i in infos])
+        by_level = {}
+        for lv in levels:
+            sel = np.array([members_level[m] == lv for m in req_members])
+            if sel.any():
+                by_level["L%d" % lv] = dict(requests=int(sel.sum()),
+                                            p50_ms=round(float(np.median(lat[sel])) * 1e3, 2),
+                                            fp_passes_mean=round(float(its[sel].mean()), 2),
+                                            converged=int(conv[sel].sum()),
+                                            diverged=int(div[sel].sum()),
+                                            slots=srv.slots(lv))
+        out[name] = dict(steps_per_s=round(st["throughput"], 3),
+                         p50_ms=round(st["p50_latency_s"] * 1e3, 2),
+                         p99_ms=round(st["p99_latency_s"] * 1e3, 2),
+                         mean_ms=round(st["mean_latency_s"] * 1e3, 2),
+                         p50_service_ms=round(st["p50_service_s"] * 1e3, 2),
+                         mean_members_per_pass=round(st["mean_occupancy"], 3),
+                         requests=int(n_requests), offered_req_per_s=rate,
+                         members=n_members, converged=int(conv.sum()), diverged=int(div.sum()),
+                         device_GB=round(st["device_bytes"] / 1e9, 2),
+                         substeps=dict(zip(["L%d" % lv for lv in levels], substeps)),
+                         levels=by_level)
+        log("%s: %.2f steps/s, p50 %.0f ms, p99 %.0f ms, %.2f members/pass" %
+            (name, st["throughput"], st["p50_latency_s"] * 1e3, st["p99_latency_s"] * 1e3,
+             st["mean_occupancy"]))
+        srv.close()
+    return out
+
+
 def cpu_baseline(rp, ci, v, b, budget_s):
     """Oracle CSR GMRES(30)-MGS (OpenMP) on the host cores, same A and b."""
     from oracle import oracle as O
@@ -117,6 +216,9 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--cold-spmv", action="store_true",
                     help="also time the SpMV round-robin over 40 matrix copies (> 512 MB)")
+    ap.add_argument("--serve-requests", type=int, default=256,
+                    help="requests of the ensemble serving leg (0: skip it)")
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 256 MiB device copy (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -218,6 +320,28 @@ def main():
         log("batched64: %.1f solves/s (%.1f ms per 64), %d block iters, max rel %.2e" %
             (batched["solves_per_s"], batched["batch_ms"], binfo["iterations"],
              binfo["max_rel_residual"]))
+        # the same 64 systems as independent columns sharing each SpMM
+        for name, prec in (("batched64_f32_gmres_cgs2", P.F32), ("batched64_f64_gmres_cgs2", P.F64)):
+            Xc = torch.zeros_like(B64)
+            A.solve_batch(Xc, B64, rtol=1e-10, precision=prec, stream=stream)  # warmup
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(nb):
+                cinfo = A.solve_batch(Xc, B64, rtol=1e-10, precision=prec, stream=stream)
+            torch.cuda.synchronize()
+            wc = time.perf_counter() - t0
+            variants[name] = dict(
+                solves_per_s=round(64 * nb / wc, 3), rhs=64, batch_ms=round(wc / nb * 1e3, 3),
+                max_iterations=max(i["iterations"] for i in cinfo),
+                max_rel_residual=max(i["rel_residual"] for i in cinfo),
+                converged=all(i["converged"] for i in cinfo), x=Xc[:, 0].clone())
+            log("%s: %.1f solves/s (%.1f ms per 64), max iters %d, max rel %.2e" %
+                (name, variants[name]["solves_per_s"], variants[name]["batch_ms"],
+                 variants[name]["max_iterations"], variants[name]["max_rel_residual"]))
+
+    # ---- serving: ensemble BarVortex steps on mixed levels ----
+    if not args.no_variants and args.serve_requests > 0:
+        variants.update(serve_leg(args.serve_requests, args.serve_rate))
 
     # ---- roofline of the FP64 SpMV (Solver::mult_vector) ----
     y = torch.zeros_like(b)
@@ -261,7 +385,8 @@ def main():
         nt = np.linalg.norm(xt)
         fwd["fp64_mgs"] = float(np.linalg.norm(x64 - xt) / nt)
         for name, d in variants.items():
-            fwd[name] = float(np.linalg.norm(d["x"].cpu().numpy() - xt) / nt)
+            if d.get("x") is not None:
+                fwd[name] = float(np.linalg.norm(d["x"].cpu().numpy() - xt) / nt)
         if batched is not None:
             batched["fwd_err_vs_oracle_col0"] = float(np.linalg.norm(batched["x0"] - xt) / nt)
         rate, threads, ns, dt, cinfo = cpu_baseline(rp, ci, v, bh, args.cpu_budget)

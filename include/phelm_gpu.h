@@ -277,15 +277,31 @@ typedef struct {
     double tol;            /* stop at ||u_n1 - u_n|| / ||u|| < tol, ref. 1e-14  */
     double rtol_mass;      /* mass solves of L() and of the Jacobian           */
     double rtol_helmholtz; /* A_ solve                                        */
-    double rtol_poisson;   /* laplace_ solve (multigrid-preconditioned)       */
+    double rtol_poisson;   /* laplace_ solve (multigrid-preconditioned); its
+                              FP64 residual floor is ~1e-13..1e-12 at levels
+                              7-8 (stiffness rows sum to zero)               */
     int precision;         /* PHG_F64, or PHG_F32 / PHG_BF16: mass and Helmholtz
                               solves in FP32 Krylov with FP64 refinement      */
+    int max_solver_iters;  /* cap of every inner solve                        */
+    double stall_floor;    /* also end the step when the change is below this
+                              and no longer contracting (rel >= half the value
+                              two passes before; passes alternate):
+                              the iterate then moves by rounding and solve-
+                              tolerance noise only; 0 keeps the reference's
+                              tol-only test                                   */
 } phg_bv_opts;
 typedef struct {
     int iterations;        /* fixed-point passes of the last step             */
     double rel_change;     /* ||u_n1 - u_n|| / ||u|| of the last pass         */
-    int converged;         /* rel_change < tol                                */
+    int converged;         /* rel_change < tol, or stalled below stall_floor  */
+    int stalled;           /* ended by the stall_floor rule                   */
+    int diverged;          /* ended because the change grew past the state
+                              (rel_change > 1 after 3 passes), 1000x above
+                              its smallest value, or became NaN: the fixed
+                              point does not contract, e.g. for grid-scale
+                              vorticity at large tau |v| / h; u1 = u         */
     int solver_iterations; /* GMRES steps of the member's solves (all kinds)  */
+    int it_mass, it_jacobian, it_helmholtz, it_poisson; /* the same by solve */
 } phg_bv_info;
 typedef struct phg_barvortex_s *phg_barvortex_t;
 
@@ -331,6 +347,12 @@ typedef struct {
     double coriolis;                  /* lh = coriolis * sin(latitude)        */
     phg_bv_conf conf;
     phg_bv_opts opts;
+    int verbose;                      /* 1: a stderr line per finished step  */
+    int substeps[PHG_SERVE_MAXLEVELS]; /* per level: a request runs this many
+                                         BarVortex::calc steps of tau/k
+                                         (0 or 1: one step of tau); for levels
+                                         where the fixed point does not
+                                         contract at tau                      */
 } phg_serve_config;
 typedef struct {
     int completed;
@@ -345,7 +367,8 @@ typedef struct phg_server_s *phg_server_t;
 
 /* Levels 5-8, budget 200 GiB, BarVortexConf of the ensemble workload in units
  * of 1/Earth's angular velocity: tau = 600 s, sigma 1.6e-2, mu 8e-5,
- * k1 = k2 = 1, theta 1/2, lh = 2 sin(latitude); default phg_bv_opts. */
+ * k1 = k2 = 1, theta 1/2, lh = 2 sin(latitude); default phg_bv_opts with at
+ * most 100 fixed-point passes per step. */
 void phg_serve_default_config(phg_serve_config *c);
 int phg_serve_create(const phg_serve_config *cfg, phg_server_t *out);
 void phg_serve_destroy(phg_server_t S);

@@ -23,6 +23,7 @@
 // batched GMRES (mass-dominated, well conditioned).  The Helmholtz and
 // Poisson solves start from the current iterate (w_n, u_n): once the iterate
 // stops moving they return it unchanged and the step ends.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -60,8 +61,10 @@ struct phg_barvortex_s {
     DevBuf<double> part, nrm;       // reductions
     DevBuf<int> dmask;
     int G = 0;
-    std::vector<int> status, iters, solver_iters;  // per slot
-    std::vector<double> nr0, rel;
+    std::vector<int> status, iters;                 // per slot
+    std::vector<int> it_mass, it_jac, it_helm, it_pois;  // GMRES steps per slot and solve
+    std::vector<double> nr0, rel, rel_prev, rel_prev2, rel_min;
+    std::vector<int> stalled, diverged;
     ~phg_barvortex_s()
     {
         delete idt;
@@ -249,7 +252,7 @@ int norms(phg_barvortex_s* B, const double* a, const double* b, std::vector<doub
     return PHG_OK;
 }
 
-phg_gmres_opts solve_opts(double rtol, int precision, int ortho, int use_x0)
+phg_gmres_opts solve_opts(double rtol, int precision, int ortho, int use_x0, int max_iters)
 {
     phg_gmres_opts o;
     phg_gmres_default_opts(&o);
@@ -259,7 +262,7 @@ phg_gmres_opts solve_opts(double rtol, int precision, int ortho, int use_x0)
     o.precision = precision;
     o.inner_rtol = 1e-4;
     o.use_x0 = use_x0;
-    o.max_iters = 20000;
+    o.max_iters = max_iters;
     return o;
 }
 
@@ -298,10 +301,10 @@ void zero_cols(phg_barvortex_s* B, size_t rows, double* x, hipStream_t s)
 }
 
 int batched(phg_barvortex_s* B, phg_matrix_s* A, const double* rhs, double* x, double rtol,
-            int precision, int ortho, int use_x0, phg_mg_s* pc, std::vector<int>& acc,
-            hipStream_t s)
+            int precision, int ortho, int use_x0, phg_mg_s* pc, int max_iters,
+            std::vector<int>& acc, hipStream_t s)
 {
-    phg_gmres_opts o = solve_opts(rtol, precision, ortho, use_x0);
+    phg_gmres_opts o = solve_opts(rtol, precision, ortho, use_x0, max_iters);
     std::vector<phg_gmres_info> inf(B->S);
     const int rc = gmres_batch_solve(A, B->S, rhs, x, &o, inf.data(), s, pc);
     if (rc < 0) return rc;
@@ -327,7 +330,7 @@ int calc1(phg_barvortex_s* B, const double* u, double* out, const phg_bv_opts& o
     zero_cols(B, n, B->Q.p, s);
     PHG_HIP(hipGetLastError());
     int rc = batched(B, B->idt, B->Q.p, B->X.p, o.rtol_mass, o.precision, PHG_CGS1, 0, nullptr,
-                     acc, s);
+                     o.max_solver_iters, acc, s);
     if (rc) return rc;
     hipLaunchKernelGGL(bv_p2u, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
                        B->mesh->d_isb.p, B->mesh->d_p2io.p, B->X.p, out, B->dmask.p);
@@ -344,9 +347,9 @@ int init_slots(phg_barvortex_s* B, const std::vector<int>& mask, const phg_bv_op
     const phg_bv_conf& c = B->conf;
     int rc = upload_mask(B, mask, s);
     if (rc) return rc;
-    rc = calc1(B, B->U.p, B->W.p, o, B->solver_iters, s);       // w = L(u)
+    rc = calc1(B, B->U.p, B->W.p, o, B->it_mass, s);            // w = L(u)
     if (rc) return rc;
-    rc = calc1(B, B->W.p, B->T1.p, o, B->solver_iters, s);      // dw = L(w)
+    rc = calc1(B, B->W.p, B->T1.p, o, B->it_mass, s);           // dw = L(w)
     if (rc) return rc;
     // FC = w / tau + mu (1 - theta) dw;  FC = FC - sigma (1 - theta) w
     hipLaunchKernelGGL(bv_axpby, g256(V * S), dim3(BV_BS), 0, s, V, S, 1.0 / c.tau, B->W.p,
@@ -366,6 +369,11 @@ int init_slots(phg_barvortex_s* B, const std::vector<int>& mask, const phg_bv_op
             B->nr0[k] = nr[k];
             B->iters[k] = 0;
             B->rel[k] = 1.0;
+            B->rel_prev[k] = 1.0;
+            B->rel_prev2[k] = 1.0;
+            B->rel_min[k] = 1.0;
+            B->stalled[k] = 0;
+            B->diverged[k] = 0;
         }
     return PHG_OK;
 }
@@ -389,13 +397,13 @@ int iterate_slots(phg_barvortex_s* B, const std::vector<int>& mask, const phg_bv
     PHG_HIP(hipGetLastError());
     // jac = J(tmp2, tmp1): 8 mass solves per slot, one batched solve
     {
-        phg_gmres_opts jo = solve_opts(o.rtol_mass, o.precision, PHG_CGS1, 0);
+        phg_gmres_opts jo = solve_opts(o.rtol_mass, o.precision, PHG_CGS1, 0, o.max_solver_iters);
         jo.restart = 32;
         std::vector<phg_gmres_info> inf((size_t)8 * S);
         rc = jacobian_calc2_strided(B->J, S, B->T2.p, B->T1.p, S, 1, B->JAC.p, S, 1, 1.0, &jo,
                                     inf.data(), s);
         if (rc < 0) return rc;
-        count_iters(inf, S, B->solver_iters, 8);
+        count_iters(inf, S, B->it_jac, 8);
     }
     // F = FC_inner - jac;  rp = idt F
     hipLaunchKernelGGL(bv_rhs, g256((size_t)n * S), dim3(BV_BS), 0, s, n, S, B->mesh->d_inner.p,
@@ -407,7 +415,7 @@ int iterate_slots(phg_barvortex_s* B, const std::vector<int>& mask, const phg_bv
                        B->Wn.p, B->X.p, B->dmask.p);
     PHG_HIP(hipGetLastError());
     rc = batched(B, B->A, B->RP.p, B->X.p, o.rtol_helmholtz, o.precision, PHG_CGS2, 1, nullptr,
-                 B->solver_iters, s);
+                 o.max_solver_iters, B->it_helm, s);
     if (rc) return rc;
     hipLaunchKernelGGL(bv_p2u, g256(V * S), dim3(BV_BS), 0, s, B->V, S, B->mesh->d_isb.p,
                        B->mesh->d_p2io.p, B->X.p, B->Wn.p, B->dmask.p);
@@ -422,7 +430,7 @@ int iterate_slots(phg_barvortex_s* B, const std::vector<int>& mask, const phg_bv
     phg_matrix_t lap = nullptr;
     phg_mg_matrix(B->mg, &lap);
     rc = batched(B, lap, B->Q.p, B->Z.p, o.rtol_poisson, PHG_F64, PHG_CGS2, 1, B->mg,
-                 B->solver_iters, s);
+                 o.max_solver_iters, B->it_pois, s);
     if (rc) return rc;
     hipLaunchKernelGGL(bv_p2u, g256(V * S), dim3(BV_BS), 0, s, B->V, S, B->mesh->d_isb.p,
                        B->mesh->d_p2io.p, B->Z.p, B->Un1.p, B->dmask.p);
@@ -446,8 +454,10 @@ void phg_bv_default_opts(phg_bv_opts* o)
     o->tol = 1e-14;
     o->rtol_mass = 1e-13;
     o->rtol_helmholtz = 1e-12;
-    o->rtol_poisson = 1e-12;
+    o->rtol_poisson = 1e-10;
     o->precision = PHG_F64;
+    o->max_solver_iters = 1000;
+    o->stall_floor = 1e-8;
 }
 
 size_t phg_barvortex_estimate_bytes(int n_points, int slots)
@@ -522,9 +532,14 @@ int phg_barvortex_create(phg_mesh_t m, const phg_bv_conf* conf, const double* d_
     PHG_HIP(B->dmask.alloc(S));
     B->status.assign(S, SLOT_FREE);
     B->iters.assign(S, 0);
-    B->solver_iters.assign(S, 0);
+    for (auto* v : {&B->it_mass, &B->it_jac, &B->it_helm, &B->it_pois}) v->assign(S, 0);
     B->nr0.assign(S, 0.0);
     B->rel.assign(S, 0.0);
+    B->rel_prev.assign(S, 1.0);
+    B->rel_prev2.assign(S, 1.0);
+    B->rel_min.assign(S, 1.0);
+    B->stalled.assign(S, 0);
+    B->diverged.assign(S, 0);
     PHG_HIP(hipStreamSynchronize(s));
     *out = B.release();
     return PHG_OK;
@@ -544,7 +559,7 @@ int phg_barvortex_load(phg_barvortex_t B, int slot, const double* d_u, phg_strea
     hipLaunchKernelGGL(bv_col_in, g256(B->V), dim3(BV_BS), 0, s, B->V, B->S, slot, d_u, B->U.p);
     PHG_HIP(hipGetLastError());
     B->status[slot] = SLOT_NEW;
-    B->solver_iters[slot] = 0;
+    for (auto* v : {&B->it_mass, &B->it_jac, &B->it_helm, &B->it_pois}) (*v)[slot] = 0;
     return PHG_OK;
 }
 
@@ -584,8 +599,20 @@ int phg_barvortex_iterate(phg_barvortex_t B, const phg_bv_opts* opts, int* state
         for (int k = 0; k < S; ++k) {
             if (!run[k]) continue;
             B->iters[k] += 1;
+            B->rel_prev2[k] = B->rel_prev[k];
+            B->rel_prev[k] = B->rel[k];
             B->rel[k] = B->nr0[k] > 0.0 ? nr[k] / B->nr0[k] : nr[k];
-            if (B->rel[k] < o.tol || std::isnan(nr[k]) || B->iters[k] >= o.max_iters)
+            B->stalled[k] = B->rel[k] >= o.tol && B->rel[k] < o.stall_floor &&
+                            B->iters[k] >= 3 && B->rel[k] >= 0.5 * B->rel_prev2[k];
+            // not contracting: the change grew past the state, or 1000x above
+            // its smallest value (a mode the fixed point amplifies)
+            B->diverged[k] = !std::isfinite(B->rel[k]) ||
+                             (B->iters[k] >= 3 && B->rel[k] > 1.0) ||
+                             (B->iters[k] >= 4 && B->rel[k] > 1e3 * B->rel_min[k] &&
+                              B->rel[k] > o.stall_floor);
+            B->rel_min[k] = std::min(B->rel_min[k], B->rel[k]);
+            if (B->rel[k] < o.tol || B->stalled[k] || B->diverged[k] ||
+                B->iters[k] >= o.max_iters)
                 B->status[k] = SLOT_DONE;
         }
     }
@@ -594,8 +621,15 @@ int phg_barvortex_iterate(phg_barvortex_t B, const phg_bv_opts* opts, int* state
         if (infos) {
             infos[k].iterations = B->iters[k];
             infos[k].rel_change = B->rel[k];
-            infos[k].converged = B->status[k] == SLOT_DONE && B->rel[k] < o.tol;
-            infos[k].solver_iterations = B->solver_iters[k];
+            infos[k].converged =
+                B->status[k] == SLOT_DONE && !B->diverged[k] && (B->rel[k] < o.tol || B->stalled[k]);
+            infos[k].stalled = B->stalled[k];
+            infos[k].diverged = B->diverged[k];
+            infos[k].it_mass = B->it_mass[k];
+            infos[k].it_jacobian = B->it_jac[k];
+            infos[k].it_helmholtz = B->it_helm[k];
+            infos[k].it_poisson = B->it_pois[k];
+            infos[k].solver_iterations = B->it_mass[k] + B->it_jac[k] + B->it_helm[k] + B->it_pois[k];
         }
     }
     return PHG_OK;
@@ -608,7 +642,9 @@ int phg_barvortex_store(phg_barvortex_t B, int slot, double* d_u1, phg_stream_t 
         return PHG_ERR_ARG;
     }
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(bv_col_out, g256(B->V), dim3(BV_BS), 0, s, B->V, B->S, slot, B->Un.p, d_u1);
+    // a diverged step leaves the member's state as it was
+    hipLaunchKernelGGL(bv_col_out, g256(B->V), dim3(BV_BS), 0, s, B->V, B->S, slot,
+                       B->diverged[slot] ? B->U.p : B->Un.p, d_u1);
     PHG_HIP(hipGetLastError());
     B->status[slot] = SLOT_FREE;
     return PHG_OK;

@@ -564,6 +564,10 @@ __global__ void bk_cycle_start(int KP, int K, const double* __restrict__ part, i
     if ((threadIdx.x & 63) != 0) return;
     if (!done) {
         const double beta = sqrt(s);
+        // a whole restart cycle that lowered the true residual by less than
+        // 1 % means it sits at its rounding floor above rtol: the column
+        // stops there (not converged)
+        const bool stalled = c->cycles > 0 && c->iters > c->iters0 && beta > 0.99 * c->beta;
         c->beta = beta;
         c->rel = beta / c->bnorm;
         c->k = 0;
@@ -571,7 +575,7 @@ __global__ void bk_cycle_start(int KP, int K, const double* __restrict__ part, i
         if (c->rel <= rtol || beta == 0.0) {
             c->converged = 1;
             c->active = 0;
-        } else if (c->iters >= max_iters) {
+        } else if (c->iters >= max_iters || stalled || !isfinite(beta)) {
             c->active = 0;
         } else {
             c->active = 1;

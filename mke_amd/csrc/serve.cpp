@@ -17,6 +17,8 @@
 //   free slots (FIFO by arrival; a member's next request waits for its
 //   previous one), so requests that arrive while others run join them.
 // * Member states stay resident on the device between requests.
+// * Optional sub-steps per level: a request runs k steps of tau / k in its
+//   slot, for levels where the fixed point does not contract at tau.
 // Requests are released at their arrival times (open loop, wall clock), and
 // latency is finish - arrival.
 #include <hip/hip_runtime.h>
@@ -25,6 +27,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -43,7 +46,7 @@ struct Member {
 };
 
 struct Engine {
-    int level = 0, S = 0, V = 0;
+    int level = 0, S = 0, V = 0, substeps = 1;
     phg_mesh_t mesh = nullptr;
     phg_barvortex_t bv = nullptr;
     hipStream_t stream = nullptr;
@@ -98,7 +101,7 @@ void engine_loop(phg_server_s* Sv, int e, std::vector<int> reqs, RunShared* R)
     (void)hipSetDevice(Sv->device);
     Engine& E = *Sv->engines[e];
     const int S = E.S;
-    std::vector<int> slot_req(S, -1), state(S, 0);
+    std::vector<int> slot_req(S, -1), state(S, 0), slot_left(S, 0);
     std::vector<phg_bv_info> inf(S);
     std::deque<int> waiting;
     size_t next = 0, left = reqs.size();
@@ -131,6 +134,7 @@ void engine_loop(phg_server_s* Sv, int e, std::vector<int> reqs, RunShared* R)
             }
             M.busy = true;
             slot_req[free_slot] = r;
+            slot_left[free_slot] = E.substeps;
             R->start[r] = now_s(R->t0);
             it = waiting.erase(it);
         }
@@ -159,6 +163,15 @@ void engine_loop(phg_server_s* Sv, int e, std::vector<int> reqs, RunShared* R)
                 fail(rc);
                 return;
             }
+            if (--slot_left[k] > 0 && !inf[k].diverged) {
+                // next sub-step of the request: the slot restarts from the new state
+                rc = phg_barvortex_load(E.bv, k, M.state.p, E.stream);
+                if (rc) {
+                    fail(rc);
+                    return;
+                }
+                continue;
+            }
             if (hipStreamSynchronize(E.stream) != hipSuccess) {
                 set_error("serve: stream error");
                 fail(PHG_ERR_HIP);
@@ -166,6 +179,12 @@ void engine_loop(phg_server_s* Sv, int e, std::vector<int> reqs, RunShared* R)
             }
             R->finish[r] = now_s(R->t0);
             if (R->infos) R->infos[r] = inf[k];
+            if (Sv->cfg.verbose)
+                fprintf(stderr,
+                        "[serve] L%d request %d member %d: %d passes (rel %.1e), "
+                        "arrived %.3fs admitted %.3fs done %.3fs, engine passes %ld\n",
+                        E.level, r, R->member[r], inf[k].iterations, inf[k].rel_change,
+                        R->arrival[r], R->start[r], R->finish[r], E.passes);
             M.busy = false;
             slot_req[k] = -1;
             --left;
@@ -205,6 +224,7 @@ void phg_serve_default_config(phg_serve_config* c)
     c->conf.k2 = 1.0;
     c->conf.theta = 0.5;
     phg_bv_default_opts(&c->opts);
+    c->opts.max_iters = 100;  // a step not done in 100 passes returns unconverged
 }
 
 int phg_serve_create(const phg_serve_config* cfg, phg_server_t* out)
@@ -237,6 +257,7 @@ int phg_serve_create(const phg_serve_config* cfg, phg_server_t* out)
         }
         used += need;
         E->S = S;
+        E->substeps = cfg->substeps[i] > 1 ? cfg->substeps[i] : 1;
         PHG_HIP(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
         // lh = coriolis * sin(latitude) (proj of conf.coriolis, barvortex_impl.h:121)
         std::vector<double> lh(E->V);
@@ -246,7 +267,9 @@ int phg_serve_create(const phg_serve_config* cfg, phg_server_t* out)
         }
         DevBuf<double> d_lh;
         PHG_HIP(d_lh.upload(lh.data(), lh.size()));
-        rc = phg_barvortex_create(E->mesh, &cfg->conf, d_lh.p, S, nullptr, &E->bv);
+        phg_bv_conf conf = cfg->conf;
+        conf.tau = cfg->conf.tau / E->substeps;
+        rc = phg_barvortex_create(E->mesh, &conf, d_lh.p, S, nullptr, &E->bv);
         if (rc) return rc;
         Sv->engines.push_back(std::move(E));
     }

@@ -441,12 +441,22 @@ class BvConf(C.Structure):
 
 class BvOpts(C.Structure):
     _fields_ = [("max_iters", _I), ("tol", _D), ("rtol_mass", _D), ("rtol_helmholtz", _D),
-                ("rtol_poisson", _D), ("precision", _I)]
+                ("rtol_poisson", _D), ("precision", _I), ("max_solver_iters", _I),
+                ("stall_floor", _D)]
 
 
 class BvInfo(C.Structure):
-    _fields_ = [("iterations", _I), ("rel_change", _D), ("converged", _I),
-                ("solver_iterations", _I)]
+    _fields_ = [("iterations", _I), ("rel_change", _D), ("converged", _I), ("stalled", _I),
+                ("diverged", _I),
+                ("solver_iterations", _I), ("it_mass", _I), ("it_jacobian", _I),
+                ("it_helmholtz", _I), ("it_poisson", _I)]
+
+
+def _bv_info(i):
+    return dict(iterations=i.iterations, rel_change=i.rel_change, converged=bool(i.converged),
+                stalled=bool(i.stalled), diverged=bool(i.diverged),
+                solver_iterations=i.solver_iterations, it_mass=i.it_mass,
+                it_jacobian=i.it_jacobian, it_helmholtz=i.it_helmholtz, it_poisson=i.it_poisson)
 
 
 def bv_opts(**kw):
@@ -479,9 +489,7 @@ class BarVortex:
                                       C.byref(opts) if opts is not None else None, infos,
                                       _stream(stream))
         check(rc, "phg_barvortex_calc")
-        return [dict(iterations=i.iterations, rel_change=i.rel_change,
-                     converged=bool(i.converged), solver_iterations=i.solver_iterations)
-                for i in infos]
+        return [_bv_info(i) for i in infos]
 
     def load(self, slot, u, stream=None):
         check(lib().phg_barvortex_load(self.h, slot, _ptr(u), _stream(stream)), "phg_barvortex_load")
@@ -491,8 +499,7 @@ class BarVortex:
         infos = (BvInfo * self.slots)()
         check(lib().phg_barvortex_iterate(self.h, C.byref(opts) if opts is not None else None,
                                           state, infos, _stream(stream)), "phg_barvortex_iterate")
-        return list(state), [dict(iterations=i.iterations, rel_change=i.rel_change,
-                                  converged=bool(i.converged)) for i in infos]
+        return list(state), [_bv_info(i) for i in infos]
 
     def store(self, slot, u1, stream=None):
         check(lib().phg_barvortex_store(self.h, slot, _ptr(u1), _stream(stream)),
@@ -513,7 +520,7 @@ class BarVortex:
 class ServeConfig(C.Structure):
     _fields_ = [("n_levels", _I), ("levels", _I * 8), ("slots", _I * 8),
                 ("hbm_budget", C.c_size_t), ("coriolis", _D), ("conf", BvConf),
-                ("opts", BvOpts)]
+                ("opts", BvOpts), ("verbose", _I), ("substeps", _I * 8)]
 
 
 class ServeStats(C.Structure):
@@ -528,18 +535,20 @@ class Server:
     resident on the device."""
 
     def __init__(self, levels=(5, 6, 7, 8), slots=None, hbm_budget=None, precision=F64,
-                 **conf):
+                 verbose=False, substeps=None, **conf):
         c = ServeConfig()
         lib().phg_serve_default_config(C.byref(c))
         c.n_levels = len(levels)
         for i, lv in enumerate(levels):
             c.levels[i] = lv
             c.slots[i] = 0 if slots is None else slots[i]
+            c.substeps[i] = 1 if substeps is None else substeps[i]
         if hbm_budget is not None:
             c.hbm_budget = int(hbm_budget)
         for k, v in conf.items():
             setattr(c.conf, k, v)
         c.opts.precision = precision
+        c.verbose = int(verbose)
         self.cfg = c
         self.h = C.c_void_p()
         check(lib().phg_serve_create(C.byref(c), C.byref(self.h)), "phg_serve_create")
@@ -569,10 +578,8 @@ class Server:
         st = ServeStats()
         check(lib().phg_serve_run(self.h, n, _pi(m), _pd(a), _pd(start), _pd(finish), infos,
                                   C.byref(st)), "phg_serve_run")
-        its = np.array([infos[r].iterations for r in range(n)])
-        conv = np.array([bool(infos[r].converged) for r in range(n)])
         stats = {f: getattr(st, f) for f, _ in ServeStats._fields_}
-        return start, finish, its, conv, stats
+        return start, finish, [_bv_info(infos[r]) for r in range(n)], stats
 
     def close(self):
         if self.h:

@@ -3,7 +3,8 @@
 serving loop relies on.
 
 Bars: the GPU step's u1 within 1e-9 (SphereNorm-relative) of the oracle's
-with the same number of fixed-point passes (+-2): the GPU solves stop at
+in at most 2 more fixed-point passes (it may end earlier by the stall rule, once
+the change no longer contracts below 1e-11): the GPU solves stop at
 relative residuals 1e-12..1e-13; a member's u1 is bit-identical alone, at
 another slot, in a full batch, and when admitted while others are running."""
 import numpy as np
@@ -55,7 +56,7 @@ def test_barvortex_step_vs_oracle(level):
     for q in range(2):
         ref, it, rel = ob.calc(U[q], max_iters=100)
         assert infos[q]["converged"], infos[q]
-        assert abs(infos[q]["iterations"] - it) <= 2, (infos[q], it)
+        assert 3 <= infos[q]["iterations"] <= it + 2, (infos[q], it)
         err = ob.norm(got[q] - ref) / ob.norm(ref)
         assert err <= 1e-9, (q, err)
         assert ob.norm(got[q] - U[q]) > 1e-5 * ob.norm(U[q])  # the step moved the state
