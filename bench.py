@@ -189,6 +189,61 @@ def serve_leg(n_requests, rate, seed=20251202, verbose=False):
     return out
 
 
+def l8_leg(steps, oracle_check, seed=20251203):
+    """Mixed precision on the level-8 mesh (655,362 vertices, 4,587,509
+    nonzeros): (M - 1e-2 L) u = M f, f of degree <= 64 with N(0,1) max(l,1)^-2
+    coefficients; FP64 forms and FP32/BF16 inner solves under FP64
+    refinement, each to a true FP64 relative residual of 1e-10, with the
+    forward error against a tight FP64 oracle solve of the same CSR system
+    (rtol 1e-13, host cores) when oracle_check is set."""
+    import numpy as np
+    import torch
+
+    import mke_amd as P
+    from tests._common import real_sph_harm_field_lean
+
+    stream = torch.cuda.current_stream()
+    t = time.perf_counter()
+    mesh = P.Mesh.sphere(8)
+    cm, cl = P.helmholtz_coeffs(tau=1.0, sigma=0.0, theta=1.0, mu=TAU)
+    A = P.Matrix.assemble(mesh, P.HELMHOLTZ, cm, cl, stream=stream)
+    Mm = P.Matrix.assemble(mesh, P.MASS, stream=stream)
+    a = mesh.arrays()
+    f = real_sph_harm_field_lean(a["pr"], 64, seed, decay=2.0, device="cuda")[a["inner"]]
+    b = torch.zeros(A.n_rows, dtype=torch.float64, device="cuda")
+    Mm.mult_vector(b, torch.as_tensor(f, device="cuda"), stream=stream)
+    torch.cuda.synchronize()
+    log("L8: mesh, matrices and rhs in %.1fs (n=%d nnz=%d)" %
+        (time.perf_counter() - t, A.n_rows, A.nnz))
+    out, xs = {}, {}
+    for name, kw in (("L8_fp64_mgs", dict(ortho=P.MGS)),
+                     ("L8_fp64_cgs2", dict(ortho=P.CGS2)),
+                     ("L8_mixed_f32_cgs2", dict(ortho=P.CGS2, precision=P.F32)),
+                     ("L8_mixed_bf16_cgs2", dict(ortho=P.CGS2, precision=P.BF16))):
+        x = torch.zeros_like(b)
+        w, lat, q50, q99, inf = time_solves(A, b, x, steps, 1, stream, restart=30,
+                                            rtol=1e-10, **kw)
+        out[name] = dict(solves_per_s=round(len(inf) / w, 3), p50_ms=round(q50, 3),
+                         p99_ms=round(q99, 3), iterations=inf[-1]["iterations"],
+                         rel_residual=inf[-1]["rel_residual"], converged=inf[-1]["converged"],
+                         level=8, rhs="real Y_lm, l<=64, c ~ N(0,1) max(l,1)^-2")
+        xs[name] = x.cpu().numpy()
+        log("%s: %.2f solves/s, p50 %.1f ms, %d iters, rel %.2e" %
+            (name, out[name]["solves_per_s"], q50, inf[-1]["iterations"],
+             inf[-1]["rel_residual"]))
+    if oracle_check:
+        from oracle import oracle as O
+        t = time.perf_counter()
+        rp, ci, v = A.csr()
+        xt, info = O.gmres(rp, ci, v, b.cpu().numpy(), restart=30, rtol=1e-13,
+                           max_iters=400000)
+        nt = np.linalg.norm(xt)
+        for name in out:
+            out[name]["fwd_err_vs_oracle"] = float(np.linalg.norm(xs[name] - xt) / nt)
+        log("L8 oracle solve: %d iterations in %.1fs" % (info["iterations"], time.perf_counter() - t))
+    return out
+
+
 def cpu_baseline(rp, ci, v, b, budget_s):
     """Oracle CSR GMRES(30)-MGS (OpenMP) on the host cores, same A and b."""
     from oracle import oracle as O
@@ -219,6 +274,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=256,
                     help="requests of the ensemble serving leg (0: skip it)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--no-l8", action="store_true",
+                    help="skip the level-8 mixed-precision leg")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 256 MiB device copy (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -322,6 +379,10 @@ def main():
              binfo["max_rel_residual"]))
         # the same 64 systems as independent columns sharing each SpMM
         for name, prec in (("batched64_f32_gmres_cgs2", P.F32), ("batched64_f64_gmres_cgs2", P.F64)):
+            rpt, tune_ms = A.tune_batch(64, restart=30, ortho=P.CGS2, precision=prec,
+                                        stream=stream)
+            log("%s: autotuned %d rows per thread (cycle ms %s)" %
+                (name, rpt, {k: round(v, 2) for k, v in tune_ms.items()}))
             Xc = torch.zeros_like(B64)
             A.solve_batch(Xc, B64, rtol=1e-10, precision=prec, stream=stream)  # warmup
             torch.cuda.synchronize()
@@ -334,10 +395,16 @@ def main():
                 solves_per_s=round(64 * nb / wc, 3), rhs=64, batch_ms=round(wc / nb * 1e3, 3),
                 max_iterations=max(i["iterations"] for i in cinfo),
                 max_rel_residual=max(i["rel_residual"] for i in cinfo),
-                converged=all(i["converged"] for i in cinfo), x=Xc[:, 0].clone())
+                converged=all(i["converged"] for i in cinfo), x=Xc[:, 0].clone(),
+                rows_per_thread=rpt,
+                tune_cycle_ms={str(k): round(v, 3) for k, v in tune_ms.items()})
             log("%s: %.1f solves/s (%.1f ms per 64), max iters %d, max rel %.2e" %
                 (name, variants[name]["solves_per_s"], variants[name]["batch_ms"],
                  variants[name]["max_iterations"], variants[name]["max_rel_residual"]))
+
+    # ---- level 8: mixed precision vs FP64 ----
+    if not args.no_variants and not args.no_l8:
+        variants.update(l8_leg(max(2, args.steps // 10), not args.no_cpu))
 
     # ---- serving: ensemble BarVortex steps on mixed levels ----
     if not args.no_variants and args.serve_requests > 0:

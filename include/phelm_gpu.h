@@ -148,6 +148,10 @@ int phg_u2p(phg_mesh_t m, const double *d_u, double *d_p, phg_stream_t s);
 /* u[i] = boundary ? (bnd ? bnd[p2io[i]] : 0) : p[p2io[i]]  (p2u) */
 int phg_p2u(phg_mesh_t m, const double *d_p, const double *d_bnd, double *d_u,
             phg_stream_t s);
+/* b[k] = u[outer[k]]  (proj_bnd) */
+int phg_proj_bnd(phg_mesh_t m, const double *d_u, double *d_b, phg_stream_t s);
+/* u[i] = bnd ? bnd[p2io[i]] : 0 at the boundary points only  (set_bnd) */
+int phg_set_bnd(phg_mesh_t m, const double *d_bnd, double *d_u, phg_stream_t s);
 
 /* ---- GMRES (Solver::solve: src/mesh.h:740-746, src/impl/slaplace_impl.h:171) */
 typedef struct {
@@ -212,6 +216,26 @@ int phg_block_gmres(phg_matrix_t A, int nrhs, const double *d_B, double *d_X,
 int phg_gmres_batch(phg_matrix_t A, int nrhs, const double *d_B, double *d_X,
                     const phg_gmres_opts *opts, phg_gmres_info *infos,
                     phg_stream_t s);
+/* Launch shape of the batched solves (no reference counterpart: the CPU
+ * solve has none).  The row kernels give each thread rows_per_thread rows
+ * (1, 2, 4 or 8) of one column; the value is kept per process for each
+ * (n_rows, nrhs rounded up to a power of two, precision) shape and fixes the
+ * grouping of every dot product's block partials, so results are bitwise
+ * reproducible for a fixed value.  Untuned shapes take the largest value
+ * that still gives >= 1024 workgroups.  phg_gmres_batch_tune times one
+ * restart cycle (opts->restart steps, opts->ortho and opts->precision; NULL:
+ * defaults with CGS2) of nrhs hash-valued right-hand sides on A for each value,
+ * keeps the fastest for the shape and returns it in *rows_per_thread (may be
+ * NULL); ms (4 entries, may be NULL) receives the cycle times for 1, 2, 4, 8.
+ * It allocates scratch vectors; do not call it while solves of the same
+ * shape run on other streams. */
+int phg_gmres_batch_tune(phg_matrix_t A, int nrhs, const phg_gmres_opts *opts,
+                         phg_stream_t s, int *rows_per_thread, double *ms);
+/* The current value for a shape (>= 1), or a negative error code. */
+int phg_gmres_batch_shape(int n_rows, int nrhs, int precision);
+/* Pins a shape's value (e.g. one a previous process tuned); 0 returns the
+ * shape to the untuned default. */
+int phg_gmres_batch_set_shape(int n_rows, int nrhs, int precision, int rows_per_thread);
 
 /* ---- multigrid-preconditioned solves on the icosahedral sphere ----------
  * No single reference call: it makes the solves of SphereLaplace::solve

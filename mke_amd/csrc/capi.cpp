@@ -21,6 +21,8 @@ int axpby_f64(int n, double k1, const double* a, double k2, const double* b, dou
               hipStream_t s);
 int u2p_f64(phg_mesh_s* m, const double* u, double* p, hipStream_t s);
 int p2u_f64(phg_mesh_s* m, const double* p, const double* bnd, double* u, hipStream_t s);
+int proj_bnd_f64(phg_mesh_s* m, const double* u, double* b, hipStream_t s);
+int set_bnd_f64(phg_mesh_s* m, const double* bnd, double* u, hipStream_t s);
 int gmres_solve(phg_matrix_s* A, const double* b, double* x, const phg_gmres_opts* opts,
                 phg_gmres_info* info, hipStream_t s);
 void csr_structure(const SphereMesh& m, std::vector<int>& rp, std::vector<int>& ci);
@@ -339,6 +341,24 @@ int phg_p2u(phg_mesh_t m, const double* d_p, const double* d_bnd, double* d_u, p
     return p2u_f64(m, d_p, d_bnd, d_u, (hipStream_t)s);
 }
 
+int phg_proj_bnd(phg_mesh_t m, const double* d_u, double* d_b, phg_stream_t s)
+{
+    if (!m || !d_u || !d_b) {
+        set_error("phg_proj_bnd: null argument");
+        return PHG_ERR_ARG;
+    }
+    return proj_bnd_f64(m, d_u, d_b, (hipStream_t)s);
+}
+
+int phg_set_bnd(phg_mesh_t m, const double* d_bnd, double* d_u, phg_stream_t s)
+{
+    if (!m || !d_u) {
+        set_error("phg_set_bnd: null argument");
+        return PHG_ERR_ARG;
+    }
+    return set_bnd_f64(m, d_bnd, d_u, (hipStream_t)s);
+}
+
 void phg_gmres_default_opts(phg_gmres_opts* o)
 {
     o->restart = 30;
@@ -426,6 +446,36 @@ int phg_gmres_batch(phg_matrix_t A, int nrhs, const double* d_B, double* d_X,
         return PHG_ERR_ARG;
     }
     return gmres_batch_solve(A, nrhs, d_B, d_X, &o, infos, (hipStream_t)s);
+}
+
+int phg_gmres_batch_tune(phg_matrix_t A, int nrhs, const phg_gmres_opts* opts, phg_stream_t s,
+                         int* rows_per_thread, double* ms)
+{
+    if (!A) {
+        set_error("phg_gmres_batch_tune: null argument");
+        return PHG_ERR_ARG;
+    }
+    phg_gmres_opts o;
+    if (opts) o = *opts;
+    else {
+        phg_gmres_default_opts(&o);
+        o.ortho = PHG_CGS2;
+    }
+    return gmres_batch_tune(A, nrhs, &o, (hipStream_t)s, rows_per_thread, ms);
+}
+
+int phg_gmres_batch_shape(int n_rows, int nrhs, int precision)
+{
+    if (n_rows < 1 || nrhs < 1 || nrhs > 64 || precision < PHG_F64 || precision > PHG_BF16) {
+        set_error("phg_gmres_batch_shape: bad argument");
+        return PHG_ERR_ARG;
+    }
+    return batch_shape_get(n_rows, nrhs, precision);
+}
+
+int phg_gmres_batch_set_shape(int n_rows, int nrhs, int precision, int rows_per_thread)
+{
+    return batch_shape_set(n_rows, nrhs, precision, rows_per_thread);
 }
 
 }  // extern "C"

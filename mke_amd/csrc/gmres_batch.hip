@@ -27,6 +27,8 @@
 // one hipGraph and replayed; the host reads a K-entry summary per cycle.
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "ell.h"
 #include "phg_internal.h"
@@ -36,7 +38,7 @@ namespace phg {
 constexpr int BB_BS = 256;
 constexpr int BB_MAXM = 32;
 constexpr int BB_MAXK = 64;
-constexpr int BB_RPT = 8;  // rows per thread
+constexpr int BB_RPT = 8;  // largest rows per thread of a launch shape
 constexpr int BB_LB = 4;   // basis vectors per load batch
 
 struct BCol {
@@ -54,7 +56,7 @@ struct BSum {
 };
 
 struct BatchWork {
-    int n = 0, m = 0, kp = 0, G = 0, prec = 0;
+    int n = 0, m = 0, kp = 0, G = 0, prec = 0, rpt = 0;
     DevBuf<char> V, W;    // (m+1) x n x kp and 2 x n x kp of the Krylov type
     DevBuf<char> Z;       // m x n x kp preconditioned vectors (with a preconditioner)
     DevBuf<double> part;  // (m+2) x kp x G block partials
@@ -179,7 +181,7 @@ struct BRow<float, MT> {
 // Block partials of <V_i, w> for i < nd (i == own_i taken from `own`), or
 // of ||w||^2 when nd == 0, for this thread's rows; written to
 // part[((out0 + i) * KP + q) * G + block].  All threads of the block call it.
-template <typename VT>
+template <typename VT, int RPT>
 __device__ __forceinline__ void bdots(int nd, int own_i, const VT* own, const VT* __restrict__ V,
                                       size_t bs, int n, int KP, int q, int base, int RB,
                                       bool act, const VT* w, double* __restrict__ part, int G,
@@ -193,7 +195,7 @@ __device__ __forceinline__ void bdots(int nd, int own_i, const VT* own, const VT
         if (nd == 0) {
             double a = 0.0;
 #pragma unroll
-            for (int t = 0; t < BB_RPT; ++t) {
+            for (int t = 0; t < RPT; ++t) {
                 const int row = base + t * RB;
                 if (act && row < n) a += dd(w[t]) * dd(w[t]);
             }
@@ -201,12 +203,12 @@ __device__ __forceinline__ void bdots(int nd, int own_i, const VT* own, const VT
 #pragma unroll
             for (int u = 1; u < BB_LB; ++u) acc[u] = 0.0;
         } else {
-            VT vb[BB_LB][BB_RPT];
+            VT vb[BB_LB][RPT];
 #pragma unroll
             for (int u = 0; u < BB_LB; ++u) {
                 const int i = i0 + u;
 #pragma unroll
-                for (int t = 0; t < BB_RPT; ++t) {
+                for (int t = 0; t < RPT; ++t) {
                     const int row = base + t * RB;
                     vb[u][t] = (act && i < nd && i != own_i && row < n)
                                    ? V[(size_t)i * bs + (size_t)row * KP + q] : VT(0);
@@ -217,7 +219,7 @@ __device__ __forceinline__ void bdots(int nd, int own_i, const VT* own, const VT
                 const int i = i0 + u;
                 double a = 0.0;
 #pragma unroll
-                for (int t = 0; t < BB_RPT; ++t) {
+                for (int t = 0; t < RPT; ++t) {
                     const int row = base + t * RB;
                     if (act && i < nd && row < n) a += dd(w[t]) * dd(i == own_i ? own[t] : vb[u][t]);
                 }
@@ -246,7 +248,7 @@ __device__ __forceinline__ void bdots(int nd, int own_i, const VT* own, const VT
 // ---------------------------------------------------------------- kernels --
 // Arnoldi step j: v_j = W_src / h (stored), W_dst = A v_j, partial dots of
 // W_dst with v_0..v_j (first CGS pass).
-template <typename VT, typename MT>
+template <typename VT, typename MT, int RPT>
 __global__ void __launch_bounds__(BB_BS)
 bk_step_spmm(int n, int KP, int K, int width, const int* __restrict__ col,
              const MT* __restrict__ val, const VT* __restrict__ Wsrc, VT* __restrict__ V,
@@ -255,15 +257,15 @@ bk_step_spmm(int n, int KP, int K, int width, const int* __restrict__ col,
 {
     const int q = threadIdx.x & (KP - 1);
     const int RB = BB_BS / KP;
-    const int base = lblock() * RB * BB_RPT + (int)threadIdx.x / KP;
+    const int base = lblock() * RB * RPT + (int)threadIdx.x / KP;
     const bool act = q < K && cols[q].active;
     double h = 1.0;
     if (act) h = j == 0 ? cols[q].beta : cols[q].hn;
     const VT inv = (VT)(1.0 / h);
     VT* Vj = V + (size_t)j * bs;
-    VT w[BB_RPT], vj[BB_RPT];
+    VT w[RPT], vj[RPT];
 #pragma unroll
-    for (int t = 0; t < BB_RPT; ++t) {
+    for (int t = 0; t < RPT; ++t) {
         const int row = base + t * RB;
         w[t] = VT(0);
         vj[t] = VT(0);
@@ -275,7 +277,7 @@ bk_step_spmm(int n, int KP, int K, int width, const int* __restrict__ col,
             Wdst[o] = w[t];
         }
     }
-    bdots<VT>(j + 1, j, vj, V, bs, n, KP, q, base, RB, act, w, part, G, 0);
+    bdots<VT, RPT>(j + 1, j, vj, V, bs, n, KP, q, base, RB, act, w, part, G, 0);
 }
 
 // Right-preconditioned step: v_j = W_src / h (stored) for the preconditioner.
@@ -294,7 +296,7 @@ bk_scale(int n, int KP, int K, const VT* __restrict__ Wsrc, VT* __restrict__ Vj,
 
 // W_dst = A z_j (z_j = M^-1 v_j, the preconditioned vector) and the partial
 // dots of W_dst with v_0..v_j.
-template <typename VT, typename MT>
+template <typename VT, typename MT, int RPT>
 __global__ void __launch_bounds__(BB_BS)
 bk_step_spmm_pc(int n, int KP, int K, int width, const int* __restrict__ col,
                 const MT* __restrict__ val, const VT* __restrict__ Zj, const VT* __restrict__ V,
@@ -303,11 +305,11 @@ bk_step_spmm_pc(int n, int KP, int K, int width, const int* __restrict__ col,
 {
     const int q = threadIdx.x & (KP - 1);
     const int RB = BB_BS / KP;
-    const int base = lblock() * RB * BB_RPT + (int)threadIdx.x / KP;
+    const int base = lblock() * RB * RPT + (int)threadIdx.x / KP;
     const bool act = q < K && cols[q].active;
-    VT w[BB_RPT];
+    VT w[RPT];
 #pragma unroll
-    for (int t = 0; t < BB_RPT; ++t) {
+    for (int t = 0; t < RPT; ++t) {
         const int row = base + t * RB;
         w[t] = VT(0);
         if (act && row < n) {
@@ -315,7 +317,7 @@ bk_step_spmm_pc(int n, int KP, int K, int width, const int* __restrict__ col,
             Wdst[(size_t)row * KP + q] = w[t];
         }
     }
-    bdots<VT>(j + 1, -1, nullptr, V, bs, n, KP, q, base, RB, act, w, part, G, 0);
+    bdots<VT, RPT>(j + 1, -1, nullptr, V, bs, n, KP, q, base, RB, act, w, part, G, 0);
 }
 
 // Sums the partials of the nd dots of a pass, one wavefront per (i, column):
@@ -338,30 +340,30 @@ bk_reduce(int KP, int K, int nd, int j, int pass, const double* __restrict__ par
 
 // w -= sum_{i<=j} hc_i v_i, then the partial dots with v_0..v_j (next pass)
 // or the partial of ||w||^2 (final pass).
-template <typename VT>
+template <typename VT, int RPT>
 __global__ void __launch_bounds__(BB_BS)
 bk_update(int n, int KP, int K, VT* __restrict__ W, const VT* __restrict__ V, size_t bs, int j,
           int final_pass, double* __restrict__ part, int G, const BCol* __restrict__ cols)
 {
     const int q = threadIdx.x & (KP - 1);
     const int RB = BB_BS / KP;
-    const int base = lblock() * RB * BB_RPT + (int)threadIdx.x / KP;
+    const int base = lblock() * RB * RPT + (int)threadIdx.x / KP;
     const bool act = q < K && cols[q].active;
-    VT w[BB_RPT];
+    VT w[RPT];
 #pragma unroll
-    for (int t = 0; t < BB_RPT; ++t) {
+    for (int t = 0; t < RPT; ++t) {
         const int row = base + t * RB;
         w[t] = (act && row < n) ? W[(size_t)row * KP + q] : VT(0);
     }
     for (int i0 = 0; i0 <= j; i0 += BB_LB) {
-        VT vb[BB_LB][BB_RPT];
+        VT vb[BB_LB][RPT];
         VT hv[BB_LB];
 #pragma unroll
         for (int u = 0; u < BB_LB; ++u) {
             const int i = i0 + u;
             hv[u] = (act && i <= j) ? (VT)cols[q].hc[i] : VT(0);
 #pragma unroll
-            for (int t = 0; t < BB_RPT; ++t) {
+            for (int t = 0; t < RPT; ++t) {
                 const int row = base + t * RB;
                 vb[u][t] = (act && i <= j && row < n) ? V[(size_t)i * bs + (size_t)row * KP + q]
                                                       : VT(0);
@@ -370,15 +372,15 @@ bk_update(int n, int KP, int K, VT* __restrict__ W, const VT* __restrict__ V, si
 #pragma unroll
         for (int u = 0; u < BB_LB; ++u)
 #pragma unroll
-            for (int t = 0; t < BB_RPT; ++t) w[t] = w[t] - hv[u] * vb[u][t];
+            for (int t = 0; t < RPT; ++t) w[t] = w[t] - hv[u] * vb[u][t];
     }
 #pragma unroll
-    for (int t = 0; t < BB_RPT; ++t) {
+    for (int t = 0; t < RPT; ++t) {
         const int row = base + t * RB;
         if (act && row < n) W[(size_t)row * KP + q] = w[t];
     }
-    if (final_pass) bdots<VT>(0, -1, nullptr, V, bs, n, KP, q, base, RB, act, w, part, G, j + 1);
-    else bdots<VT>(j + 1, -1, nullptr, V, bs, n, KP, q, base, RB, act, w, part, G, 0);
+    if (final_pass) bdots<VT, RPT>(0, -1, nullptr, V, bs, n, KP, q, base, RB, act, w, part, G, j + 1);
+    else bdots<VT, RPT>(j + 1, -1, nullptr, V, bs, n, KP, q, base, RB, act, w, part, G, 0);
 }
 
 // Givens rotation of step j for each column (one wavefront per column) from
@@ -455,19 +457,19 @@ bk_solve_y(int K, BCol* __restrict__ cols)
 }
 
 // X += V y per column (FP64 accumulation); X is n x K (the caller's layout).
-template <typename VT>
+template <typename VT, int RPT>
 __global__ void __launch_bounds__(BB_BS)
 bk_xupdate(int n, int KP, int K, const VT* __restrict__ V, size_t bs, double* __restrict__ X,
            const BCol* __restrict__ cols)
 {
     const int q = threadIdx.x & (KP - 1);
     const int RB = BB_BS / KP;
-    const int base = lblock() * RB * BB_RPT + (int)threadIdx.x / KP;
+    const int base = lblock() * RB * RPT + (int)threadIdx.x / KP;
     if (q >= K) return;
     const BCol* c = cols + q;
     const int k = c->k;
     if (c->converged || k == 0) return;
-    for (int t = 0; t < BB_RPT; ++t) {
+    for (int t = 0; t < RPT; ++t) {
         const int row = base + t * RB;
         if (row >= n) break;
         double s = 0.0;
@@ -477,7 +479,7 @@ bk_xupdate(int n, int KP, int K, const VT* __restrict__ V, size_t bs, double* __
 }
 
 // r = b - A x (FP64), W0 = r in the Krylov type, partial ||r||^2 per column.
-template <typename VT>
+template <typename VT, int RPT>
 __global__ void __launch_bounds__(BB_BS)
 bk_residual(int n, int KP, int K, int width, const int* __restrict__ col,
             const double* __restrict__ val, const double* __restrict__ B,
@@ -486,11 +488,11 @@ bk_residual(int n, int KP, int K, int width, const int* __restrict__ col,
 {
     const int q = threadIdx.x & (KP - 1);
     const int RB = BB_BS / KP;
-    const int base = lblock() * RB * BB_RPT + (int)threadIdx.x / KP;
+    const int base = lblock() * RB * RPT + (int)threadIdx.x / KP;
     const bool act = q < K && !cols[q].converged;
-    VT w[BB_RPT];
+    VT w[RPT];
 #pragma unroll
-    for (int t = 0; t < BB_RPT; ++t) {
+    for (int t = 0; t < RPT; ++t) {
         const int row = base + t * RB;
         w[t] = VT(0);
         if (act && row < n) {
@@ -513,24 +515,25 @@ bk_residual(int n, int KP, int K, int width, const int* __restrict__ col,
             W0[(size_t)row * KP + q] = w[t];
         }
     }
-    bdots<VT>(0, -1, nullptr, nullptr, 0, n, KP, q, base, RB, act, w, part, G, 0);
+    bdots<VT, RPT>(0, -1, nullptr, nullptr, 0, n, KP, q, base, RB, act, w, part, G, 0);
 }
 
 // ||b_q||^2 partials: the same reduction with B read as the vector.
+template <int RPT>
 __global__ void __launch_bounds__(BB_BS)
 bk_sumsq(int n, int KP, int K, const double* __restrict__ B, double* __restrict__ part, int G)
 {
     const int q = threadIdx.x & (KP - 1);
     const int RB = BB_BS / KP;
-    const int base = lblock() * RB * BB_RPT + (int)threadIdx.x / KP;
+    const int base = lblock() * RB * RPT + (int)threadIdx.x / KP;
     const bool act = q < K;
-    double w[BB_RPT];
+    double w[RPT];
 #pragma unroll
-    for (int t = 0; t < BB_RPT; ++t) {
+    for (int t = 0; t < RPT; ++t) {
         const int row = base + t * RB;
         w[t] = (act && row < n) ? B[(size_t)row * K + q] : 0.0;
     }
-    bdots<double>(0, -1, nullptr, nullptr, 0, n, KP, q, base, RB, act, w, part, G, 0);
+    bdots<double, RPT>(0, -1, nullptr, nullptr, 0, n, KP, q, base, RB, act, w, part, G, 0);
 }
 
 __global__ void bk_set_bnorm(int KP, int K, const double* __restrict__ part, int G,
@@ -619,7 +622,14 @@ int next_pow2(int k)
     return p;
 }
 
-int prepare_batch(phg_matrix_s* A, int slot, int prec, int m, int kp, bool pc, BatchWork** out)
+int blocks_for(int n, int kp, int rpt)
+{
+    const int rows_per_block = (BB_BS / kp) * rpt;
+    return std::max(1, (n + rows_per_block - 1) / rows_per_block);
+}
+
+int prepare_batch(phg_matrix_s* A, int slot, int prec, int m, int kp, int rpt, bool pc,
+                  BatchWork** out)
 {
     BatchWork*& w = A->batch[slot];
     const int n = A->n_rows;
@@ -631,24 +641,31 @@ int prepare_batch(phg_matrix_s* A, int slot, int prec, int m, int kp, bool pc, B
         b->m = m;
         b->kp = kp;
         b->prec = prec;
-        const int rows_per_block = (BB_BS / kp) * BB_RPT;
-        b->G = std::max(1, (n + rows_per_block - 1) / rows_per_block);
         const size_t es = prec == PHG_F64 ? 8 : 4;
         const size_t bs = (size_t)n * kp;
         PHG_HIP(b->V.alloc((size_t)(m + 1) * bs * es));
         PHG_HIP(b->W.alloc(2 * bs * es));
-        PHG_HIP(b->part.alloc((size_t)(m + 2) * kp * b->G));
+        // sized for the smallest rows-per-thread (most blocks)
+        PHG_HIP(b->part.alloc((size_t)(m + 2) * kp * blocks_for(n, kp, 1)));
         PHG_HIP(b->cols.alloc(kp));
         PHG_HIP(b->sum.alloc(kp));
         PHG_HIP(hipHostMalloc((void**)&b->host, sizeof(BSum) * kp, hipHostMallocDefault));
         w = b.release();
+    }
+    if (w->rpt != rpt) {  // another launch shape: new block count, new graph
+        w->rpt = rpt;
+        w->G = blocks_for(n, kp, rpt);
+        if (w->exec) {
+            PHG_HIP(hipGraphExecDestroy(w->exec));
+            w->exec = nullptr;
+        }
     }
     if (pc && !w->Z.p) PHG_HIP(w->Z.alloc((size_t)w->m * w->n * w->kp * (prec == PHG_F64 ? 8 : 4)));
     *out = w;
     return PHG_OK;
 }
 
-template <typename VT, typename MT>
+template <typename VT, typename MT, int RPT>
 void enqueue_batch_cycle(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K,
                          const double* B, double* X, int m, int passes, double rtol,
                          double inner_rtol, int max_iters, phg_mg_s* pc, hipStream_t s)
@@ -672,33 +689,33 @@ void enqueue_batch_cycle(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K,
             hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_scale<VT>), dim3((unsigned)((bs + BB_BS - 1) / BB_BS)),
                                block, 0, s, n, KP, K, src, Vj, j, cols);
             precond_apply<VT>(pc, KP, Vj, Zj, s);
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_step_spmm_pc<VT, MT>), grid, block, 0, s, n, KP,
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_step_spmm_pc<VT, MT, RPT>), grid, block, 0, s, n, KP,
                                K, E.width, E.col.p, E.val.p, Zj, V, bs, dst, j, part, G, cols);
         } else {
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_step_spmm<VT, MT>), grid, block, 0, s, n, KP, K,
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_step_spmm<VT, MT, RPT>), grid, block, 0, s, n, KP, K,
                                E.width, E.col.p, E.val.p, src, V, bs, dst, j, part, G, cols);
         }
         for (int p = 0; p < passes; ++p) {
             const int nd = j + 1;
             hipLaunchKernelGGL(bk_reduce, dim3((nd * KP + 3) / 4), block, 0, s, KP, K, nd, j, p,
                                part, G, cols);
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_update<VT>), grid, block, 0, s, n, KP, K, dst,
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_update<VT, RPT>), grid, block, 0, s, n, KP, K, dst,
                                V, bs, j, p == passes - 1 ? 1 : 0, part, G, cols);
         }
         hipLaunchKernelGGL(bk_givens, dim3(cb), block, 0, s, KP, K, j, m, part, G, cols, rtol,
                            inner_rtol, max_iters);
     }
     hipLaunchKernelGGL(bk_solve_y, dim3(cb), block, 0, s, K, cols);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_xupdate<VT>), grid, block, 0, s, n, KP, K,
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_xupdate<VT, RPT>), grid, block, 0, s, n, KP, K,
                        pc ? (const VT*)w->Z.p : (const VT*)V, bs, X, cols);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_residual<VT>), grid, block, 0, s, n, KP, K,
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_residual<VT, RPT>), grid, block, 0, s, n, KP, K,
                        A->ell64.width, A->ell64.col.p, A->ell64.val.p, B, X, W0, part, G, cols);
     hipLaunchKernelGGL(bk_cycle_start, dim3(cb), block, 0, s, KP, K, part, G, cols, w->sum.p,
                        rtol, max_iters);
 }
 
-template <typename VT, typename MT>
-int batch_run(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const double* B, double* X,
+template <typename VT, typename MT, int RPT>
+int batch_run_t(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const double* B, double* X,
               const phg_gmres_opts& o, phg_gmres_info* infos, phg_mg_s* pc, hipStream_t s)
 {
     const int n = w->n, KP = w->kp, G = w->G, m = o.restart;
@@ -716,9 +733,9 @@ int batch_run(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const doub
         const size_t tot = (size_t)n * K;
         hipLaunchKernelGGL(bk_fill0, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, tot, X);
     }
-    hipLaunchKernelGGL(bk_sumsq, grid, block, 0, s, n, KP, K, B, part, G);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_sumsq<RPT>), grid, block, 0, s, n, KP, K, B, part, G);
     hipLaunchKernelGGL(bk_set_bnorm, dim3(cb), block, 0, s, KP, K, part, G, cols);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_residual<VT>), grid, block, 0, s, n, KP, K,
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_residual<VT, RPT>), grid, block, 0, s, n, KP, K,
                        A->ell64.width, A->ell64.col.p, A->ell64.val.p, B, X, W0, part, G, cols);
     hipLaunchKernelGGL(bk_cycle_start, dim3(cb), block, 0, s, KP, K, part, G, cols, w->sum.p,
                        o.rtol, o.max_iters);
@@ -738,7 +755,7 @@ int batch_run(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const doub
             if (!w->cap) PHG_HIP(hipStreamCreateWithFlags(&w->cap, hipStreamNonBlocking));
             hipGraph_t g = nullptr;
             PHG_HIP(hipStreamBeginCapture(w->cap, hipStreamCaptureModeThreadLocal));
-            enqueue_batch_cycle<VT, MT>(A, E, w, K, B, X, m, passes, o.rtol, inner, o.max_iters,
+            enqueue_batch_cycle<VT, MT, RPT>(A, E, w, K, B, X, m, passes, o.rtol, inner, o.max_iters,
                                         pc, w->cap);
             PHG_HIP(hipStreamEndCapture(w->cap, &g));
             hipError_t e = hipGraphInstantiate(&w->exec, g, nullptr, nullptr, 0);
@@ -763,7 +780,7 @@ int batch_run(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const doub
         if (o.graph) {
             PHG_HIP(hipGraphLaunch(w->exec, s));
         } else {
-            enqueue_batch_cycle<VT, MT>(A, E, w, K, B, X, m, passes, o.rtol, inner, o.max_iters,
+            enqueue_batch_cycle<VT, MT, RPT>(A, E, w, K, B, X, m, passes, o.rtol, inner, o.max_iters,
                                         pc, s);
             PHG_HIP(hipGetLastError());
         }
@@ -781,6 +798,66 @@ int batch_run(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const doub
         }
     }
     return all ? PHG_OK : PHG_ERR_NOCONV;
+}
+
+template <typename VT, typename MT>
+int batch_run(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const double* B, double* X,
+              const phg_gmres_opts& o, phg_gmres_info* infos, phg_mg_s* pc, hipStream_t s)
+{
+    switch (w->rpt) {
+    case 1: return batch_run_t<VT, MT, 1>(A, E, w, K, B, X, o, infos, pc, s);
+    case 2: return batch_run_t<VT, MT, 2>(A, E, w, K, B, X, o, infos, pc, s);
+    case 4: return batch_run_t<VT, MT, 4>(A, E, w, K, B, X, o, infos, pc, s);
+    default: return batch_run_t<VT, MT, BB_RPT>(A, E, w, K, B, X, o, infos, pc, s);
+    }
+}
+
+// ------------------------------------------------------- launch shapes --
+// Rows per thread of the row kernels for an (n, KP, precision) shape.  A
+// shape's value fixes the block partials of every dot product, so it is kept
+// per process: every matrix of that shape uses the same value and a column's
+// result does not depend on which object solved it.  Untuned shapes take
+// the largest value that still gives >= 1024 workgroups (4 per CU), or 1.
+struct ShapeKey {
+    int n, kp, prec;
+    bool operator<(const ShapeKey& o) const
+    {
+        return n != o.n ? n < o.n : kp != o.kp ? kp < o.kp : prec < o.prec;
+    }
+};
+std::mutex g_shape_mu;
+std::map<ShapeKey, int> g_shape_rpt;
+
+int default_rpt(int n, int kp)
+{
+    for (int r = BB_RPT; r > 1; r >>= 1)
+        if (blocks_for(n, kp, r) >= 1024) return r;
+    return 1;
+}
+
+int shape_rpt(int n, int kp, int prec)
+{
+    std::lock_guard<std::mutex> g(g_shape_mu);
+    auto it = g_shape_rpt.find({n, kp, prec});
+    return it == g_shape_rpt.end() ? default_rpt(n, kp) : it->second;
+}
+
+void set_shape_rpt(int n, int kp, int prec, int rpt)
+{
+    std::lock_guard<std::mutex> g(g_shape_mu);
+    g_shape_rpt[{n, kp, prec}] = rpt;
+}
+
+// Deterministic right-hand sides for timing: values in [-1, 1) from a hash.
+__global__ void bk_fill_hash(size_t n, double* __restrict__ x)
+{
+    const size_t i = (size_t)lblock() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long h = (i + 1) * 0x9E3779B97F4A7C15ull;
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 29;
+    x[i] = (double)(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
 }
 
 }  // namespace
@@ -833,7 +910,7 @@ int gmres_batch_solve(phg_matrix_s* A, int K, const double* B, double* X,
     BatchWork* w = nullptr;
     const int slot = o.precision == PHG_F64 ? 0 : 1;
     rc = prepare_batch(A, slot, o.precision == PHG_F64 ? PHG_F64 : PHG_F32, o.restart, kp,
-                       pc != nullptr, &w);
+                       shape_rpt(A->n_rows, kp, o.precision), pc != nullptr, &w);
     if (rc) return rc;
     if (o.precision == PHG_F64)
         return batch_run<double, double>(A, A->ell64, w, K, B, X, o, infos, pc, s);
@@ -845,6 +922,91 @@ int gmres_batch_solve(phg_matrix_s* A, int K, const double* B, double* X,
     rc = ensure_ell16(A, s);
     if (rc) return rc;
     return batch_run<float, uint16_t>(A, A->ell16, w, K, B, X, o, infos, nullptr, s);
+}
+
+int batch_shape_get(int n, int K, int prec)
+{
+    return shape_rpt(n, next_pow2(K), prec);
+}
+
+int batch_shape_set(int n, int K, int prec, int rpt)
+{
+    if (n < 1 || K < 1 || K > BB_MAXK || prec < PHG_F64 || prec > PHG_BF16 ||
+        (rpt != 0 && rpt != 1 && rpt != 2 && rpt != 4 && rpt != 8)) {
+        set_error("gmres_batch_set_shape: bad argument (rows per thread 0, 1, 2, 4 or 8)");
+        return PHG_ERR_ARG;
+    }
+    if (rpt == 0) {
+        std::lock_guard<std::mutex> g(g_shape_mu);
+        g_shape_rpt.erase({n, next_pow2(K), prec});
+    } else {
+        set_shape_rpt(n, next_pow2(K), prec, rpt);
+    }
+    return PHG_OK;
+}
+
+// Autotuner: times one restart cycle of `K` hash-valued right-hand sides for
+// each rows-per-thread value and keeps the fastest for the shape.
+int gmres_batch_tune(phg_matrix_s* A, int K, const phg_gmres_opts* opts, hipStream_t s,
+                     int* best_out, double* ms_out)
+{
+    static const int cand[4] = {1, 2, 4, 8};
+    const int n = A->n_rows;
+    if (K < 1 || K > BB_MAXK || n < 1) {
+        set_error("gmres_batch_tune: nrhs must be in 1..64 and the matrix non-empty");
+        return PHG_ERR_ARG;
+    }
+    phg_gmres_opts o = *opts;
+    o.max_iters = o.restart;   // exactly one cycle of `restart` steps per column
+    o.rtol = 0.0;
+    o.inner_rtol = 0.0;
+    o.use_x0 = 0;
+    o.graph = 1;
+    DevBuf<double> B, X;
+    PHG_HIP(B.alloc((size_t)n * K));
+    PHG_HIP(X.alloc((size_t)n * K));
+    const size_t tot = (size_t)n * K;
+    hipLaunchKernelGGL(bk_fill_hash, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, tot,
+                       B.p);
+    PHG_HIP(hipGetLastError());
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    PHG_HIP(hipEventCreate(&e0));
+    hipError_t he = hipEventCreate(&e1);
+    if (he != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        PHG_HIP(he);
+    }
+    const int kp = next_pow2(K);
+    int best = BB_RPT, rc = PHG_OK;
+    float best_ms = 0.0f;
+    for (int c = 0; c < 4 && rc == PHG_OK; ++c) {
+        set_shape_rpt(n, kp, o.precision, cand[c]);
+        float t = 0.0f;
+        for (int rep = 0; rep < 4 && rc == PHG_OK; ++rep) {  // rep 0 captures the graph
+            if (hipEventRecord(e0, s) != hipSuccess) rc = PHG_ERR_HIP;
+            int r = gmres_batch_solve(A, K, B.p, X.p, &o, nullptr, s);
+            if (r != PHG_OK && r != PHG_ERR_NOCONV) rc = r;
+            if (rc == PHG_OK && (hipEventRecord(e1, s) != hipSuccess ||
+                                 hipEventSynchronize(e1) != hipSuccess))
+                rc = PHG_ERR_HIP;
+            float ms = 0.0f;
+            if (rc == PHG_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = PHG_ERR_HIP;
+            if (rc == PHG_OK && rep > 0 && (rep == 1 || ms < t)) t = ms;
+        }
+        if (rc == PHG_OK) {
+            if (ms_out) ms_out[c] = t;
+            if (c == 0 || t < best_ms) {
+                best_ms = t;
+                best = cand[c];
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == PHG_ERR_HIP) set_error("gmres_batch_tune: HIP event error");
+    set_shape_rpt(n, kp, o.precision, rc == PHG_OK ? best : default_rpt(n, kp));
+    if (best_out) *best_out = best;
+    return rc;
 }
 
 }  // namespace phg

@@ -114,6 +114,10 @@ int mg_rows(const phg_mg_s* M);
 int gmres_batch_solve(phg_matrix_s* A, int K, const double* B, double* X,
                       const phg_gmres_opts* opts, phg_gmres_info* infos, hipStream_t s,
                       phg_mg_s* pc = nullptr);
+int gmres_batch_tune(phg_matrix_s* A, int K, const phg_gmres_opts* opts, hipStream_t s,
+                     int* best, double* ms);
+int batch_shape_get(int n, int K, int prec);
+int batch_shape_set(int n, int K, int prec, int rpt);
 int ensure_mesh_device(phg_mesh_s* m);
 int ensure_ell64(phg_matrix_s* A, hipStream_t s);
 int ensure_ell32(phg_matrix_s* A, hipStream_t s);

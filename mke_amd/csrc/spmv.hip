@@ -143,6 +143,22 @@ __global__ void p2u_kernel(int sz, const int* __restrict__ isb, const int* __res
     else u[i] = p[p2io[i]];
 }
 
+// F[k] = F1[outer[k]]  (proj_bnd)
+__global__ void proj_bnd_kernel(int os, const int* __restrict__ outer,
+                                const double* __restrict__ F1, double* __restrict__ F)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < os) F[i] = F1[outer[i]];
+}
+
+// u[i] = bnd ? bnd[p2io[i]] : 0 at boundary points, inner points untouched (set_bnd)
+__global__ void set_bnd_kernel(int sz, const int* __restrict__ isb, const int* __restrict__ p2io,
+                               const double* __restrict__ bnd, double* __restrict__ u)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < sz && isb[i]) u[i] = bnd ? bnd[p2io[i]] : 0.0;
+}
+
 int max_row(const phg_matrix_s* A)
 {
     int w = 0;
@@ -272,6 +288,30 @@ int p2u_f64(phg_mesh_s* m, const double* p, const double* bnd, double* u, hipStr
     if (sz)
         hipLaunchKernelGGL(p2u_kernel, dim3((sz + 255) / 256), dim3(256), 0, s, sz, m->d_isb.p,
                            m->d_p2io.p, p, bnd, u);
+    PHG_HIP(hipGetLastError());
+    return PHG_OK;
+}
+
+int proj_bnd_f64(phg_mesh_s* m, const double* u, double* b, hipStream_t s)
+{
+    int rc = ensure_mesh_device(m);
+    if (rc) return rc;
+    const int os = (int)m->h.outer.size();
+    if (os)
+        hipLaunchKernelGGL(proj_bnd_kernel, dim3((os + 255) / 256), dim3(256), 0, s, os,
+                           m->d_outer.p, u, b);
+    PHG_HIP(hipGetLastError());
+    return PHG_OK;
+}
+
+int set_bnd_f64(phg_mesh_s* m, const double* bnd, double* u, hipStream_t s)
+{
+    int rc = ensure_mesh_device(m);
+    if (rc) return rc;
+    const int sz = m->h.n_points;
+    if (sz)
+        hipLaunchKernelGGL(set_bnd_kernel, dim3((sz + 255) / 256), dim3(256), 0, s, sz,
+                           m->d_isb.p, m->d_p2io.p, bnd, u);
     PHG_HIP(hipGetLastError());
     return PHG_OK;
 }

@@ -10,6 +10,7 @@ for device memory and streams); all arithmetic runs in the HIP library.
 There is no CPU fallback: if the library is missing, import fails.
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -71,10 +72,15 @@ SIGNATURES = {
     "phg_axpby": (_I, [_I, _D, _P, _D, _P, _P, _P]),
     "phg_u2p": (_I, [_P, _P, _P, _P]),
     "phg_p2u": (_I, [_P, _P, _P, _P, _P]),
+    "phg_proj_bnd": (_I, [_P, _P, _P, _P]),
+    "phg_set_bnd": (_I, [_P, _P, _P, _P]),
     "phg_gmres_default_opts": (None, [C.POINTER(GmresOpts)]),
     "phg_gmres": (_I, [_P, _P, _P, C.POINTER(GmresOpts), C.POINTER(GmresInfo), _P]),
     "phg_block_gmres": (_I, [_P, _I, _P, _P, C.POINTER(GmresOpts), C.POINTER(BlockInfo), _P]),
     "phg_gmres_batch": (_I, [_P, _I, _P, _P, C.POINTER(GmresOpts), C.POINTER(GmresInfo), _P]),
+    "phg_gmres_batch_tune": (_I, [_P, _I, C.POINTER(GmresOpts), _P, _PI, _PD]),
+    "phg_gmres_batch_shape": (_I, [_I, _I, _I]),
+    "phg_gmres_batch_set_shape": (_I, [_I, _I, _I, _I]),
     "phg_mg_create": (_I, [_P, _I, _D, _D, _P, C.POINTER(_P)]),
     "phg_mg_destroy": (None, [_P]),
     "phg_mg_matrix": (_I, [_P, C.POINTER(_P)]),
@@ -154,6 +160,19 @@ def _pi(a):
 
 def _pd(a):
     return a.ctypes.data_as(_PD)
+
+
+def batch_shape(n_rows, nrhs, precision=F64):
+    """Rows per thread the batched solves use for this shape (process-wide)."""
+    return check(lib().phg_gmres_batch_shape(int(n_rows), int(nrhs), int(precision)),
+                 "phg_gmres_batch_shape")
+
+
+def set_batch_shape(n_rows, nrhs, precision, rows_per_thread):
+    """Pins the rows per thread of a batched-solve shape (1, 2, 4 or 8; 0
+    returns it to the untuned default)."""
+    check(lib().phg_gmres_batch_set_shape(int(n_rows), int(nrhs), int(precision),
+                                          int(rows_per_thread)), "phg_gmres_batch_set_shape")
 
 
 def sphere_mesh_text(level):
@@ -346,6 +365,20 @@ class Matrix:
         check(rc, "phg_gmres_batch")
         return [dict(iterations=i.iterations, restarts=i.restarts, refinements=i.refinements,
                      rel_residual=i.rel_residual, converged=bool(i.converged)) for i in infos]
+
+    def tune_batch(self, nrhs, restart=30, ortho=CGS2, precision=F64, stream=None):
+        """Autotunes the rows per thread of the batched solves for this
+        matrix's (n, nrhs, precision) shape: times one restart cycle per
+        candidate (1, 2, 4, 8), keeps the fastest for the process and returns
+        (rows_per_thread, {candidate: ms})."""
+        o = GmresOpts()
+        lib().phg_gmres_default_opts(C.byref(o))
+        o.restart, o.ortho, o.precision = restart, ortho, precision
+        best = C.c_int(0)
+        ms = (C.c_double * 4)()
+        check(lib().phg_gmres_batch_tune(self.h, int(nrhs), C.byref(o), _stream(stream),
+                                         C.byref(best), ms), "phg_gmres_batch_tune")
+        return best.value, {r: ms[i] for i, r in enumerate((1, 2, 4, 8))}
 
     def close(self):
         if self.h:
@@ -657,14 +690,93 @@ def helmholtz_coeffs(tau=1.0, sigma=0.0, theta=1.0, mu=1e-2):
 
 class SphereLaplace:
     """SphereLaplace<T, Matrix> (src/slaplace.h:60-108): idt_ (mass),
-    laplace_ (stiffness) and the boundary matrices, assembled on the GPU."""
+    laplace_ (stiffness) and the boundary matrices, assembled on the GPU, with
+    calc1/calc2/solve (src/impl/slaplace_impl.h:84-192).  Vectors are FP64
+    device tensors: full-mesh (V) for F, bnd of outer size."""
 
     def __init__(self, mesh, stream=None):
         self.m = mesh
         self.idt_ = Matrix.assemble(mesh, MASS, stream=stream)
         self.laplace_ = Matrix.assemble(mesh, LAPLACE, stream=stream)
         self.bnd1_ = Matrix.assemble_boundary(mesh, MASS)       # laplace_bnd1_cb
+        self.bnd2_ = Matrix.assemble_boundary(mesh, HELMHOLTZ, 0.0, -1.0)  # laplace_bnd2_cb
         self.bnd3_ = Matrix.assemble_boundary(mesh, LAPLACE)    # slaplace_integrate_cb
+        self._mg = None
+        self.stream = stream
+
+    def _vec(self, n):
+        import torch
+        return torch.zeros(n, dtype=torch.float64, device="cuda")
+
+    def calc2(self, ans, F, rtol=1e-13):
+        """ans (inner) = idt^-1 (laplace_ u2p(F) + bnd3_ proj_bnd(F))."""
+        s = _stream(self.stream)
+        rs, os_ = self.m.inner_size, self.m.outer_size
+        inn, out, tmp = self._vec(rs), self._vec(rs), self._vec(max(os_, 1))
+        check(lib().phg_u2p(self.m.h, _ptr(F), _ptr(inn), s), "u2p")
+        check(lib().phg_proj_bnd(self.m.h, _ptr(F), _ptr(tmp), s), "proj_bnd")
+        self.laplace_.mult_vector(out, inn, self.stream)
+        self.bnd3_.mult_vector(inn, tmp, self.stream)
+        check(lib().phg_axpby(rs, 1.0, _ptr(out), 1.0, _ptr(inn), _ptr(out), s), "vec_sum")
+        return self.idt_.solve(ans, out, ortho=CGS2, rtol=rtol, stream=self.stream)
+
+    def calc1(self, ans, F, bnd=None, rtol=1e-13):
+        """calc2, then p2u with the boundary values bnd (None: zero)."""
+        p = self._vec(self.m.inner_size)
+        info = self.calc2(p, F, rtol)
+        check(lib().phg_p2u(self.m.h, _ptr(p), _ptr(bnd) if bnd is not None else None,
+                            _ptr(ans), _stream(self.stream)), "p2u")
+        return info
+
+    def solve(self, ans, F, bnd=None, rtol=1e-10, max_iters=2000):
+        """laplace_ x = idt_ u2p(F) (+ bnd2_ bnd), ans = p2u(x, bnd): the
+        Poisson solve, multigrid-preconditioned GMRES."""
+        s = _stream(self.stream)
+        rs = self.m.inner_size
+        inn, b = self._vec(rs), self._vec(rs)
+        check(lib().phg_u2p(self.m.h, _ptr(F), _ptr(inn), s), "u2p")
+        self.idt_.mult_vector(b, inn, self.stream)
+        if bnd is not None:
+            self.bnd2_.mult_vector(inn, bnd, self.stream)
+            check(lib().phg_axpby(rs, 1.0, _ptr(b), 1.0, _ptr(inn), _ptr(b), s), "vec_sum")
+        if self._mg is None:
+            self._mg = Multigrid(self.m, LAPLACE)
+        x = self._vec(rs)
+        info = self._mg.solve_batch(x.reshape(-1, 1), b.reshape(-1, 1), rtol=rtol,
+                                    max_iters=max_iters)[0]
+        check(lib().phg_p2u(self.m.h, _ptr(x), _ptr(bnd) if bnd is not None else None,
+                            _ptr(ans), s), "p2u")
+        return info
+
+
+class SphereNorm:
+    """SphereNorm<T> (src/norm.h:58-120): NORM_ = generate_full_matrix with
+    sphere_scalar_cb; scalar(u, v) = u . NORM_ v (fast_scalar,
+    src/mesh.h:550-591), norm = sqrt(scalar(u, u)), dist = norm(u - v)."""
+
+    def __init__(self, mesh, stream=None):
+        self.m = mesh
+        self.NORM_ = Matrix.assemble_full(mesh, MASS, stream=stream)
+        self.stream = stream
+
+    def scalar(self, u, v):
+        import torch
+        tmp = torch.zeros(self.m.n_points, dtype=torch.float64, device="cuda")
+        self.NORM_.mult_vector(tmp, v, self.stream)
+        r = C.c_double()
+        check(lib().phg_dot(self.m.n_points, _ptr(u), _ptr(tmp), C.byref(r),
+                            _stream(self.stream)), "vec_scalar2")
+        return r.value
+
+    def norm(self, u):
+        return math.sqrt(self.scalar(u, u))
+
+    def dist(self, u, v):
+        import torch
+        d = torch.zeros(self.m.n_points, dtype=torch.float64, device="cuda")
+        check(lib().phg_axpby(self.m.n_points, 1.0, _ptr(u), -1.0, _ptr(v), _ptr(d),
+                              _stream(self.stream)), "vec_diff")
+        return self.norm(d)
 
 
 class Helmholtz:

@@ -44,6 +44,54 @@ def real_sph_harm_field(xyz, lmax, seed, decay=0.0):
     return f
 
 
+def real_sph_harm_field_lean(xyz, lmax, seed, decay=0.0, device=None):
+    """The field of real_sph_harm_field (same coefficients, drawn in the same
+    (l, m) order) with O(n) memory: the Legendre recurrence runs per m with
+    two rolling rows, so degree 64 on 655,362 points needs a few vectors
+    instead of one per (l, m).  Sums in m-major order (differs from
+    real_sph_harm_field by rounding).  With `device`, evaluates in torch
+    FP64 on that device and returns a numpy array."""
+    rng = np.random.default_rng(seed)
+    coef = {}
+    for l in range(lmax + 1):
+        scale = max(l, 1) ** (-decay)
+        for m in range(-l, l + 1):
+            coef[(l, m)] = rng.standard_normal() * scale
+    if device is not None:
+        import torch
+        xp = torch
+        t = torch.as_tensor(np.ascontiguousarray(xyz), dtype=torch.float64, device=device)
+        x, y, z = t[:, 0], t[:, 1], t[:, 2]
+        ct = torch.clamp(z, -1.0, 1.0)
+        st = torch.sqrt(torch.clamp(1.0 - ct * ct, min=0.0))
+        phi = torch.atan2(y, x)
+    else:
+        xp = np
+        x, y, z = xyz[:, 0], xyz[:, 1], xyz[:, 2]
+        ct = np.clip(z, -1.0, 1.0)
+        st = np.sqrt(np.maximum(0.0, 1.0 - ct * ct))
+        phi = np.arctan2(y, x)
+    f = xp.zeros_like(ct)
+    pmm = xp.ones_like(ct)
+    for m in range(lmax + 1):
+        if m > 0:
+            pmm = -pmm * (2 * m - 1) * st
+        cosm, sinm = xp.cos(m * phi), xp.sin(m * phi)
+        p2, p1 = None, pmm
+        for l in range(m, lmax + 1):
+            if l == m + 1:
+                p2, p1 = p1, ct * (2 * m + 1) * pmm
+            elif l > m + 1:
+                p2, p1 = p1, ((2 * l - 1) * ct * p1 - (l + m - 1) * p2) / (l - m)
+            norm = math.sqrt((2 * l + 1) / (4 * math.pi) * math.factorial(l - m) / math.factorial(l + m))
+            if m == 0:
+                f = f + coef[(l, 0)] * norm * p1
+            else:
+                f = f + (coef[(l, m)] * math.sqrt(2) * norm) * p1 * cosm
+                f = f + (coef[(l, -m)] * math.sqrt(2) * norm) * p1 * sinm
+    return f.cpu().numpy() if device is not None else f
+
+
 @functools.lru_cache(maxsize=8)
 def oracle_mesh(level):
     from oracle import mesh_ref, oracle as O

@@ -1,0 +1,167 @@
+// Drives the C++ host classes of include/phelm_gpu.hpp on one sphere mesh
+// and writes their outputs as raw FP64 files for the GPU test to compare
+// with the oracle and with the Python host classes:
+//
+//   mirror_check LEVEL OUTDIR
+//
+// Files (little-endian doubles): spmv_add.bin / spmv_asm.bin (Solver built
+// by add() vs the GPU-assembled one, same x), solve_x.bin (Solver::solve of
+// the Helmholtz system), lap_calc2.bin, lap_calc1.bin, lap_solve.bin
+// (SphereLaplace), jac_calc2.bin, jac_calc2t.bin (SphereJacobian),
+// bv_u1.bin (one BarVortex::calc step), inputs u.bin, v.bin, x.bin, psi.bin
+// and summary.txt (scalars).
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "phelm_gpu.hpp"
+
+using namespace phelm_gpu;
+
+static void dump(const std::string& dir, const char* name, const std::vector<double>& v)
+{
+    const std::string p = dir + "/" + name;
+    FILE* f = std::fopen(p.c_str(), "wb");
+    if (!f || std::fwrite(v.data(), sizeof(double), v.size(), f) != v.size()) {
+        std::fprintf(stderr, "cannot write %s\n", p.c_str());
+        std::exit(2);
+    }
+    std::fclose(f);
+}
+
+static std::vector<double> from_dev(const ArrayDevice<double>& a) { return a.host(); }
+
+static ArrayDevice<double> to_dev(const std::vector<double>& h)
+{
+    ArrayDevice<double> a(h.size());
+    a.upload(h.data());
+    return a;
+}
+
+int main(int argc, char** argv)
+{
+    if (argc != 3) {
+        std::fprintf(stderr, "usage: mirror_check LEVEL OUTDIR\n");
+        return 2;
+    }
+    const int level = std::atoi(argv[1]);
+    const std::string out = argv[2];
+    try {
+        check(phg_init(0), "phg_init");
+        Mesh m = Mesh::sphere(level);
+        const int V = m.ps_size(), n = m.inner_size();
+        std::vector<double> pr((size_t)V * 3);
+        check(phg_mesh_arrays(m.handle(), pr.data(), nullptr, nullptr, nullptr, nullptr, nullptr,
+                              nullptr, nullptr, nullptr, nullptr),
+              "phg_mesh_arrays");
+        std::vector<int> inner((size_t)n);
+        check(phg_mesh_arrays(m.handle(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                              nullptr, inner.data(), nullptr, nullptr),
+              "phg_mesh_arrays");
+
+        // smooth test fields on every point
+        std::vector<double> u((size_t)V), v((size_t)V), psi((size_t)V);
+        int pole = 0;
+        for (int i = 0; i < V; ++i) {
+            const double x = pr[3 * i], y = pr[3 * i + 1], z = pr[3 * i + 2];
+            u[i] = std::sin(2.0 * x) + 0.5 * y * z;
+            v[i] = std::cos(1.5 * y) * z + 0.25 * x;
+            const double ph = std::asin(std::fmax(-1.0, std::fmin(1.0, z)));
+            const double la = std::atan2(y, x);
+            const double w = 7.848e-6 / 7.292e-5;
+            psi[i] = -w * std::sin(ph) + w * std::pow(std::cos(ph), 4) * std::sin(ph) *
+                                             std::cos(4.0 * la);
+            if (z < pr[3 * pole + 2]) pole = i;
+        }
+        const double p0 = psi[pole];  // the south pole (Dirichlet point) is 0
+        for (int i = 0; i < V; ++i) psi[i] -= p0;
+        dump(out, "u.bin", u);
+        dump(out, "v.bin", v);
+        dump(out, "psi.bin", psi);
+        ArrayDevice<double> du = to_dev(u), dv = to_dev(v), dpsi = to_dev(psi);
+
+        // Solver<double>: add() in reverse row order vs the assembled matrix
+        const double cm = 1.0, cl = -1e-2;
+        Solver<double> Aasm = Solver<double>::assemble(m, PHG_HELMHOLTZ, cm, cl);
+        long nnz = 0;
+        int nr = 0, nc = 0;
+        check(phg_matrix_info(Aasm.handle(), &nr, &nc, &nnz), "phg_matrix_info");
+        std::vector<int> rp((size_t)n + 1), ci((size_t)nnz);
+        std::vector<double> vals((size_t)nnz);
+        check(phg_matrix_csr(Aasm.handle(), rp.data(), ci.data(), vals.data()), "phg_matrix_csr");
+        Solver<double> Aadd(n);
+        for (int i = n - 1; i >= 0; --i)
+            for (int k = rp[i + 1] - 1; k >= rp[i]; --k) Aadd.add(i, ci[k], vals[k]);
+        std::vector<double> xh((size_t)n);
+        for (int i = 0; i < n; ++i) xh[i] = std::sin(0.001 * i) + 0.1;
+        dump(out, "x.bin", xh);
+        ArrayDevice<double> dx = to_dev(xh), y1((size_t)n), y2((size_t)n);
+        Aadd.mult_vector(y1.data(), dx.data());
+        Aasm.mult_vector(y2.data(), dx.data());
+        dump(out, "spmv_add.bin", from_dev(y1));
+        dump(out, "spmv_asm.bin", from_dev(y2));
+        ArrayDevice<double> sol((size_t)n);
+        Aadd.solve(sol.data(), y2.data());
+        dump(out, "solve_x.bin", from_dev(sol));
+        const phg_gmres_info si = Aadd.last_info();
+
+        // SphereLaplace
+        SphereLaplace<> L(m);
+        ArrayDevice<double> c2((size_t)n), c1((size_t)V), ls((size_t)V);
+        L.calc2(c2.data(), du.data());
+        L.calc1(c1.data(), du.data(), nullptr);
+        L.solve(ls.data(), dv.data(), nullptr);
+        dump(out, "lap_calc2.bin", from_dev(c2));
+        dump(out, "lap_calc1.bin", from_dev(c1));
+        dump(out, "lap_solve.bin", from_dev(ls));
+        const phg_gmres_info li = L.last_info();
+
+        // SphereNorm
+        SphereNorm<> N(m);
+        const double nu = N.norm(du.data()), suv = N.scalar(du.data(), dv.data()),
+                     duv = N.dist(du.data(), dv.data());
+
+        // SphereJacobian
+        SphereJacobian J(m);
+        ArrayDevice<double> j2((size_t)n), j2t((size_t)n);
+        J.calc2(j2.data(), du.data(), dv.data());
+        J.calc2t(j2t.data(), du.data(), dv.data());
+        dump(out, "jac_calc2.bin", from_dev(j2));
+        dump(out, "jac_calc2t.bin", from_dev(j2t));
+
+        // BarVortex: one step of the Rossby-Haurwitz state
+        std::vector<double> lh((size_t)V);
+        for (int i = 0; i < V; ++i) lh[i] = 2.0 * std::fmax(-1.0, std::fmin(1.0, pr[3 * i + 2]));
+        ArrayDevice<double> dlh = to_dev(lh), u1((size_t)V);
+        BarVortex bv(m, BarVortexConf{600.0 * 7.292e-5, 1.6e-2, 8e-5, 1.0, 1.0, 0.5}, dlh.data());
+        bv.calc(u1.data(), dpsi.data(), nullptr, nullptr, 0.0);
+        dump(out, "bv_u1.bin", from_dev(u1));
+        const phg_bv_info bi = bv.last_info();
+        bool threw = false;
+        try {
+            bv.calc(u1.data(), dpsi.data(), dpsi.data(), nullptr, 0.0);
+        } catch (const Error& e) {
+            threw = e.code() == PHG_ERR_ARG;
+        }
+
+        const std::string sp = out + "/summary.txt";
+        FILE* f = std::fopen(sp.c_str(), "w");
+        if (!f) return 2;
+        std::fprintf(f, "level %d\npoints %d\ninner %d\nnnz %ld\n", level, V, n, nnz);
+        std::fprintf(f, "solve_converged %d\nsolve_iterations %d\nsolve_rel %.17g\n", si.converged,
+                     si.iterations, si.rel_residual);
+        std::fprintf(f, "lap_solve_converged %d\nlap_solve_rel %.17g\n", li.converged,
+                     li.rel_residual);
+        std::fprintf(f, "norm_u %.17g\nscalar_uv %.17g\ndist_uv %.17g\n", nu, suv, duv);
+        std::fprintf(f, "bv_converged %d\nbv_iterations %d\n", bi.converged, bi.iterations);
+        std::fprintf(f, "bv_bnd_throws %d\n", threw ? 1 : 0);
+        std::fclose(f);
+        std::printf("mirror_check level %d ok\n", level);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "mirror_check: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

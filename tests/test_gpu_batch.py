@@ -143,3 +143,33 @@ def test_jacobian_calc2_vs_oracle(level):
     onet = torch.zeros_like(one)
     J.calc2t(onet, _dev(us[1]), _dev(vs[1]))
     assert torch.equal(onet, -one)
+
+
+def test_batch_autotuned_launch_shapes():
+    """The autotuner picks one of the rows-per-thread values; every value
+    solves to rtol, and a fixed value gives bitwise-repeatable results."""
+    print("batch autotune", flush=True)
+    level, K = 5, 8
+    pm = P.Mesh.load(oracle_mesh(level)[0])
+    A = P.Matrix.assemble(pm, P.HELMHOLTZ, CM, CL)
+    n = A.n_rows
+    assert P.batch_shape(n, K) == 1          # untuned: 10241 rows, 8 columns -> 321 blocks
+    best, ms = A.tune_batch(K, restart=30)
+    assert best in (1, 2, 4, 8) and P.batch_shape(n, K) == best
+    assert all(t > 0 for t in ms.values()) and ms[best] == min(ms.values())
+    B = _rhs(level, K)
+    dB = _dev(B)
+    try:
+        for rpt in (1, 2, 4, 8):
+            P.set_batch_shape(n, K, P.F64, rpt)
+            X1 = torch.zeros_like(dB)
+            assert all(i["converged"] for i in A.solve_batch(X1, dB, rtol=1e-10))
+            _check_columns(level, B, X1.cpu().numpy(), 1e-10)
+            X2 = torch.zeros_like(dB)
+            A.solve_batch(X2, dB, rtol=1e-10)
+            assert torch.equal(X1, X2), rpt
+    finally:
+        P.set_batch_shape(n, K, P.F64, 0)
+    assert P.batch_shape(n, K) == 1
+    with pytest.raises(P.PhelmError):
+        P.set_batch_shape(n, K, P.F64, 3)

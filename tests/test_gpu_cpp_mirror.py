@@ -1,0 +1,143 @@
+"""GPU test of the C++ host classes (include/phelm_gpu.hpp) through the
+compiled check program tests/cpp/mirror_check: Solver<double> built by
+add() equals the GPU-assembled matrix bit for bit, and Solver::solve,
+SphereLaplace::calc1/calc2/solve, SphereNorm and SphereJacobian agree with
+the oracle's matrices (scipy sparse direct solves) or, for the Jacobian and
+BarVortex, with the Python host classes bit for bit (same library, same
+options)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+sp = pytest.importorskip("scipy.sparse")
+spla = pytest.importorskip("scipy.sparse.linalg")
+
+import mke_amd as P
+from oracle import oracle as O
+from tests._common import oracle_matrix, oracle_mesh
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "cpp", "mirror_check")
+DEV = "cuda:0"
+LEVEL = 4
+
+
+def _csr(t, shape):
+    rp, ci, v = t
+    return sp.csr_matrix((v, ci, rp), shape=shape)
+
+
+@pytest.fixture(scope="module")
+def run(tmp_path_factory):
+    if not os.path.exists(EXE):
+        pytest.fail("tests/cpp/mirror_check is not built (run __graft_entry__.build())")
+    out = str(tmp_path_factory.mktemp("mirror"))
+    print("mirror_check level %d" % LEVEL, flush=True)
+    r = subprocess.run([EXE, str(LEVEL), out], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    summ = {}
+    for line in open(os.path.join(out, "summary.txt")):
+        k, v = line.split()
+        summ[k] = float(v)
+    load = lambda name: np.fromfile(os.path.join(out, name), dtype=np.float64)  # noqa: E731
+    return summ, load
+
+
+def test_solver_add_and_solve(run):
+    summ, load = run
+    assert np.array_equal(load("spmv_add.bin"), load("spmv_asm.bin"))
+    cm, cl = 1.0, -1e-2
+    rp, ci, v = oracle_matrix(LEVEL, O.KIND_HELMHOLTZ, cm, cl)
+    b = O.spmv(rp, ci, v, load("x.bin"))
+    assert np.array_equal(b, load("spmv_asm.bin"))
+    x = load("solve_x.bin")
+    rel = np.linalg.norm(b - O.spmv(rp, ci, v, x)) / np.linalg.norm(b)
+    assert summ["solve_converged"] == 1 and rel <= 1e-10
+    assert np.linalg.norm(x - load("x.bin")) / np.linalg.norm(load("x.bin")) <= 1e-8
+
+
+def test_sphere_laplace(run):
+    summ, load = run
+    text, om = oracle_mesh(LEVEL)
+    n, V = om.n_inner, len(load("u.bin"))
+    inner = np.asarray(om.inner)
+    outer = np.setdiff1d(np.arange(V), inner)
+    Mm = _csr(oracle_matrix(LEVEL, O.KIND_MASS), (n, n))
+    Lm = _csr(oracle_matrix(LEVEL, O.KIND_LAPLACE), (n, n))
+    B3 = _csr(om.assemble_boundary(O.KIND_LAPLACE), (n, len(outer)))
+    u, v = load("u.bin"), load("v.bin")
+    # calc2 = M^-1 (L u_in + B3 u_out)
+    want = spla.spsolve(Mm.tocsc(), Lm @ u[inner] + B3 @ u[outer])
+    got = load("lap_calc2.bin")
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) <= 1e-11
+    c1 = load("lap_calc1.bin")
+    assert np.array_equal(c1[inner], got) and np.all(c1[outer] == 0.0)
+    # solve: L x = M v_in, zero at the pole
+    rhs = Mm @ v[inner]
+    xs = load("lap_solve.bin")
+    assert summ["lap_solve_converged"] == 1
+    assert np.linalg.norm(rhs - Lm @ xs[inner]) / np.linalg.norm(rhs) <= 1e-10
+    xd = spla.spsolve(Lm.tocsc(), rhs)
+    assert np.linalg.norm(xs[inner] - xd) / np.linalg.norm(xd) <= 1e-7
+    assert np.all(xs[outer] == 0.0)
+
+
+def test_sphere_norm(run):
+    summ, load = run
+    _, om = oracle_mesh(LEVEL)
+    u, v = load("u.bin"), load("v.bin")
+    V = len(u)
+    N = _csr(om.assemble_full(O.KIND_MASS), (V, V))
+    assert abs(summ["norm_u"] - np.sqrt(u @ (N @ u))) <= 1e-12 * summ["norm_u"]
+    assert abs(summ["scalar_uv"] - u @ (N @ v)) <= 1e-12 * np.sqrt((u @ (N @ u)) * (v @ (N @ v)))
+    d = u - v
+    assert abs(summ["dist_uv"] - np.sqrt(d @ (N @ d))) <= 1e-12 * summ["dist_uv"]
+
+
+def test_sphere_jacobian_and_barvortex_equal_python(run):
+    summ, load = run
+    m = P.Mesh.sphere(LEVEL)
+    du = torch.as_tensor(load("u.bin"), device=DEV)
+    dv = torch.as_tensor(load("v.bin"), device=DEV)
+    J = P.SphereJacobian(m)
+    ans = torch.zeros(m.inner_size, dtype=torch.float64, device=DEV)
+    J.calc2(ans, du, dv)
+    assert np.array_equal(ans.cpu().numpy(), load("jac_calc2.bin"))
+    assert np.array_equal(-load("jac_calc2.bin"), load("jac_calc2t.bin"))
+    pr = m.arrays()["pr"]
+    lh = torch.as_tensor(2.0 * np.clip(pr[:, 2], -1, 1), device=DEV)
+    bv = P.BarVortex(m, lh, tau=600.0 * 7.292e-5, sigma=1.6e-2, mu=8e-5, k1=1.0, k2=1.0,
+                     theta=0.5, slots=1)
+    u = torch.as_tensor(load("psi.bin"), device=DEV).reshape(1, -1)
+    u1 = torch.zeros_like(u)
+    bv.calc(u1, u)
+    assert summ["bv_converged"] == 1 and summ["bv_bnd_throws"] == 1
+    assert np.array_equal(u1.cpu().numpy()[0], load("bv_u1.bin"))
+
+
+def test_python_sphere_laplace_and_norm_equal_cpp(run):
+    """The Python SphereLaplace / SphereNorm methods make the same library
+    calls with the same options as the C++ classes: same bits."""
+    summ, load = run
+    m = P.Mesh.sphere(LEVEL)
+    du = torch.as_tensor(load("u.bin"), device=DEV)
+    dv = torch.as_tensor(load("v.bin"), device=DEV)
+    L = P.SphereLaplace(m)
+    c2 = torch.zeros(m.inner_size, dtype=torch.float64, device=DEV)
+    L.calc2(c2, du)
+    assert np.array_equal(c2.cpu().numpy(), load("lap_calc2.bin"))
+    c1 = torch.zeros(m.n_points, dtype=torch.float64, device=DEV)
+    L.calc1(c1, du)
+    assert np.array_equal(c1.cpu().numpy(), load("lap_calc1.bin"))
+    ls = torch.zeros(m.n_points, dtype=torch.float64, device=DEV)
+    info = L.solve(ls, dv)
+    assert info["converged"] and np.array_equal(ls.cpu().numpy(), load("lap_solve.bin"))
+    N = P.SphereNorm(m)
+    assert N.norm(du) == summ["norm_u"]
+    assert N.scalar(du, dv) == summ["scalar_uv"]
+    assert N.dist(du, dv) == summ["dist_uv"]

@@ -75,3 +75,12 @@ def test_mesh_load_rejects_truncated_text():
         P.Mesh.load(text[: len(text) // 2])
     with pytest.raises(P.PhelmError):
         P.Mesh.load("")
+
+
+def test_cpp_host_classes_compile():
+    """include/phelm_gpu.hpp (the C++ Solver / SphereLaplace / SphereJacobian /
+    SphereNorm / BarVortex classes over the C ABI) compiles with g++."""
+    import subprocess
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "cpp"), "syntax"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
