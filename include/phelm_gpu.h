@@ -218,15 +218,16 @@ int phg_gmres_batch(phg_matrix_t A, int nrhs, const double *d_B, double *d_X,
                     phg_stream_t s);
 /* Launch shape of the batched solves (no reference counterpart: the CPU
  * solve has none).  The row kernels give each thread rows_per_thread rows
- * (1, 2, 4 or 8) of one column; the value is kept per process for each
+ * (1, 2, 4, 8 or 16) of one column; the value is kept per process for each
  * (n_rows, nrhs rounded up to a power of two, precision) shape and fixes the
  * grouping of every dot product's block partials, so results are bitwise
  * reproducible for a fixed value.  Untuned shapes take the largest value
- * that still gives >= 1024 workgroups.  phg_gmres_batch_tune times one
+ * (at most 8) that still gives >= 1024 workgroups.  phg_gmres_batch_tune times one
  * restart cycle (opts->restart steps, opts->ortho and opts->precision; NULL:
  * defaults with CGS2) of nrhs hash-valued right-hand sides on A for each value,
  * keeps the fastest for the shape and returns it in *rows_per_thread (may be
- * NULL); ms (4 entries, may be NULL) receives the cycle times for 1, 2, 4, 8.
+ * NULL); ms (5 entries, may be NULL) receives the cycle times for 1, 2, 4, 8
+ * and 16.
  * It allocates scratch vectors; do not call it while solves of the same
  * shape run on other streams. */
 int phg_gmres_batch_tune(phg_matrix_t A, int nrhs, const phg_gmres_opts *opts,

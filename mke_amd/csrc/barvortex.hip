@@ -35,6 +35,7 @@ int jacobian_calc2_strided(phg_jacobian_s* J, int np, const double* d_u, const d
                            size_t su, size_t sk, double* d_ans, size_t ai, size_t ak,
                            double sign, const phg_gmres_opts* opts, phg_gmres_info* infos,
                            hipStream_t s);
+int jacobian_clear_pairs(phg_jacobian_s* J, int np, unsigned bits, hipStream_t s);
 }  // namespace phg
 
 using namespace phg;
@@ -347,6 +348,11 @@ int init_slots(phg_barvortex_s* B, const std::vector<int>& mask, const phg_bv_op
     const phg_bv_conf& c = B->conf;
     int rc = upload_mask(B, mask, s);
     if (rc) return rc;
+    unsigned fresh = 0;
+    for (int k = 0; k < S; ++k)
+        if (mask[k]) fresh |= 1u << k;
+    rc = jacobian_clear_pairs(B->J, S, fresh, s);               // Jacobian solves from zero
+    if (rc) return rc;
     rc = calc1(B, B->U.p, B->W.p, o, B->it_mass, s);            // w = L(u)
     if (rc) return rc;
     rc = calc1(B, B->W.p, B->T1.p, o, B->it_mass, s);           // dw = L(w)
@@ -395,9 +401,17 @@ int iterate_slots(phg_barvortex_s* B, const std::vector<int>& mask, const phg_bv
     hipLaunchKernelGGL(bv_axpby, g256(V * S), dim3(BV_BS), 0, s, V, S, c.theta, B->Un.p,
                        1.0 - c.theta, B->U.p, B->T2.p, B->dmask.p, 1);
     PHG_HIP(hipGetLastError());
-    // jac = J(tmp2, tmp1): 8 mass solves per slot, one batched solve
+    // jac = J(tmp2, tmp1): 8 mass solves per slot, one batched solve.  Each
+    // slot's solves start from its solutions of the previous pass (cleared
+    // when the slot starts a step); idle slots are cleared so that their
+    // zero right-hand sides need no step.
     {
-        phg_gmres_opts jo = solve_opts(o.rtol_mass, o.precision, PHG_CGS1, 0, o.max_solver_iters);
+        unsigned idle = 0;
+        for (int k = 0; k < S; ++k)
+            if (!mask[k]) idle |= 1u << k;
+        rc = jacobian_clear_pairs(B->J, S, idle, s);
+        if (rc) return rc;
+        phg_gmres_opts jo = solve_opts(o.rtol_mass, o.precision, PHG_CGS1, 1, o.max_solver_iters);
         jo.restart = 32;
         std::vector<phg_gmres_info> inf((size_t)8 * S);
         rc = jacobian_calc2_strided(B->J, S, B->T2.p, B->T1.p, S, 1, B->JAC.p, S, 1, 1.0, &jo,

@@ -27,6 +27,7 @@
 // the consumer adds the partials in block order.  Results depend only on the
 // inputs and the launch shape, never on timing.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "ell.h"
@@ -558,7 +559,12 @@ int prepare_work(phg_matrix_s* A, int prec, int m, GmresWork** out)
         w->n = n;
         w->m = m;
         w->prec = prec;
-        int G = std::max(1, std::min((n + GM_BS - 1) / GM_BS, 512));
+        // target block count: 256 (one per CU: the shortest reduction chain
+        // of the MGS passes at levels 6-8), or PHG_GMRES_BLOCKS (launch-shape
+        // experiments; it changes the grouping of the dot partials)
+        int target = 256;
+        if (const char* e = getenv("PHG_GMRES_BLOCKS")) target = std::max(1, std::min(atoi(e), GM_MAXG));
+        int G = std::max(1, std::min((n + GM_BS - 1) / GM_BS, target));
         int rpt = (n + G * GM_BS - 1) / (G * GM_BS);
         if (rpt > GM_RPT) {
             rpt = GM_RPT;

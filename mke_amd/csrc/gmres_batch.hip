@@ -26,6 +26,7 @@
 // is in the batch, provided KP is the same.  A restart cycle is captured in
 // one hipGraph and replayed; the host reads a K-entry summary per cycle.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -38,7 +39,7 @@ namespace phg {
 constexpr int BB_BS = 256;
 constexpr int BB_MAXM = 32;
 constexpr int BB_MAXK = 64;
-constexpr int BB_RPT = 8;  // largest rows per thread of a launch shape
+constexpr int BB_RPT = 16;  // largest rows per thread of a launch shape
 constexpr int BB_LB = 4;   // basis vectors per load batch
 
 struct BCol {
@@ -383,6 +384,69 @@ bk_update(int n, int KP, int K, VT* __restrict__ W, const VT* __restrict__ V, si
     else bdots<VT, RPT>(j + 1, -1, nullptr, V, bs, n, KP, q, base, RB, act, w, part, G, 0);
 }
 
+// First CGS2 pass fused with the dots of the second: w -= sum_{i<=j} hc_i v_i
+// row by row, with the row's v_0..v_j kept in registers for the partial
+// dots of the updated w with v_0..v_j.  The same operations in the same
+// order as bk_update (non-final pass) followed by its bdots, so the same
+// bits, with one read of the basis instead of two.
+template <typename VT, int RPT>
+__global__ void __launch_bounds__(BB_BS)
+bk_update_dots(int n, int KP, int K, VT* __restrict__ W, const VT* __restrict__ V, size_t bs,
+               int j, double* __restrict__ part, int G, const BCol* __restrict__ cols)
+{
+    __shared__ VT shv[BB_MAXM][BB_MAXK];
+    __shared__ double sh[BB_BS / 64][BB_LB][BB_MAXK];
+    const int q = threadIdx.x & (KP - 1);
+    const int RB = BB_BS / KP;
+    const int base = lblock() * RB * RPT + (int)threadIdx.x / KP;
+    const bool act = q < K && cols[q].active;
+    for (int e = threadIdx.x; e < BB_MAXM * KP; e += BB_BS) {
+        const int i = e / KP, qq = e % KP;
+        shv[i][qq] = (qq < K && i <= j && cols[qq].active) ? (VT)cols[qq].hc[i] : VT(0);
+    }
+    __syncthreads();
+    double acc[BB_MAXM];
+#pragma unroll
+    for (int i = 0; i < BB_MAXM; ++i) acc[i] = 0.0;
+    for (int t = 0; t < RPT; ++t) {
+        const int row = base + t * RB;
+        if (!act || row >= n) continue;
+        const size_t o = (size_t)row * KP + q;
+        VT vv[BB_MAXM];
+#pragma unroll
+        for (int i = 0; i < BB_MAXM; ++i) vv[i] = i <= j ? V[(size_t)i * bs + o] : VT(0);
+        VT w = W[o];
+#pragma unroll
+        for (int i = 0; i < BB_MAXM; ++i)
+            if (i <= j) w = w - shv[i][q] * vv[i];
+        W[o] = w;
+#pragma unroll
+        for (int i = 0; i < BB_MAXM; ++i)
+            if (i <= j) acc[i] += dd(w) * dd(vv[i]);
+    }
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i0 = 0; i0 < BB_MAXM; i0 += BB_LB) {
+        if (i0 > j) break;
+#pragma unroll
+        for (int u = 0; u < BB_LB; ++u) {
+            const double s = col_wave_sum(acc[i0 + u], KP);
+            if (lane < KP) sh[wid][u][lane] = s;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < BB_LB * KP; e += BB_BS) {
+            const int u = e / KP, qq = e % KP;
+            if (i0 + u <= j) {
+                double s = 0.0;
+#pragma unroll
+                for (int ww = 0; ww < BB_BS / 64; ++ww) s += sh[ww][u][qq];
+                part[((size_t)(i0 + u) * KP + qq) * G + lblock()] = s;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Givens rotation of step j for each column (one wavefront per column) from
 // H(0..j, j) and the norm partials; decides whether the column goes on.
 __global__ void __launch_bounds__(BB_BS)
@@ -615,6 +679,17 @@ void precond_apply<float>(phg_mg_s*, int, const float*, float*, hipStream_t)
 {
 }
 
+// The fused first-pass kernel (default on; PHG_BATCH_FUSED=0 selects the
+// two-read form, which gives the same bits).
+bool fused_update_dots()
+{
+    static const bool on = [] {
+        const char* e = getenv("PHG_BATCH_FUSED");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 int next_pow2(int k)
 {
     int p = 1;
@@ -699,8 +774,12 @@ void enqueue_batch_cycle(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K,
             const int nd = j + 1;
             hipLaunchKernelGGL(bk_reduce, dim3((nd * KP + 3) / 4), block, 0, s, KP, K, nd, j, p,
                                part, G, cols);
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_update<VT, RPT>), grid, block, 0, s, n, KP, K, dst,
-                               V, bs, j, p == passes - 1 ? 1 : 0, part, G, cols);
+            if (p < passes - 1 && fused_update_dots())
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_update_dots<VT, RPT>), grid, block, 0, s, n,
+                                   KP, K, dst, V, bs, j, part, G, cols);
+            else
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_update<VT, RPT>), grid, block, 0, s, n, KP,
+                                   K, dst, V, bs, j, p == passes - 1 ? 1 : 0, part, G, cols);
         }
         hipLaunchKernelGGL(bk_givens, dim3(cb), block, 0, s, KP, K, j, m, part, G, cols, rtol,
                            inner_rtol, max_iters);
@@ -808,6 +887,7 @@ int batch_run(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const doub
     case 1: return batch_run_t<VT, MT, 1>(A, E, w, K, B, X, o, infos, pc, s);
     case 2: return batch_run_t<VT, MT, 2>(A, E, w, K, B, X, o, infos, pc, s);
     case 4: return batch_run_t<VT, MT, 4>(A, E, w, K, B, X, o, infos, pc, s);
+    case 8: return batch_run_t<VT, MT, 8>(A, E, w, K, B, X, o, infos, pc, s);
     default: return batch_run_t<VT, MT, BB_RPT>(A, E, w, K, B, X, o, infos, pc, s);
     }
 }
@@ -830,7 +910,7 @@ std::map<ShapeKey, int> g_shape_rpt;
 
 int default_rpt(int n, int kp)
 {
-    for (int r = BB_RPT; r > 1; r >>= 1)
+    for (int r = 8; r > 1; r >>= 1)
         if (blocks_for(n, kp, r) >= 1024) return r;
     return 1;
 }
@@ -932,8 +1012,8 @@ int batch_shape_get(int n, int K, int prec)
 int batch_shape_set(int n, int K, int prec, int rpt)
 {
     if (n < 1 || K < 1 || K > BB_MAXK || prec < PHG_F64 || prec > PHG_BF16 ||
-        (rpt != 0 && rpt != 1 && rpt != 2 && rpt != 4 && rpt != 8)) {
-        set_error("gmres_batch_set_shape: bad argument (rows per thread 0, 1, 2, 4 or 8)");
+        (rpt != 0 && rpt != 1 && rpt != 2 && rpt != 4 && rpt != 8 && rpt != 16)) {
+        set_error("gmres_batch_set_shape: bad argument (rows per thread 0, 1, 2, 4, 8 or 16)");
         return PHG_ERR_ARG;
     }
     if (rpt == 0) {
@@ -950,7 +1030,7 @@ int batch_shape_set(int n, int K, int prec, int rpt)
 int gmres_batch_tune(phg_matrix_s* A, int K, const phg_gmres_opts* opts, hipStream_t s,
                      int* best_out, double* ms_out)
 {
-    static const int cand[4] = {1, 2, 4, 8};
+    static const int cand[5] = {1, 2, 4, 8, 16};
     const int n = A->n_rows;
     if (K < 1 || K > BB_MAXK || n < 1) {
         set_error("gmres_batch_tune: nrhs must be in 1..64 and the matrix non-empty");
@@ -979,7 +1059,7 @@ int gmres_batch_tune(phg_matrix_s* A, int K, const phg_gmres_opts* opts, hipStre
     const int kp = next_pow2(K);
     int best = BB_RPT, rc = PHG_OK;
     float best_ms = 0.0f;
-    for (int c = 0; c < 4 && rc == PHG_OK; ++c) {
+    for (int c = 0; c < 5 && rc == PHG_OK; ++c) {
         set_shape_rpt(n, kp, o.precision, cand[c]);
         float t = 0.0f;
         for (int rep = 0; rep < 4 && rc == PHG_OK; ++rep) {  // rep 0 captures the graph

@@ -143,6 +143,16 @@ __global__ void jk_combine(int n, int np, const double* __restrict__ X, double s
     ans[(size_t)i * ai + (size_t)k * ak] = a;
 }
 
+// Zeroes the mass-solve solutions of the pairs k with bit k of `bits` set
+// (X is n x 8 np row-major, pair k in columns 8k..8k+7).
+__global__ void jk_clear(int n, int np, unsigned bits, double* __restrict__ X)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * 8 * np) return;
+    const int k = (int)(e % (8 * np)) >> 3;
+    if (bits >> k & 1u) X[e] = 0.0;
+}
+
 bool same_structure(const phg_matrix_s* a, const phg_matrix_s* b)
 {
     return a->n_rows == b->n_rows && a->n_cols == b->n_cols && a->h_rp == b->h_rp &&
@@ -153,9 +163,27 @@ bool same_structure(const phg_matrix_s* a, const phg_matrix_s* b)
 
 namespace phg {
 
+// Zeroes the stored mass-solve solutions of the pairs in `bits` (bit k =
+// pair k of an np-pair call), so that a later call with opts->use_x0 starts
+// those pairs from zero and the others from their previous solutions.
+int jacobian_clear_pairs(phg_jacobian_s* J, int np, unsigned bits, hipStream_t s)
+{
+    if (!J || np < 1 || np > JK_MAXPAIRS) {
+        set_error("jacobian_clear_pairs: bad argument");
+        return PHG_ERR_ARG;
+    }
+    const size_t t = (size_t)J->n * 8 * np;
+    if (t && bits)
+        hipLaunchKernelGGL(jk_clear, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, s, J->n, np,
+                           bits, J->X.p);
+    PHG_HIP(hipGetLastError());
+    return PHG_OK;
+}
+
 // calc2 (sign 1) or calc2t (sign -1) of np pairs; point p of pair k is read
 // at p * su + k * sk of u and v, J(u_k, v_k) at inner point i is stored at
-// i * ai + k * ak of ans.
+// i * ai + k * ak of ans.  With opts->use_x0 the 8 np mass solves start
+// from the solutions of the previous call (same np), which the object keeps.
 int jacobian_calc2_strided(phg_jacobian_s* J, int np, const double* d_u, const double* d_v,
                            size_t su, size_t sk, double* d_ans, size_t ai, size_t ak,
                            double sign, const phg_gmres_opts* opts, phg_gmres_info* infos,

@@ -142,15 +142,26 @@ __global__ void mg_prolong_add(int nf, int KP, const int* __restrict__ p0,
     xf[e] += v;
 }
 
-__global__ void mg_dense(int nc, int KP, const double* __restrict__ inv,
-                         const double* __restrict__ b, double* __restrict__ x)
+// x = inv b on the coarsest level: one workgroup per row i; thread t takes
+// column q = t % KP and the j = t / KP, t / KP + 256 / KP, ... terms, and the
+// 256 / KP partial sums of a column are added in a fixed order (a column's
+// sum does not involve the other columns).
+__global__ void __launch_bounds__(256)
+mg_dense(int nc, int KP, const double* __restrict__ inv, const double* __restrict__ b,
+         double* __restrict__ x)
 {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)nc * KP) return;
-    const int i = (int)(e / KP), q = (int)(e % KP);
+    __shared__ double part[256];
+    const int i = blockIdx.x, t = threadIdx.x;
+    const int q = t % KP, g = t / KP, ng = 256 / KP;
     double s = 0.0;
-    for (int j = 0; j < nc; ++j) s += inv[(size_t)i * nc + j] * b[(size_t)j * KP + q];
-    x[e] = s;
+    for (int j = g; j < nc; j += ng) s += inv[(size_t)i * nc + j] * b[(size_t)j * KP + q];
+    part[t] = s;
+    __syncthreads();
+    if (t < KP) {
+        double a = 0.0;
+        for (int k = 0; k < ng; ++k) a += part[k * KP + t];
+        x[(size_t)i * KP + t] = a;
+    }
 }
 
 __global__ void mg_diag_inv(int n, const int* __restrict__ rp, const int* __restrict__ ci,
@@ -322,8 +333,8 @@ void mg_apply(phg_mg_s* M, int KP, const double* v, double* z, hipStream_t s)
     }
     MgLevel& cl = *M->lv[nlev - 1];
     double* xcoarse = (nlev == 1) ? z : cl.x0.p;
-    hipLaunchKernelGGL(mg_dense, grid_for((size_t)M->nc * KP), dim3(256), 0, s, M->nc, KP,
-                       M->cinv.p, bk, xcoarse);
+    hipLaunchKernelGGL(mg_dense, dim3((unsigned)M->nc), dim3(256), 0, s, M->nc, KP, M->cinv.p, bk,
+                       xcoarse);
     // ascend
     for (int k = nlev - 2; k >= 0; --k) {
         MgLevel& l = *M->lv[k];

@@ -169,7 +169,7 @@ def batch_shape(n_rows, nrhs, precision=F64):
 
 
 def set_batch_shape(n_rows, nrhs, precision, rows_per_thread):
-    """Pins the rows per thread of a batched-solve shape (1, 2, 4 or 8; 0
+    """Pins the rows per thread of a batched-solve shape (1, 2, 4, 8 or 16; 0
     returns it to the untuned default)."""
     check(lib().phg_gmres_batch_set_shape(int(n_rows), int(nrhs), int(precision),
                                           int(rows_per_thread)), "phg_gmres_batch_set_shape")
@@ -369,16 +369,16 @@ class Matrix:
     def tune_batch(self, nrhs, restart=30, ortho=CGS2, precision=F64, stream=None):
         """Autotunes the rows per thread of the batched solves for this
         matrix's (n, nrhs, precision) shape: times one restart cycle per
-        candidate (1, 2, 4, 8), keeps the fastest for the process and returns
+        candidate (1, 2, 4, 8, 16), keeps the fastest for the process and returns
         (rows_per_thread, {candidate: ms})."""
         o = GmresOpts()
         lib().phg_gmres_default_opts(C.byref(o))
         o.restart, o.ortho, o.precision = restart, ortho, precision
         best = C.c_int(0)
-        ms = (C.c_double * 4)()
+        ms = (C.c_double * 5)()
         check(lib().phg_gmres_batch_tune(self.h, int(nrhs), C.byref(o), _stream(stream),
                                          C.byref(best), ms), "phg_gmres_batch_tune")
-        return best.value, {r: ms[i] for i, r in enumerate((1, 2, 4, 8))}
+        return best.value, {r: ms[i] for i, r in enumerate((1, 2, 4, 8, 16))}
 
     def close(self):
         if self.h:

@@ -155,12 +155,12 @@ def test_batch_autotuned_launch_shapes():
     n = A.n_rows
     assert P.batch_shape(n, K) == 1          # untuned: 10241 rows, 8 columns -> 321 blocks
     best, ms = A.tune_batch(K, restart=30)
-    assert best in (1, 2, 4, 8) and P.batch_shape(n, K) == best
+    assert best in (1, 2, 4, 8, 16) and P.batch_shape(n, K) == best
     assert all(t > 0 for t in ms.values()) and ms[best] == min(ms.values())
     B = _rhs(level, K)
     dB = _dev(B)
     try:
-        for rpt in (1, 2, 4, 8):
+        for rpt in (1, 2, 4, 8, 16):
             P.set_batch_shape(n, K, P.F64, rpt)
             X1 = torch.zeros_like(dB)
             assert all(i["converged"] for i in A.solve_batch(X1, dB, rtol=1e-10))
@@ -173,3 +173,44 @@ def test_batch_autotuned_launch_shapes():
     assert P.batch_shape(n, K) == 1
     with pytest.raises(P.PhelmError):
         P.set_batch_shape(n, K, P.F64, 3)
+
+
+def test_batch_fused_first_pass_gives_the_same_bits(tmp_path):
+    """The fused CGS2 first pass (update + next dots in one read of the basis)
+    makes the same operations in the same order as the two-kernel form
+    (PHG_BATCH_FUSED=0, run in a child process): identical solutions."""
+    import os
+    import subprocess
+    import sys
+    print("batch fused vs unfused", flush=True)
+    level, K = 4, 8
+    B = _rhs(level, K)
+    np.save(tmp_path / "B.npy", B)
+    pm = P.Mesh.load(oracle_mesh(level)[0])
+    A = P.Matrix.assemble(pm, P.HELMHOLTZ, CM, CL)
+    for prec in (P.F64, P.F32):
+        X = torch.zeros(B.shape, dtype=torch.float64, device=DEV)
+        A.solve_batch(X, _dev(B), rtol=1e-10, ortho=P.CGS2, precision=prec)
+        np.save(tmp_path / ("X%d.npy" % prec), X.cpu().numpy())
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    script = (
+        "import sys, numpy as np, torch\n"
+        "sys.path.insert(0, %r)\n"
+        "import mke_amd as P\n"
+        "from tests._common import oracle_mesh\n"
+        "from tests.test_gpu_batch import CM, CL\n"
+        "B = np.load(%r)\n"
+        "A = P.Matrix.assemble(P.Mesh.load(oracle_mesh(%d)[0]), P.HELMHOLTZ, CM, CL)\n"
+        "for prec in (P.F64, P.F32):\n"
+        "    X = torch.zeros(B.shape, dtype=torch.float64, device='cuda:0')\n"
+        "    A.solve_batch(X, torch.as_tensor(B, device='cuda:0'), rtol=1e-10, ortho=P.CGS2,"
+        " precision=prec)\n"
+        "    np.save(%r %% prec, X.cpu().numpy())\n"
+    ) % (root, str(tmp_path / "B.npy"), level, str(tmp_path / "U%d.npy"))
+    env = dict(os.environ, PHG_BATCH_FUSED="0")
+    r = subprocess.run([sys.executable, "-c", script], env=env, capture_output=True, text=True,
+                       timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    for prec in (P.F64, P.F32):
+        assert np.array_equal(np.load(tmp_path / ("X%d.npy" % prec)),
+                              np.load(tmp_path / ("U%d.npy" % prec))), prec
