@@ -284,6 +284,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--no-l8", action="store_true",
                     help="skip the level-8 mixed-precision leg")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the ensemble serving leg and print its results (profiling)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 256 MiB device copy (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -299,6 +301,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a GPU"
     torch.cuda.set_device(0)
+    if args.serve_only:
+        print(json.dumps({"serve_only": serve_leg(args.serve_requests, args.serve_rate)}))
+        return
     stream = torch.cuda.current_stream()
     dev = "cuda:0"
 

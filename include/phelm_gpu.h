@@ -358,6 +358,58 @@ int phg_barvortex_iterate(phg_barvortex_t B, const phg_bv_opts *opts, int *state
                           phg_bv_info *infos, phg_stream_t s);
 int phg_barvortex_store(phg_barvortex_t B, int slot, double *d_u1, phg_stream_t s);
 
+/* ---- Baroclin implicit step (examples/baroclin.h:60-292,
+ *      examples/baroclin.cpp:69-181, 445-606) ------------------------------ */
+/* Baroclin's parameters without the callbacks: the Coriolis/orography term
+ * is a mesh function (proj(lh, m, coriolis), baroclin.cpp:81-82) and the
+ * forcing callbacks f, g are passed per call as mesh functions. */
+typedef struct {
+    double tau, sigma, mu, sigma1, mu1, alpha;
+} phg_bc_conf;
+typedef struct {
+    int max_iters;         /* fixed-point passes, reference: 20                */
+    double tol;            /* stop at max(dist(u1_n1, u1_n), dist(u2_n1, u2_n))
+                              < tol (absolute, SphereNorm), reference: 1e-8   */
+    double rtol_mass;      /* mass solves of L() and of the Jacobian           */
+    double rtol_system;    /* the 4n x 4n solve of A_                          */
+    int max_solver_iters;  /* cap of every inner solve                        */
+    int precondition;      /* 1: block lower-triangular preconditioner of the
+                              4n system (diagonal A11, A22; one Poisson V-cycle
+                              per Laplace block); 0: plain GMRES, as the
+                              reference's solve                              */
+} phg_bc_opts;
+typedef struct {
+    int iterations;        /* fixed-point passes                               */
+    double distance;       /* the last pass's max distance                     */
+    int converged;         /* distance < tol                                   */
+    int it_mass, it_jacobian, it_system; /* GMRES steps by solve              */
+} phg_bc_info;
+typedef struct phg_baroclin_s *phg_baroclin_t;
+
+void phg_bc_default_opts(phg_bc_opts *o);
+/* Baroclin::Baroclin (baroclin.cpp:69-103) for up to `slots` (1, 2, 4, 8)
+ * members at once: A_ as nine GPU-assembled blocks (bit-identical to the
+ * reference's integrate_cb values) stacked into the 4n x 4n CSR,
+ * SphereLaplace's, SphereJacobian's and SphereNorm's matrices and the Poisson
+ * V-cycle hierarchy.  d_lh: n_points values (device).  alpha != 0. */
+int phg_baroclin_create(phg_mesh_t m, const phg_bc_conf *conf, const double *d_lh,
+                        int slots, phg_stream_t s, phg_baroclin_t *out);
+void phg_baroclin_destroy(phg_baroclin_t B);
+int phg_baroclin_slots(phg_baroclin_t B);
+/* A_ (owned by B). */
+int phg_baroclin_matrix(phg_baroclin_t B, phg_matrix_t *out);
+/* Baroclin::calc(u11, u21, u1, u2, bnd = 0, t) (baroclin.cpp:445-606) for
+ * nmembers (<= slots) members: d_u1, d_u2, d_u11, d_u21 hold nmembers mesh
+ * functions of n_points values each, one after the other; d_f, d_g (n_points
+ * values each, may be NULL) are the forcing callbacks f_ and g_ evaluated at
+ * the mesh points for this step's t, shared by the members.  A member's
+ * result does not depend on the other members.  infos: nmembers host entries
+ * (may be NULL). */
+int phg_baroclin_calc(phg_baroclin_t B, int nmembers, const double *d_u1,
+                      const double *d_u2, const double *d_f, const double *d_g,
+                      double *d_u11, double *d_u21, const phg_bc_opts *opts,
+                      phg_bc_info *infos, phg_stream_t s);
+
 /* ---- serving: implicit-step requests of ensemble members on several mesh
  *      levels, continuously batched (no counterpart in the reference, which
  *      steps one BarVortex at a time, examples/test_barvortex.cpp:441-470;

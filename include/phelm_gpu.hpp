@@ -11,6 +11,7 @@
 //   SphereNorm       src/norm.h:58-120 with fast_scalar/norm/dist
 //                    (src/mesh.h:550-591)
 //   BarVortex        examples/barvortex.h:62-155 (calc only)
+//   Baroclin         examples/baroclin.h:60-292 (calc only)
 //
 // Vector arguments are device pointers (the reference's GPU build passes
 // device arrays, examples/schafe_impl.h:134-149); ArrayDevice<T> owns one.
@@ -481,6 +482,85 @@ private:
     phg_barvortex_t h_ = nullptr;
     phg_bv_opts opts_{};
     phg_bv_info info_{};
+};
+
+// Baroclin (examples/baroclin.h:60-292): the constructor takes the
+// reference's callbacks; calc() evaluates f and g at the mesh points' zone-0
+// chart coordinates for its t (as baroclin.cpp:520-535 does) and runs the
+// step on the device.  Boundary (pole) values are zero, as the reference
+// driver passes them; bnd must be null or all zero (it is not read).
+class Baroclin {
+public:
+    typedef double (*rp_t)(double x, double y, double t, double sigma, double mu, double sigma1,
+                           double mu1, double alpha);
+    typedef double (*coriolis_t)(double phi, double lambda);
+
+    Baroclin(const Mesh& m, rp_t f, rp_t g, coriolis_t coriolis, double tau, double sigma,
+             double mu, double sigma1, double mu1, double alpha)
+        : m_(m), f_(f), g_(g), conf_{tau, sigma, mu, sigma1, mu1, alpha}
+    {
+        int V = 0, T = 0, ni = 0, no = 0, nz = 0;
+        check(phg_mesh_info(m.handle(), &V, &T, &ni, &no, &nz), "phg_mesh_info");
+        std::vector<double> coords((size_t)V * nz * 2);
+        check(phg_mesh_arrays(m.handle(), nullptr, coords.data(), nullptr, nullptr, nullptr,
+                              nullptr, nullptr, nullptr, nullptr, nullptr),
+              "phg_mesh_arrays");
+        x_.resize((size_t)V);
+        y_.resize((size_t)V);
+        std::vector<double> lh((size_t)V);
+        for (int i = 0; i < V; ++i) {
+            x_[i] = coords[(size_t)i * nz * 2];
+            y_[i] = coords[(size_t)i * nz * 2 + 1];
+            lh[i] = coriolis ? coriolis(x_[i], y_[i]) : 0.0;
+        }
+        ArrayDevice<double> dlh(lh.size());
+        dlh.upload(lh.data());
+        fv_.resize((size_t)V);
+        gv_.resize((size_t)V);
+        check(phg_baroclin_create(m.handle(), &conf_, dlh.data(), 1, nullptr, &h_),
+              "Baroclin");
+        phg_bc_default_opts(&opts_);
+    }
+    Baroclin(const Baroclin&) = delete;
+    Baroclin& operator=(const Baroclin&) = delete;
+    ~Baroclin()
+    {
+        if (h_) phg_baroclin_destroy(h_);
+    }
+    // (u1, u2) -> (u11, u21), device pointers of m.ps.size() values
+    void calc(double* u11, double* u21, const double* u1, const double* u2, const double* bnd,
+              double t)
+    {
+        (void)bnd;
+        const size_t V = x_.size();
+        std::vector<double> hf(V), hg(V);
+        for (size_t i = 0; i < V; ++i) {
+            const Config c = conf_c();
+            hf[i] = f_ ? f_(x_[i], y_[i], t, c.sigma, c.mu, c.sigma1, c.mu1, c.alpha) : 0.0;
+            hg[i] = g_ ? g_(x_[i], y_[i], t, c.sigma, c.mu, c.sigma1, c.mu1, c.alpha) : 0.0;
+        }
+        fv_.upload(hf.data());
+        gv_.upload(hg.data());
+        check(phg_baroclin_calc(h_, 1, u1, u2, f_ ? fv_.data() : nullptr,
+                                g_ ? gv_.data() : nullptr, u11, u21, &opts_, &info_, nullptr),
+              "Baroclin::calc");
+    }
+    phg_bc_opts& opts() { return opts_; }
+    const phg_bc_info& last_info() const { return info_; }
+
+private:
+    struct Config {
+        double sigma, mu, sigma1, mu1, alpha;
+    };
+    Config conf_c() const { return {conf_.sigma, conf_.mu, conf_.sigma1, conf_.mu1, conf_.alpha}; }
+    const Mesh& m_;
+    rp_t f_, g_;
+    phg_bc_conf conf_;
+    std::vector<double> x_, y_;
+    ArrayDevice<double> fv_, gv_;
+    phg_baroclin_t h_ = nullptr;
+    phg_bc_opts opts_{};
+    phg_bc_info info_{};
 };
 
 }  // namespace phelm_gpu

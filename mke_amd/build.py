@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libphelm_hip.so")
 
-HIP_SOURCES = ["assemble.hip", "spmv.hip", "gmres.hip", "gmres_batch.hip", "mg.hip", "jacobian.hip", "barvortex.hip", "block.hip"]
+HIP_SOURCES = ["assemble.hip", "spmv.hip", "gmres.hip", "gmres_batch.hip", "mg.hip", "jacobian.hip", "barvortex.hip", "baroclin.hip", "block.hip"]
 CPP_SOURCES = ["mesh.cpp", "capi.cpp", "serve.cpp"]
 
 # -ffp-contract=off everywhere: the assembly must match the CPU restatement

@@ -1,0 +1,627 @@
+// Baroclin::calc (examples/baroclin.cpp:445-606) device resident, for S
+// members (slots) on one mesh at once.
+//
+// The two-layer baroclinic implicit step.  With w_k = L(u_k)
+// (SphereLaplace::calc1), the constant parts FC, GC of the right-hand sides
+// and the iterate (u1_n, u2_n, w1_n, w2_n) starting at (u1, u2, w1, w2),
+// each pass (at most 20) evaluates, with avg(a, a_n) = (1-theta) a + theta a_n,
+//     F = -J(avg u1, avg w1 + lh) - J(avg u2, avg w2) + FC
+//     G = -J(avg u1, avg w2) - J(avg u2, avg w1 + lh) + alpha^2 J(avg u1, avg u2) + GC
+// (five SphereJacobian::calc2, one batched call per member), the right part
+// [M F; M G; 0; 0] and the 4n x 4n solve of A_ for (w1_n, w2_n, u1_n1,
+// u2_n1), and stops when max(dist(u1_n1, u1_n), dist(u2_n1, u2_n)) < 1e-8
+// (SphereNorm).  Boundary (pole) values are zero, as the reference driver
+// passes them (examples/test_baroclin.cpp:215).
+//
+// A_ (examples/baroclin.cpp:69-181) is assembled as its nine n x n blocks:
+// every block formula of integrate_cb is c_mass * (mass element) + c_lap *
+// (stiffness element) with the coefficients evaluated as the reference
+// writes them, which (theta = 1/2 being a power of two) gives the reference's
+// values bit for bit; the 4n x 4n CSR stacks them.  The right part uses the
+// assembled mass matrix (generate_right_part's element-order sums of
+// F_j * mass, within rounding).
+//
+// The 4n system is solved by the batched GMRES over the slots, right-
+// preconditioned (flexible form) by the block lower-triangular
+//     P = [A11 0 0 0; A21 A22 0 0; M 0 -L 0; 0 M 0 -L]
+// with A11, A22 replaced by their diagonals (mass-dominated: 1/tau M +
+// O(mu) L) and -L^-1 by one V-cycle of the Poisson multigrid.  A24 (the
+// alpha^2 u2 coupling) and A12 are left to GMRES.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "ell.h"
+#include "phg_internal.h"
+#include "slot_kernels.h"
+
+namespace phg {
+int jacobian_calc2_strided(phg_jacobian_s* J, int np, const double* d_u, const double* d_v,
+                           size_t su, size_t sk, double* d_ans, size_t ai, size_t ak,
+                           double sign, const phg_gmres_opts* opts, phg_gmres_info* infos,
+                           hipStream_t s);
+}  // namespace phg
+
+using namespace phg;
+
+namespace {
+
+constexpr int BC_PAIRS = 5;       // Jacobians per member and pass
+// block order of integrate_cb's push_back calls and their (row, column) block
+constexpr int BC_BR[9] = {0, 0, 1, 1, 1, 2, 2, 3, 3};
+constexpr int BC_BC[9] = {0, 1, 0, 1, 3, 0, 2, 1, 3};
+
+// d[i] = 1 / A[i][i]
+__global__ void bc_diag_inv(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                            const double* __restrict__ val, double* __restrict__ d)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k)
+        if (ci[k] == i) a = val[k];
+    d[i] = a != 0.0 ? 1.0 / a : 0.0;
+}
+
+// y[i][q] = kb * (B x)[i][q] + kc * c[i][q]  (ELL of B, n x KP vectors)
+__global__ void bc_spmm_lin(int n, int KP, int width, const int* __restrict__ col,
+                            const double* __restrict__ val, const double* __restrict__ x,
+                            double kb, double kc, const double* __restrict__ c,
+                            double* __restrict__ y)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * KP) return;
+    const int i = (int)(e / KP), q = (int)(e % KP);
+    double t = 0.0;
+    for (int k = 0; k < width; ++k) {
+        const size_t o = (size_t)k * n + i;
+        t += val[o] * x[(size_t)col[o] * KP + q];
+    }
+    y[e] = kb * t + kc * c[e];
+}
+
+// z[i][q] = d[i] * v[i][q]  (row scaling)
+__global__ void bc_scale_rows(int n, int KP, const double* __restrict__ d,
+                              const double* __restrict__ v, double* __restrict__ z)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * KP) return;
+    z[e] = d[e / KP] * v[e];
+}
+
+// Jacobian pair inputs of member s: pairs (A1,B1) (A2,B2) (A1,B2) (A2,B1)
+// (A1,A2) with A_k = avg(u_k), B1 = avg(w1) + lh, B2 = avg(w2); PU/PV are
+// V x P row-major, pair s * 5 + t.  Masked-out members get zeros.
+__global__ void bc_pairs(int V, int S, int P0, int P, double th, const double* __restrict__ u1,
+                         const double* __restrict__ u1n, const double* __restrict__ u2,
+                         const double* __restrict__ u2n, const double* __restrict__ w1,
+                         const double* __restrict__ w1n, const double* __restrict__ w2,
+                         const double* __restrict__ w2n, const double* __restrict__ lh,
+                         double* __restrict__ PU, double* __restrict__ PV,
+                         const int* __restrict__ mask)
+{
+    // pairs P0 .. P0 + P - 1 of the S * 5, written at column pair - P0
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)V * P) return;
+    const int p = (int)(e / P), c = (int)(e % P);
+    const int pair = P0 + c, s = pair / BC_PAIRS, t = pair % BC_PAIRS;
+    double a = 0.0, b = 0.0;
+    if (mask[s]) {
+        const size_t o = (size_t)p * S + s;
+        const double A1 = (1.0 - th) * u1[o] + th * u1n[o];
+        const double A2 = (1.0 - th) * u2[o] + th * u2n[o];
+        const double B1 = ((1.0 - th) * w1[o] + th * w1n[o]) + lh[p];
+        const double B2 = (1.0 - th) * w2[o] + th * w2n[o];
+        switch (t) {
+        case 0: a = A1, b = B1; break;
+        case 1: a = A2, b = B2; break;
+        case 2: a = A1, b = B2; break;
+        case 3: a = A2, b = B1; break;
+        default: a = A1, b = A2; break;
+        }
+    }
+    PU[e] = a;
+    PV[e] = b;
+}
+
+// F = (-J1 - J2) + FC_inner, G = ((-J3 - J4) + alpha^2 J5) + GC_inner
+// (vec_sum1 order of baroclin.cpp:556-585); JAC is n x 5S.
+__global__ void bc_fg(int n, int S, const int* __restrict__ inner, const double* __restrict__ JAC,
+                      const double* __restrict__ FC, const double* __restrict__ GC, double a2,
+                      double* __restrict__ F, double* __restrict__ G, const int* __restrict__ mask)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * S) return;
+    const int i = (int)(e / S), s = (int)(e % S);
+    if (!mask[s]) {
+        F[e] = 0.0;
+        G[e] = 0.0;
+        return;
+    }
+    const double* j = JAC + (size_t)i * BC_PAIRS * S + (size_t)s * BC_PAIRS;
+    const size_t o = (size_t)inner[i] * S + s;
+    double f = -1.0 * j[0] + -1.0 * j[1];
+    F[e] = f + FC[o];
+    double g = -1.0 * j[2] + -1.0 * j[3];
+    g = 1.0 * g + a2 * j[4];
+    G[e] = g + GC[o];
+}
+
+// x[b * n + i][s] = u[inner[i]][s] for the four unknown blocks (the initial
+// guess of the 4n solve: the current w1, w2, u1, u2)
+__global__ void bc_pack4(int n, int S, const int* __restrict__ inner, const double* __restrict__ a,
+                         const double* __restrict__ b, const double* __restrict__ c,
+                         const double* __restrict__ d, double* __restrict__ x,
+                         const int* __restrict__ mask)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)4 * n * S) return;
+    const size_t row = e / S;
+    const int s = (int)(e % S), blk = (int)(row / n), i = (int)(row % n);
+    const double* src = blk == 0 ? a : blk == 1 ? b : blk == 2 ? c : d;
+    x[e] = mask[s] ? src[(size_t)inner[i] * S + s] : 0.0;
+}
+
+}  // namespace
+
+// The block preconditioner (see the file comment) on 4n x KP vectors.
+struct BcPrecond : BatchPrecond {
+    int n = 0, kp = 0;
+    phg_matrix_s* A21 = nullptr;   // block (2,1)
+    phg_matrix_s* M = nullptr;     // block (3,1) = mass
+    phg_mg_s* mg = nullptr;        // V-cycle of the Poisson operator L
+    DevBuf<double> d11, d22;       // inverse diagonals of A11, A22
+    DevBuf<double> t;              // n x kp
+    int rows() const override { return 4 * n; }
+    int reserve(int k) override
+    {
+        if (kp < k) {
+            PHG_HIP(t.alloc((size_t)n * k));
+            kp = k;
+        }
+        return mg_reserve(mg, k);
+    }
+    void apply(int KP, const double* v, double* z, hipStream_t s) override
+    {
+        const size_t nb = (size_t)n * KP;
+        const dim3 g = g256(nb), b(BV_BS);
+        const double *v0 = v, *v1 = v + nb, *v2 = v + 2 * nb, *v3 = v + 3 * nb;
+        double *z0 = z, *z1 = z + nb, *z2 = z + 2 * nb, *z3 = z + 3 * nb;
+        const Ell<double>& E21 = A21->ell64;
+        const Ell<double>& EM = M->ell64;
+        // z0 = D11^-1 v0;  z1 = D22^-1 (v1 - A21 z0)
+        hipLaunchKernelGGL(bc_scale_rows, g, b, 0, s, n, KP, d11.p, v0, z0);
+        hipLaunchKernelGGL(bc_spmm_lin, g, b, 0, s, n, KP, E21.width, E21.col.p, E21.val.p, z0,
+                           -1.0, 1.0, v1, t.p);
+        hipLaunchKernelGGL(bc_scale_rows, g, b, 0, s, n, KP, d22.p, t.p, z1);
+        // z2 = L^-1 (M z0 - v2), z3 = L^-1 (M z1 - v3): one V-cycle each
+        hipLaunchKernelGGL(bc_spmm_lin, g, b, 0, s, n, KP, EM.width, EM.col.p, EM.val.p, z0, 1.0,
+                           -1.0, v2, t.p);
+        mg_apply(mg, KP, t.p, z2, s);
+        hipLaunchKernelGGL(bc_spmm_lin, g, b, 0, s, n, KP, EM.width, EM.col.p, EM.val.p, z1, 1.0,
+                           -1.0, v3, t.p);
+        mg_apply(mg, KP, t.p, z3, s);
+    }
+};
+
+struct phg_baroclin_s {
+    phg_mesh_s* mesh = nullptr;
+    int V = 0, n = 0, os = 0, S = 0;
+    phg_bc_conf conf{};
+    double theta = 0.5;              // baroclin.cpp:80
+    phg_matrix_s* blk[9] = {};       // the blocks of A_
+    phg_matrix_t A4 = nullptr;       // A_ (4n x 4n)
+    phg_matrix_s* bnd3 = nullptr;    // SphereLaplace::bnd3_ (stiffness, boundary)
+    phg_matrix_s* norm = nullptr;    // SphereNorm::NORM_
+    phg_jacobian_t J = nullptr;
+    phg_mg_t mg = nullptr;           // Poisson V-cycle (preconditioner)
+    BcPrecond pc;
+    DevBuf<double> lh;                                      // V
+    DevBuf<double> U1, U2, W1, W2, FC, GC, U1n, U2n, W1n, W2n, U1n1, U2n1, T;  // V x S
+    DevBuf<double> PU, PV;                                  // V x 5 (one member's pairs)
+    DevBuf<double> JAC;                                     // n x 5S
+    DevBuf<double> F, G, Y, Q, X;                           // n x S
+    DevBuf<double> Ub;                                      // outer x S
+    DevBuf<double> RP, ANS;                                 // 4n x S
+    DevBuf<double> part, nrm;
+    DevBuf<int> dmask;
+    int Gn = 0;
+    ~phg_baroclin_s()
+    {
+        for (auto* b : blk) delete b;
+        if (A4) phg_matrix_destroy(A4);
+        delete bnd3;
+        delete norm;
+        if (J) phg_jacobian_destroy(J);
+        if (mg) phg_mg_destroy(mg);
+    }
+};
+
+namespace {
+
+int upload_mask(phg_baroclin_s* B, const std::vector<int>& m, hipStream_t s)
+{
+    PHG_HIP(hipMemcpyAsync(B->dmask.p, m.data(), sizeof(int) * B->S, hipMemcpyHostToDevice, s));
+    return PHG_OK;
+}
+
+// y = A x for the masked slots (CSR, n_rows x S), accumulate into y if acc.
+void spmm(phg_baroclin_s* B, const phg_matrix_s* M, const double* X, double* Y, int acc,
+          hipStream_t s)
+{
+    hipLaunchKernelGGL(bv_spmm, g256((size_t)M->n_rows * B->S), dim3(BV_BS), 0, s, M->n_rows,
+                       B->S, M->rp.p, M->ci.p, M->val.p, X, Y, acc, B->dmask.p);
+}
+
+phg_gmres_opts solve_opts(double rtol, int ortho, int use_x0, int max_iters)
+{
+    phg_gmres_opts o;
+    phg_gmres_default_opts(&o);
+    o.restart = 30;
+    o.rtol = rtol;
+    o.ortho = ortho;
+    o.precision = PHG_F64;
+    o.use_x0 = use_x0;
+    o.max_iters = max_iters;
+    return o;
+}
+
+// SphereLaplace::calc1 (src/impl/slaplace_impl.h:153-192) with zero boundary
+// values for the masked slots: out = p2u(idt^-1 (laplace u_in + bnd3 u_bnd)).
+int calc1(phg_baroclin_s* B, const double* u, double* out, const phg_bc_opts& o,
+          std::vector<int>& acc, hipStream_t s)
+{
+    const int S = B->S, n = B->n;
+    phg_matrix_t lap = nullptr;
+    phg_mg_matrix(B->mg, &lap);
+    hipLaunchKernelGGL(bv_u2p, g256((size_t)n * S), dim3(BV_BS), 0, s, n, S, B->mesh->d_inner.p,
+                       u, B->Y.p, B->dmask.p);
+    if (B->os)
+        hipLaunchKernelGGL(bv_outer, g256((size_t)B->os * S), dim3(BV_BS), 0, s, B->os, S,
+                           B->mesh->d_outer.p, u, B->Ub.p, B->dmask.p);
+    spmm(B, lap, B->Y.p, B->Q.p, 0, s);
+    if (B->os) spmm(B, B->bnd3, B->Ub.p, B->Q.p, 1, s);
+    hipLaunchKernelGGL(bv_zero_cols, g256((size_t)n * S), dim3(BV_BS), 0, s, (size_t)n, S, B->Q.p,
+                       B->dmask.p);
+    PHG_HIP(hipGetLastError());
+    phg_gmres_opts go = solve_opts(o.rtol_mass, PHG_CGS1, 0, o.max_solver_iters);
+    std::vector<phg_gmres_info> inf(S);
+    int rc = gmres_batch_solve(B->blk[5], S, B->Q.p, B->X.p, &go, inf.data(), s);
+    if (rc < 0) return rc;
+    for (int k = 0; k < S; ++k) acc[k] += inf[k].iterations;
+    hipLaunchKernelGGL(bv_p2u, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
+                       B->mesh->d_isb.p, B->mesh->d_p2io.p, B->X.p, out, B->dmask.p);
+    PHG_HIP(hipGetLastError());
+    return PHG_OK;
+}
+
+// out[k] = ||a - b|| (SphereNorm) per slot, to the host (synchronises)
+int dists(phg_baroclin_s* B, const double* a, const double* b, std::vector<double>& out,
+          hipStream_t s)
+{
+    const int S = B->S;
+    hipLaunchKernelGGL(bv_normsq, dim3(B->Gn), dim3(BV_BS), 0, s, B->V, S, B->norm->rp.p,
+                       B->norm->ci.p, B->norm->val.p, a, b, B->part.p, B->Gn, B->dmask.p);
+    hipLaunchKernelGGL(bv_sum_cols, dim3(1), dim3(64 * BV_MAXS), 0, s, S, B->part.p, B->Gn,
+                       B->nrm.p);
+    PHG_HIP(hipGetLastError());
+    out.assign(S, 0.0);
+    PHG_HIP(hipMemcpyAsync(out.data(), B->nrm.p, sizeof(double) * S, hipMemcpyDeviceToHost, s));
+    PHG_HIP(hipStreamSynchronize(s));
+    return PHG_OK;
+}
+
+// The 4n x 4n CSR of the nine blocks (all on the inner stencil).
+int stack_blocks(phg_baroclin_s* B)
+{
+    const int n = B->n;
+    std::vector<int> rp(n + 1), ci;
+    long nnz = 0;
+    int nr = 0, nc = 0;
+    PHG_HIP(hipDeviceSynchronize());
+    if (phg_matrix_info(B->blk[0], &nr, &nc, &nnz) != PHG_OK) return PHG_ERR_ARG;
+    ci.resize((size_t)nnz);
+    std::vector<std::vector<double>> v(9, std::vector<double>((size_t)nnz));
+    for (int k = 0; k < 9; ++k) {
+        int rc = phg_matrix_csr(B->blk[k], k == 0 ? rp.data() : nullptr,
+                                k == 0 ? ci.data() : nullptr, v[k].data());
+        if (rc) return rc;
+        if (B->blk[k]->h_rp != B->blk[0]->h_rp || B->blk[k]->h_ci != B->blk[0]->h_ci) {
+            set_error("phg_baroclin_create: blocks differ in structure");
+            return PHG_ERR_ARG;
+        }
+    }
+    std::vector<int> brp((size_t)4 * n + 1, 0), bci;
+    std::vector<double> bv;
+    bci.reserve((size_t)9 * nnz);
+    bv.reserve((size_t)9 * nnz);
+    for (int r = 0; r < 4; ++r)
+        for (int i = 0; i < n; ++i) {
+            for (int c = 0; c < 4; ++c)
+                for (int k = 0; k < 9; ++k)
+                    if (BC_BR[k] == r && BC_BC[k] == c)
+                        for (int e = rp[i]; e < rp[i + 1]; ++e) {
+                            bci.push_back(ci[e] + c * n);
+                            bv.push_back(v[k][e]);
+                        }
+            brp[(size_t)r * n + i + 1] = (int)bci.size();
+        }
+    return phg_matrix_from_csr(4 * n, 4 * n, brp.data(), bci.data(), bv.data(), &B->A4);
+}
+
+}  // namespace
+
+extern "C" {
+
+void phg_bc_default_opts(phg_bc_opts* o)
+{
+    o->max_iters = 20;          // baroclin.cpp:545
+    o->tol = 1e-8;              // baroclin.cpp:598
+    o->rtol_mass = 1e-13;
+    o->rtol_system = 1e-12;
+    o->max_solver_iters = 2000;
+    o->precondition = 1;
+}
+
+int phg_baroclin_create(phg_mesh_t m, const phg_bc_conf* conf, const double* d_lh, int slots,
+                        phg_stream_t stream, phg_baroclin_t* out)
+{
+    if (!m || !conf || !d_lh || !out || slots < 1 || slots > BV_MAXS ||
+        (slots & (slots - 1)) != 0) {
+        set_error("phg_baroclin_create: bad argument (slots must be 1, 2, 4 or 8)");
+        return PHG_ERR_ARG;
+    }
+    if (!(conf->tau > 0.0) || conf->alpha == 0.0) {
+        set_error("phg_baroclin_create: tau must be positive and alpha non-zero");
+        return PHG_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    std::unique_ptr<phg_baroclin_s> B(new phg_baroclin_s());
+    B->mesh = m;
+    B->conf = *conf;
+    B->S = slots;
+    B->V = m->h.n_points;
+    B->n = (int)m->h.inner.size();
+    B->os = (int)m->h.outer.size();
+    const phg_bc_conf& c = *conf;
+    const double tau = c.tau, sigma = c.sigma, mu = c.mu, sigma1 = c.sigma1, mu1 = c.mu1,
+                 alpha = c.alpha, theta = B->theta;
+    // integrate_cb's nine element formulas as (c_mass, c_lap) pairs, the
+    // coefficients evaluated in the reference's order (baroclin.cpp:142-178)
+    const double cm[9] = {1.0 / tau + 0.5 * theta * sigma,
+                          (-0.5 * sigma) * theta,
+                          0.5 * theta * sigma,
+                          1.0 / tau + 0.5 * theta * sigma + alpha * alpha * theta * mu1,
+                          -alpha * alpha / tau - alpha * alpha * theta * sigma1,
+                          1.0, 0.0, 1.0, 0.0};
+    const double cl[9] = {-(theta * mu), 0.0, 0.0, -(theta * mu), 0.0, 0.0, -1.0, 0.0, -1.0};
+    for (int k = 0; k < 9; ++k) {
+        const int kind = (k == 5 || k == 7) ? PHG_MASS : PHG_HELMHOLTZ;
+        int rc = assemble_device(m, kind, cm[k], cl[k], s, &B->blk[k]);
+        if (rc) return rc;
+    }
+    int rc = stack_blocks(B.get());
+    if (rc) return rc;
+    rc = assemble_boundary_device(m, PHG_LAPLACE, 1.0, 0.0, s, &B->bnd3);
+    if (rc) return rc;
+    rc = assemble_device(m, PHG_MASS, 1.0, 0.0, s, &B->norm, true);
+    if (rc) return rc;
+    rc = phg_jacobian_create(m, stream, &B->J);
+    if (rc) return rc;
+    rc = phg_mg_create(m, PHG_LAPLACE, 1.0, 0.0, stream, &B->mg);
+    if (rc) return rc;
+    rc = ensure_mesh_device(m);
+    if (rc) return rc;
+    const size_t V = B->V, n = B->n, S = slots;
+    // preconditioner pieces
+    BcPrecond& P = B->pc;
+    P.n = B->n;
+    P.A21 = B->blk[2];
+    P.M = B->blk[5];
+    P.mg = B->mg;
+    for (int k : {0, 2, 3, 5}) {
+        rc = ensure_ell64(B->blk[k], s);
+        if (rc) return rc;
+    }
+    PHG_HIP(P.d11.alloc(n));
+    PHG_HIP(P.d22.alloc(n));
+    if (n) {
+        hipLaunchKernelGGL(bc_diag_inv, g256(n), dim3(BV_BS), 0, s, (int)n, B->blk[0]->rp.p,
+                           B->blk[0]->ci.p, B->blk[0]->val.p, P.d11.p);
+        hipLaunchKernelGGL(bc_diag_inv, g256(n), dim3(BV_BS), 0, s, (int)n, B->blk[3]->rp.p,
+                           B->blk[3]->ci.p, B->blk[3]->val.p, P.d22.p);
+    }
+    PHG_HIP(B->lh.alloc(V));
+    PHG_HIP(hipMemcpyAsync(B->lh.p, d_lh, V * sizeof(double), hipMemcpyDeviceToDevice, s));
+    DevBuf<double>* full[] = {&B->U1, &B->U2, &B->W1, &B->W2, &B->FC, &B->GC, &B->U1n,
+                              &B->U2n, &B->W1n, &B->W2n, &B->U1n1, &B->U2n1, &B->T};
+    for (auto* b : full) {
+        PHG_HIP(b->alloc(V * S));
+        PHG_HIP(hipMemsetAsync(b->p, 0, V * S * sizeof(double), s));
+    }
+    PHG_HIP(B->PU.alloc(V * BC_PAIRS));
+    PHG_HIP(B->PV.alloc(V * BC_PAIRS));
+    PHG_HIP(B->JAC.alloc(std::max<size_t>(n * BC_PAIRS * S, 1)));
+    DevBuf<double>* inner[] = {&B->F, &B->G, &B->Y, &B->Q, &B->X};
+    for (auto* b : inner) {
+        PHG_HIP(b->alloc(std::max<size_t>(n * S, 1)));
+        PHG_HIP(hipMemsetAsync(b->p, 0, std::max<size_t>(n * S, 1) * sizeof(double), s));
+    }
+    PHG_HIP(B->Ub.alloc(std::max<size_t>((size_t)B->os * S, 1)));
+    PHG_HIP(B->RP.alloc(std::max<size_t>(4 * n * S, 1)));
+    PHG_HIP(B->ANS.alloc(std::max<size_t>(4 * n * S, 1)));
+    PHG_HIP(hipMemsetAsync(B->RP.p, 0, std::max<size_t>(4 * n * S, 1) * sizeof(double), s));
+    B->Gn = std::max(1, (int)((V + (BV_BS / S) - 1) / (BV_BS / S)));
+    PHG_HIP(B->part.alloc((size_t)B->Gn * S));
+    PHG_HIP(B->nrm.alloc(S));
+    PHG_HIP(B->dmask.alloc(S));
+    PHG_HIP(hipStreamSynchronize(s));
+    *out = B.release();
+    return PHG_OK;
+}
+
+void phg_baroclin_destroy(phg_baroclin_t B) { delete B; }
+
+int phg_baroclin_slots(phg_baroclin_t B) { return B ? B->S : 0; }
+
+int phg_baroclin_matrix(phg_baroclin_t B, phg_matrix_t* out)
+{
+    if (!B || !out) {
+        set_error("phg_baroclin_matrix: null argument");
+        return PHG_ERR_ARG;
+    }
+    *out = B->A4;
+    return PHG_OK;
+}
+
+int phg_baroclin_calc(phg_baroclin_t B, int nmembers, const double* d_u1, const double* d_u2,
+                      const double* d_f, const double* d_g, double* d_u11, double* d_u21,
+                      const phg_bc_opts* opts, phg_bc_info* infos, phg_stream_t stream)
+{
+    if (!B || !d_u1 || !d_u2 || !d_u11 || !d_u21 || nmembers < 1 || nmembers > B->S) {
+        set_error("phg_baroclin_calc: bad argument (1 <= nmembers <= slots)");
+        return PHG_ERR_ARG;
+    }
+    phg_bc_opts o;
+    if (opts) o = *opts;
+    else phg_bc_default_opts(&o);
+    hipStream_t s = (hipStream_t)stream;
+    const int S = B->S, n = B->n;
+    const size_t V = B->V;
+    const phg_bc_conf& c = B->conf;
+    const double th = B->theta, tau = c.tau, sigma = c.sigma, mu = c.mu, sigma1 = c.sigma1,
+                 mu1 = c.mu1, alpha = c.alpha, a2 = alpha * alpha;
+    std::vector<int> mask(S, 0);
+    for (int k = 0; k < nmembers; ++k) mask[k] = 1;
+    int rc = upload_mask(B, mask, s);
+    if (rc) return rc;
+    for (int k = 0; k < nmembers; ++k) {
+        hipLaunchKernelGGL(bv_col_in, g256(V), dim3(BV_BS), 0, s, (int)V, S, k, d_u1 + k * V,
+                           B->U1.p);
+        hipLaunchKernelGGL(bv_col_in, g256(V), dim3(BV_BS), 0, s, (int)V, S, k, d_u2 + k * V,
+                           B->U2.p);
+    }
+    PHG_HIP(hipGetLastError());
+    std::vector<int> it_mass(S, 0), it_jac(S, 0), it_sys(S, 0), iters(S, 0);
+    std::vector<double> dist(S, 0.0);
+    // w = L(u), dw = L(w)  (dw1, dw2 in U1n1, U2n1 as scratch)
+    if ((rc = calc1(B, B->U1.p, B->W1.p, o, it_mass, s))) return rc;
+    if ((rc = calc1(B, B->U2.p, B->W2.p, o, it_mass, s))) return rc;
+    if ((rc = calc1(B, B->W1.p, B->U1n1.p, o, it_mass, s))) return rc;
+    if ((rc = calc1(B, B->W2.p, B->U2n1.p, o, it_mass, s))) return rc;
+    const dim3 gv = g256(V * S), bb(BV_BS);
+    // FC and GC, vec_sum1 by vec_sum1 as baroclin.cpp:496-513
+    auto lin = [&](double k1, const double* a, double k2, const double* b, double* r) {
+        hipLaunchKernelGGL(bv_axpby, gv, bb, 0, s, V, S, k1, a, k2, b, r, B->dmask.p, 1);
+    };
+    lin(-0.5 * (1.0 - th) * sigma, B->W1.p, 0.5 * (1.0 - th) * sigma, B->W2.p, B->FC.p);
+    lin(1.0, B->FC.p, mu * (1.0 - th), B->U1n1.p, B->FC.p);
+    lin(1.0, B->FC.p, 1.0 / tau, B->W1.p, B->FC.p);
+    lin(-0.5 * (1.0 - th) * sigma, B->W1.p, -0.5 * (1.0 - th) * sigma, B->W2.p, B->GC.p);
+    lin(1.0, B->GC.p, mu * (1.0 - th), B->U2n1.p, B->GC.p);
+    lin(1.0, B->GC.p, 1.0 / tau, B->W2.p, B->GC.p);
+    lin(1.0, B->GC.p, -alpha * alpha / tau, B->U2.p, B->GC.p);
+    lin(1.0, B->GC.p, -alpha * alpha * mu1 * (1 - th), B->W2.p, B->GC.p);
+    lin(1.0, B->GC.p, alpha * alpha * sigma1 * (1 - th), B->U2.p, B->GC.p);
+    // forcing: FC += f, GC += alpha^2 g (baroclin.cpp:520-535)
+    if (d_f)
+        hipLaunchKernelGGL(bv_add_shared, gv, bb, 0, s, (int)V, S, B->FC.p, 1.0, d_f, B->FC.p,
+                           B->dmask.p);
+    if (d_g)
+        hipLaunchKernelGGL(bv_add_shared, gv, bb, 0, s, (int)V, S, B->GC.p, a2, d_g, B->GC.p,
+                           B->dmask.p);
+    // iterate from (u1, u2, w1, w2)
+    hipLaunchKernelGGL(bv_copy_cols, gv, bb, 0, s, V, S, B->U1.p, B->U1n.p, B->dmask.p);
+    hipLaunchKernelGGL(bv_copy_cols, gv, bb, 0, s, V, S, B->U2.p, B->U2n.p, B->dmask.p);
+    hipLaunchKernelGGL(bv_copy_cols, gv, bb, 0, s, V, S, B->W1.p, B->W1n.p, B->dmask.p);
+    hipLaunchKernelGGL(bv_copy_cols, gv, bb, 0, s, V, S, B->W2.p, B->W2n.p, B->dmask.p);
+    hipLaunchKernelGGL(bc_pack4, g256((size_t)4 * n * S), bb, 0, s, n, S, B->mesh->d_inner.p,
+                       B->W1.p, B->W2.p, B->U1.p, B->U2.p, B->ANS.p, B->dmask.p);
+    PHG_HIP(hipGetLastError());
+    phg_gmres_opts jo = solve_opts(o.rtol_mass, PHG_CGS1, 0, o.max_solver_iters);
+    jo.restart = 32;
+    phg_gmres_opts so = solve_opts(o.rtol_system, PHG_CGS2, 1, o.max_solver_iters);
+    const int P = BC_PAIRS * S;
+    std::vector<phg_gmres_info> jinf((size_t)8 * BC_PAIRS), sinf(S);
+    std::vector<int> running = mask;
+    for (int pass = 1; pass <= o.max_iters; ++pass) {
+        // the five Jacobians of each running member, one call per member (the
+        // same call shape alone or beside other members, so the same bits)
+        for (int k = 0; k < S; ++k) {
+            if (!running[k]) continue;
+            const int p0 = BC_PAIRS * k, np = BC_PAIRS;
+            hipLaunchKernelGGL(bc_pairs, g256(V * np), bb, 0, s, (int)V, S, p0, np, th, B->U1.p,
+                               B->U1n.p, B->U2.p, B->U2n.p, B->W1.p, B->W1n.p, B->W2.p, B->W2n.p,
+                               B->lh.p, B->PU.p, B->PV.p, B->dmask.p);
+            PHG_HIP(hipGetLastError());
+            rc = jacobian_calc2_strided(B->J, np, B->PU.p, B->PV.p, np, 1, B->JAC.p + p0, P, 1,
+                                        1.0, &jo, jinf.data(), s);
+            if (rc < 0) return rc;
+            for (int q = 0; q < 8 * np; ++q) it_jac[k] += jinf[q].iterations;
+        }
+        hipLaunchKernelGGL(bc_fg, g256((size_t)n * S), bb, 0, s, n, S, B->mesh->d_inner.p,
+                           B->JAC.p, B->FC.p, B->GC.p, a2, B->F.p, B->G.p, B->dmask.p);
+        // right part [M F; M G; 0; 0] and the 4n solve from the previous answer
+        spmm(B, B->blk[5], B->F.p, B->RP.p, 0, s);
+        spmm(B, B->blk[5], B->G.p, B->RP.p + (size_t)n * S, 0, s);
+        hipLaunchKernelGGL(bv_zero_cols, g256((size_t)4 * n * S), bb, 0, s, (size_t)4 * n, S,
+                           B->RP.p, B->dmask.p);
+        hipLaunchKernelGGL(bv_zero_cols, g256((size_t)4 * n * S), bb, 0, s, (size_t)4 * n, S,
+                           B->ANS.p, B->dmask.p);
+        PHG_HIP(hipGetLastError());
+        rc = gmres_batch_solve_pc(B->A4, S, B->RP.p, B->ANS.p, &so, sinf.data(), s,
+                                  o.precondition ? &B->pc : nullptr);
+        if (rc < 0) return rc;
+        for (int k = 0; k < S; ++k) it_sys[k] += running[k] ? sinf[k].iterations : 0;
+        // w1_n, w2_n, u1_n1, u2_n1 = p2u of the four parts
+        const double* ans = B->ANS.p;
+        const size_t nb = (size_t)n * S;
+        hipLaunchKernelGGL(bv_p2u, gv, bb, 0, s, (int)V, S, B->mesh->d_isb.p, B->mesh->d_p2io.p,
+                           ans, B->W1n.p, B->dmask.p);
+        hipLaunchKernelGGL(bv_p2u, gv, bb, 0, s, (int)V, S, B->mesh->d_isb.p, B->mesh->d_p2io.p,
+                           ans + nb, B->W2n.p, B->dmask.p);
+        hipLaunchKernelGGL(bv_p2u, gv, bb, 0, s, (int)V, S, B->mesh->d_isb.p, B->mesh->d_p2io.p,
+                           ans + 2 * nb, B->U1n1.p, B->dmask.p);
+        hipLaunchKernelGGL(bv_p2u, gv, bb, 0, s, (int)V, S, B->mesh->d_isb.p, B->mesh->d_p2io.p,
+                           ans + 3 * nb, B->U2n1.p, B->dmask.p);
+        PHG_HIP(hipGetLastError());
+        std::vector<double> d1, d2;
+        if ((rc = dists(B, B->U1n1.p, B->U1n.p, d1, s))) return rc;
+        if ((rc = dists(B, B->U2n1.p, B->U2n.p, d2, s))) return rc;
+        hipLaunchKernelGGL(bv_copy_cols, gv, bb, 0, s, V, S, B->U1n1.p, B->U1n.p, B->dmask.p);
+        hipLaunchKernelGGL(bv_copy_cols, gv, bb, 0, s, V, S, B->U2n1.p, B->U2n.p, B->dmask.p);
+        PHG_HIP(hipGetLastError());
+        bool any = false;
+        for (int k = 0; k < S; ++k) {
+            if (!running[k]) continue;
+            iters[k] = pass;
+            dist[k] = std::max(d1[k], d2[k]);
+            if (dist[k] < o.tol || !std::isfinite(dist[k])) running[k] = 0;
+            any = any || running[k];
+        }
+        if (!any) break;
+        if ((rc = upload_mask(B, running, s))) return rc;
+    }
+    bool all = true;
+    for (int k = 0; k < nmembers; ++k) {
+        hipLaunchKernelGGL(bv_col_out, g256(V), dim3(BV_BS), 0, s, (int)V, S, k, B->U1n.p,
+                           d_u11 + k * V);
+        hipLaunchKernelGGL(bv_col_out, g256(V), dim3(BV_BS), 0, s, (int)V, S, k, B->U2n.p,
+                           d_u21 + k * V);
+        const bool conv = dist[k] < o.tol;
+        all = all && conv;
+        if (infos) {
+            infos[k].iterations = iters[k];
+            infos[k].distance = dist[k];
+            infos[k].converged = conv;
+            infos[k].it_mass = it_mass[k];
+            infos[k].it_jacobian = it_jac[k];
+            infos[k].it_system = it_sys[k];
+        }
+    }
+    PHG_HIP(hipGetLastError());
+    PHG_HIP(hipStreamSynchronize(s));
+    return all ? PHG_OK : PHG_ERR_NOCONV;
+}
+
+}  // extern "C"

@@ -668,14 +668,14 @@ __global__ void bk_fill0(size_t n, double* __restrict__ x)
 // ---------------------------------------------------------------- driver --
 // The preconditioner works on FP64 vectors (the form it is used with).
 template <typename VT>
-void precond_apply(phg_mg_s* pc, int KP, const VT* v, VT* z, hipStream_t s);
+void precond_apply(BatchPrecond* pc, int KP, const VT* v, VT* z, hipStream_t s);
 template <>
-void precond_apply<double>(phg_mg_s* pc, int KP, const double* v, double* z, hipStream_t s)
+void precond_apply<double>(BatchPrecond* pc, int KP, const double* v, double* z, hipStream_t s)
 {
-    mg_apply(pc, KP, v, z, s);
+    pc->apply(KP, v, z, s);
 }
 template <>
-void precond_apply<float>(phg_mg_s*, int, const float*, float*, hipStream_t)
+void precond_apply<float>(BatchPrecond*, int, const float*, float*, hipStream_t)
 {
 }
 
@@ -743,7 +743,7 @@ int prepare_batch(phg_matrix_s* A, int slot, int prec, int m, int kp, int rpt, b
 template <typename VT, typename MT, int RPT>
 void enqueue_batch_cycle(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K,
                          const double* B, double* X, int m, int passes, double rtol,
-                         double inner_rtol, int max_iters, phg_mg_s* pc, hipStream_t s)
+                         double inner_rtol, int max_iters, BatchPrecond* pc, hipStream_t s)
 {
     const int n = w->n, KP = w->kp, G = w->G;
     const size_t bs = (size_t)n * KP;
@@ -795,7 +795,7 @@ void enqueue_batch_cycle(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K,
 
 template <typename VT, typename MT, int RPT>
 int batch_run_t(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const double* B, double* X,
-              const phg_gmres_opts& o, phg_gmres_info* infos, phg_mg_s* pc, hipStream_t s)
+              const phg_gmres_opts& o, phg_gmres_info* infos, BatchPrecond* pc, hipStream_t s)
 {
     const int n = w->n, KP = w->kp, G = w->G, m = o.restart;
     const size_t bs = (size_t)n * KP;
@@ -881,7 +881,7 @@ int batch_run_t(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const do
 
 template <typename VT, typename MT>
 int batch_run(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const double* B, double* X,
-              const phg_gmres_opts& o, phg_gmres_info* infos, phg_mg_s* pc, hipStream_t s)
+              const phg_gmres_opts& o, phg_gmres_info* infos, BatchPrecond* pc, hipStream_t s)
 {
     switch (w->rpt) {
     case 1: return batch_run_t<VT, MT, 1>(A, E, w, K, B, X, o, infos, pc, s);
@@ -944,7 +944,14 @@ __global__ void bk_fill_hash(size_t n, double* __restrict__ x)
 
 int gmres_batch_solve(phg_matrix_s* A, int K, const double* B, double* X,
                       const phg_gmres_opts* opts, phg_gmres_info* infos, hipStream_t s,
-                      phg_mg_s* pc)
+                      phg_mg_s* mg)
+{
+    return gmres_batch_solve_pc(A, K, B, X, opts, infos, s, mg ? mg_precond(mg) : nullptr);
+}
+
+int gmres_batch_solve_pc(phg_matrix_s* A, int K, const double* B, double* X,
+                         const phg_gmres_opts* opts, phg_gmres_info* infos, hipStream_t s,
+                         BatchPrecond* pc)
 {
     phg_gmres_opts o = *opts;
     if (K < 1 || K > BB_MAXK) {
@@ -980,11 +987,11 @@ int gmres_batch_solve(phg_matrix_s* A, int K, const double* B, double* X,
     }
     const int kp = next_pow2(K);
     if (pc) {
-        if (o.precision != PHG_F64 || mg_rows(pc) != A->n_rows) {
-            set_error("gmres_batch: the multigrid preconditioner needs PHG_F64 and its own matrix");
+        if (o.precision != PHG_F64 || pc->rows() != A->n_rows) {
+            set_error("gmres_batch: a preconditioner needs PHG_F64 and a matrix of its size");
             return PHG_ERR_ARG;
         }
-        rc = mg_reserve(pc, kp);
+        rc = pc->reserve(kp);
         if (rc) return rc;
     }
     BatchWork* w = nullptr;

@@ -54,12 +54,24 @@ struct MgLevel {
     }
 };
 
+struct phg_mg_s;
+namespace phg {
+// One V-cycle as the batched GMRES's right preconditioner.
+struct MgPrecond : BatchPrecond {
+    phg_mg_s* M = nullptr;
+    int rows() const override { return mg_rows(M); }
+    int reserve(int kp) override { return mg_reserve(M, kp); }
+    void apply(int KP, const double* v, double* z, hipStream_t s) override { mg_apply(M, KP, v, z, s); }
+};
+}  // namespace phg
+
 struct phg_mg_s {
     int kind = 0, L = 0, kp = 0;
     double c_mass = 1.0, c_lap = 0.0;
     std::vector<std::unique_ptr<MgLevel>> lv;  // lv[0] = finest
     DevBuf<double> cinv;                       // coarsest operator's inverse
     int nc = 0;
+    phg::MgPrecond pc;
 };
 
 namespace {
@@ -293,6 +305,12 @@ int build_transfer(const SphereMesh& f, const SphereMesh& c, MgLevel& fl, int nc
 }  // namespace
 
 namespace phg {
+
+BatchPrecond* mg_precond(phg_mg_s* M)
+{
+    M->pc.M = M;
+    return &M->pc;
+}
 
 // Work vectors for kp columns (re-allocated when kp grows).
 int mg_reserve(phg_mg_s* M, int kp)

@@ -111,9 +111,22 @@ int assemble_boundary_device(phg_mesh_s* M, int kind, double c_mass, double c_la
 int mg_reserve(phg_mg_s* M, int kp);
 void mg_apply(phg_mg_s* M, int KP, const double* v, double* z, hipStream_t s);
 int mg_rows(const phg_mg_s* M);
+// Right preconditioner of the batched GMRES: z = P^-1 v for n x KP FP64
+// vectors, enqueued on s (captured into the cycle graph, so a fixed kernel
+// sequence); the object's address identifies it in the graph key.
+struct BatchPrecond {
+    virtual ~BatchPrecond() {}
+    virtual int rows() const = 0;
+    virtual int reserve(int kp) = 0;
+    virtual void apply(int KP, const double* v, double* z, hipStream_t s) = 0;
+};
+BatchPrecond* mg_precond(phg_mg_s* M);
 int gmres_batch_solve(phg_matrix_s* A, int K, const double* B, double* X,
                       const phg_gmres_opts* opts, phg_gmres_info* infos, hipStream_t s,
                       phg_mg_s* pc = nullptr);
+int gmres_batch_solve_pc(phg_matrix_s* A, int K, const double* B, double* X,
+                         const phg_gmres_opts* opts, phg_gmres_info* infos, hipStream_t s,
+                         BatchPrecond* pc);
 int gmres_batch_tune(phg_matrix_s* A, int K, const phg_gmres_opts* opts, hipStream_t s,
                      int* best, double* ms);
 int batch_shape_get(int n, int K, int prec);

@@ -96,6 +96,12 @@ SIGNATURES = {
     "phg_barvortex_load": (_I, [_P, _I, _P, _P]),
     "phg_barvortex_iterate": (_I, [_P, _P, _P, _P, _P]),
     "phg_barvortex_store": (_I, [_P, _I, _P, _P]),
+    "phg_bc_default_opts": (None, [_P]),
+    "phg_baroclin_create": (_I, [_P, _P, _P, _I, _P, C.POINTER(_P)]),
+    "phg_baroclin_destroy": (None, [_P]),
+    "phg_baroclin_slots": (_I, [_P]),
+    "phg_baroclin_matrix": (_I, [_P, C.POINTER(_P)]),
+    "phg_baroclin_calc": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "phg_serve_default_config": (None, [_P]),
     "phg_serve_create": (_I, [_P, C.POINTER(_P)]),
     "phg_serve_destroy": (None, [_P]),
@@ -541,6 +547,77 @@ class BarVortex:
     def close(self):
         if self.h:
             lib().phg_barvortex_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BcConf(C.Structure):
+    _fields_ = [("tau", _D), ("sigma", _D), ("mu", _D), ("sigma1", _D), ("mu1", _D),
+                ("alpha", _D)]
+
+
+class BcOpts(C.Structure):
+    _fields_ = [("max_iters", _I), ("tol", _D), ("rtol_mass", _D), ("rtol_system", _D),
+                ("max_solver_iters", _I), ("precondition", _I)]
+
+
+class BcInfo(C.Structure):
+    _fields_ = [("iterations", _I), ("distance", _D), ("converged", _I), ("it_mass", _I),
+                ("it_jacobian", _I), ("it_system", _I)]
+
+
+def bc_opts(**kw):
+    o = BcOpts()
+    lib().phg_bc_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Baroclin:
+    """Baroclin (examples/baroclin.h:60-292): the two-layer implicit step
+    Baroclin::calc on the GPU for up to `slots` members at once; lh is the
+    Coriolis + orography mesh function (device tensor of n_points values)."""
+
+    def __init__(self, mesh, lh, tau, sigma, mu, sigma1, mu1, alpha, slots=1, stream=None):
+        self.m = mesh
+        self.conf = BcConf(tau, sigma, mu, sigma1, mu1, alpha)
+        self.h = C.c_void_p()
+        check(lib().phg_baroclin_create(mesh.h, C.byref(self.conf), _ptr(lh), slots,
+                                        _stream(stream), C.byref(self.h)), "phg_baroclin_create")
+        self.slots = slots
+
+    def matrix(self):
+        """A_ (4n x 4n), owned by this object."""
+        h = C.c_void_p()
+        check(lib().phg_baroclin_matrix(self.h, C.byref(h)), "phg_baroclin_matrix")
+        return Matrix(h, owned=False)
+
+    def calc(self, u11, u21, u1, u2, f=None, g=None, opts=None, stream=None):
+        """u1, u2, u11, u21: (nmembers, n_points) FP64 device tensors (or
+        (n_points,)); f, g: the forcing at the mesh points for this step's t
+        (device tensors of n_points values) or None.  Returns one info dict
+        per member."""
+        k = 1 if u1.dim() == 1 else int(u1.shape[0])
+        infos = (BcInfo * k)()
+        rc = lib().phg_baroclin_calc(self.h, k, _ptr(u1), _ptr(u2),
+                                     _ptr(f) if f is not None else None,
+                                     _ptr(g) if g is not None else None, _ptr(u11), _ptr(u21),
+                                     C.byref(opts) if opts is not None else None, infos,
+                                     _stream(stream))
+        check(rc, "phg_baroclin_calc")
+        return [dict(iterations=i.iterations, distance=i.distance, converged=bool(i.converged),
+                     it_mass=i.it_mass, it_jacobian=i.it_jacobian, it_system=i.it_system)
+                for i in infos]
+
+    def close(self):
+        if self.h:
+            lib().phg_baroclin_destroy(self.h)
             self.h = None
 
     def __del__(self):

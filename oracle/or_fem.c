@@ -591,6 +591,91 @@ int or_assemble(const or_mesh *m, int kind, double c_mass, double c_lap,
     return 0;
 }
 
+/* Baroclin::Baroclin's A_ (examples/baroclin.cpp:69-181): generate_matrixv
+ * with integrate_cb, whose 9 elements per (row i, triangle, column j) are
+ * written here exactly as the reference writes them, from a = integrate_cos
+ * (phi_i phi_j) and b = slaplace(phi_j, phi_i).  The 4n x 4n matrix has
+ * nine n x n blocks with the inner stencil; vals9[k * nnz + e] is entry e of
+ * block k in the order (1,1) (1,2) (2,1) (2,2) (2,4) (3,1) (3,3) (4,2) (4,4)
+ * of the reference's push_back calls.  p = {tau, sigma, mu, sigma1, mu1,
+ * alpha, theta}. */
+int or_assemble_baroclin(const or_mesh *m, const double *p, const int *row_ptr,
+                         const int *col_idx, double *vals9)
+{
+    const double tau = p[0], sigma = p[1], mu = p[2], sigma1 = p[3], mu1 = p[4],
+                 alpha = p[5], theta = p[6];
+    const long nnz = row_ptr[m->n_inner];
+    int rs = m->n_inner;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < rs; ++i) {
+        for (int k = 0; k < 9; ++k)
+            for (int e = row_ptr[i]; e < row_ptr[i + 1]; ++e) vals9[k * nnz + e] = 0.0;
+        int pt = m->inner[i];
+        int zone = m->tri_zone[m->adj[m->adj_ptr[pt]]];
+        for (int e = m->adj_ptr[pt]; e < m->adj_ptr[pt + 1]; ++e) {
+            int t = m->adj[e];
+            double xs[3], ys[3];
+            or_poly phik[3];
+            tri_coords(m, t, zone, xs, ys);
+            or_basis_p1(xs, ys, phik);
+            const or_poly *phi_i = &phik[local_index(m, t, pt)];
+            for (int i0 = 0; i0 < 3; ++i0) {
+                int p2 = m->tri[3 * t + i0];
+                if (m->is_boundary[p2]) continue;
+                int j = m->p2io[p2];
+                double a = or_element(0, phi_i, &phik[i0], xs, ys, 0.0, 0.0);
+                double b = or_element(1, phi_i, &phik[i0], xs, ys, 0.0, 0.0);
+                double v[9];
+                v[0] = a * (1.0 / tau + 0.5 * theta * sigma) - theta * mu * b;
+                v[1] = -0.5 * a * sigma * theta;
+                v[2] = 0.5 * theta * sigma * a;
+                v[3] = a * (1.0 / tau + 0.5 * theta * sigma + alpha * alpha * theta * mu1)
+                       - theta * mu * b;
+                v[4] = a * (-alpha * alpha / tau - alpha * alpha * theta * sigma1);
+                v[5] = a;
+                v[6] = -b;
+                v[7] = a;
+                v[8] = -b;
+                long pos = find_pos(row_ptr, col_idx, i, j);
+                for (int k = 0; k < 9; ++k) vals9[k * nnz + pos] += v[k];
+            }
+        }
+    }
+    return 0;
+}
+
+/* generate_right_part (src/generators.h:245-287) with the interior branch of
+ * Baroclin's right_part_cb (examples/baroclin.cpp:329-336): b[i] sums
+ * F[j] * integrate_cos(phi_i phi_j) over the triangles around inner[i] (in
+ * adjacency order) and their inner vertices (in triangle order).  The
+ * boundary branch adds -bnd * element, zero for zero boundary values. */
+int or_right_part_mass(const or_mesh *m, const double *F, double *b)
+{
+    int rs = m->n_inner;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < rs; ++i) {
+        int pt = m->inner[i];
+        int zone = m->tri_zone[m->adj[m->adj_ptr[pt]]];
+        double acc = 0.0;
+        for (int e = m->adj_ptr[pt]; e < m->adj_ptr[pt + 1]; ++e) {
+            int t = m->adj[e];
+            double xs[3], ys[3];
+            or_poly phik[3];
+            tri_coords(m, t, zone, xs, ys);
+            or_basis_p1(xs, ys, phik);
+            const or_poly *phi_i = &phik[local_index(m, t, pt)];
+            for (int i0 = 0; i0 < 3; ++i0) {
+                int p2 = m->tri[3 * t + i0];
+                if (m->is_boundary[p2]) continue;
+                double a = or_element(0, phi_i, &phik[i0], xs, ys, 0.0, 0.0);
+                acc += F[m->p2io[p2]] * a;
+            }
+        }
+        b[i] = acc;
+    }
+    return 0;
+}
+
 /* generate_full_matrix (src/generators.h:200-234): all points x all points,
  * row p in the zone of its first triangle, no boundary skipped.  Structure:
  * row p holds every point of every triangle around p, ascending. */

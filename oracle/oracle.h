@@ -110,6 +110,13 @@ long or_csr_structure(const or_mesh *m, int *row_ptr, int *col_idx);
  * adjacency order (src/generators.cpp:57-114).  Uses OpenMP over rows. */
 int or_assemble(const or_mesh *m, int kind, double c_mass, double c_lap,
                 const int *row_ptr, const int *col_idx, double *vals);
+/* Baroclin's A_ as nine blocks (examples/baroclin.cpp:69-181); p = {tau,
+ * sigma, mu, sigma1, mu1, alpha, theta}; vals9: 9 x nnz. */
+int or_assemble_baroclin(const or_mesh *m, const double *p, const int *row_ptr,
+                         const int *col_idx, double *vals9);
+/* generate_right_part with the interior branch of Baroclin's right_part_cb
+ * (examples/baroclin.cpp:329-336): b = element-order sums of F_j * mass. */
+int or_right_part_mass(const or_mesh *m, const double *F, double *b);
 /* generate_full_matrix (src/generators.h:200-234): every point x every
  * point, no boundary skipped (SphereNorm's matrix, src/norm.h:71-74) */
 long or_full_structure(const or_mesh *m, int *row_ptr, int *col_idx);

@@ -82,6 +82,8 @@ def lib():
         L.or_assemble_full.argtypes = [C.POINTER(OrMesh), _I, _D, _D, _PI, _PI, _PD]
         L.or_assemble_boundary.argtypes = [C.POINTER(OrMesh), _I, _D, _D, _PI,
                                            _PI, _PD]
+        L.or_assemble_baroclin.argtypes = [C.POINTER(OrMesh), _PD, _PI, _PI, _PD]
+        L.or_right_part_mass.argtypes = [C.POINTER(OrMesh), _PD, _PD]
         L.or_spmv.argtypes = [_I, _PI, _PI, _PD, _PD, _PD]
         L.or_gmres.restype = _I
         L.or_gmres.argtypes = [_I, _PI, _PI, _PD, _PD, _PD, _I, _D, _I,
@@ -205,6 +207,50 @@ class Mesh:
                                    _pi(rp), _pi(np.ascontiguousarray(ci) if len(ci) else np.zeros(1, np.int32)),
                                    _pd(vals))
         return rp, ci, vals[:len(ci)]
+
+
+def baroclin_blocks(mesh, tau, sigma, mu, sigma1, mu1, alpha, theta=0.5):
+    """The nine n x n blocks of Baroclin's A_ (examples/baroclin.cpp:115-181)
+    on the inner stencil: (rp, ci, vals9) with vals9[k] the values of block
+    k in the order (1,1) (1,2) (2,1) (2,2) (2,4) (3,1) (3,3) (4,2) (4,4)."""
+    rp, ci = mesh.csr_structure()
+    vals = np.zeros(9 * len(ci))
+    prm = np.array([tau, sigma, mu, sigma1, mu1, alpha, theta], dtype=np.float64)
+    lib().or_assemble_baroclin(C.byref(mesh.c), _pd(prm), _pi(rp), _pi(ci), _pd(vals))
+    return rp, ci, vals.reshape(9, len(ci))
+
+
+BAROCLIN_BLOCKS = ((0, 0), (0, 1), (1, 0), (1, 1), (1, 3), (2, 0), (2, 2), (3, 1), (3, 3))
+
+
+def baroclin_matrix(rp, ci, vals9):
+    """The 4n x 4n CSR of the nine blocks (block (r, c) at rows r n.., columns
+    c n..), columns ascending in each row."""
+    n = len(rp) - 1
+    rows = [[] for _ in range(4)]
+    for k, (r, c) in enumerate(BAROCLIN_BLOCKS):
+        rows[r].append((c, k))
+    brp = [0]
+    bci, bv = [], []
+    for r in range(4):
+        blocks = sorted(rows[r])
+        for i in range(n):
+            for c, k in blocks:
+                s, e = rp[i], rp[i + 1]
+                bci.append(ci[s:e] + c * n)
+                bv.append(vals9[k, s:e])
+            brp.append(brp[-1] + sum(rp[i + 1] - rp[i] for _ in blocks))
+    return (np.asarray(brp, dtype=np.int32), np.concatenate(bci).astype(np.int32),
+            np.concatenate(bv))
+
+
+def right_part_mass(mesh, F):
+    """Element-order sums of F_j * mass element per inner row (the interior
+    branch of Baroclin's right_part_cb under generate_right_part)."""
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    b = np.zeros(mesh.n_inner)
+    lib().or_right_part_mass(C.byref(mesh.c), _pd(F), _pd(b))
+    return b
 
 
 def spmv(rp, ci, vals, x):

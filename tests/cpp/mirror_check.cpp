@@ -8,8 +8,10 @@
 // by add() vs the GPU-assembled one, same x), solve_x.bin (Solver::solve of
 // the Helmholtz system), lap_calc2.bin, lap_calc1.bin, lap_solve.bin
 // (SphereLaplace), jac_calc2.bin, jac_calc2t.bin (SphereJacobian),
-// bv_u1.bin (one BarVortex::calc step), inputs u.bin, v.bin, x.bin, psi.bin
-// and summary.txt (scalars).
+// bv_u1.bin (one BarVortex::calc step), bc_u11.bin / bc_u21.bin (one
+// Baroclin::calc step of (u, v) with the forcing written to bc_f.bin,
+// bc_g.bin and the Coriolis term bc_lh.bin), inputs u.bin, v.bin, x.bin,
+// psi.bin and summary.txt (scalars).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -32,6 +34,17 @@ static void dump(const std::string& dir, const char* name, const std::vector<dou
 }
 
 static std::vector<double> from_dev(const ArrayDevice<double>& a) { return a.host(); }
+
+// smooth forcing and Coriolis callbacks of the reference's signatures
+static double bc_f(double x, double y, double t, double, double, double, double, double)
+{
+    return 0.1 * std::sin(x) * std::cos(y + t);
+}
+static double bc_g(double x, double y, double t, double, double, double, double, double)
+{
+    return 0.05 * std::cos(x) * std::sin(2.0 * y - t);
+}
+static double bc_cor(double phi, double) { return 2.0 * std::sin(phi); }
 
 static ArrayDevice<double> to_dev(const std::vector<double>& h)
 {
@@ -146,6 +159,33 @@ int main(int argc, char** argv)
             threw = e.code() == PHG_ERR_ARG;
         }
 
+        // Baroclin: one step of (u, v) with the callbacks above at t = 0.25
+        std::vector<double> coords;
+        {
+            int VV = 0, TT = 0, ni = 0, no = 0, nz = 0;
+            check(phg_mesh_info(m.handle(), &VV, &TT, &ni, &no, &nz), "phg_mesh_info");
+            coords.resize((size_t)VV * nz * 2);
+            check(phg_mesh_arrays(m.handle(), nullptr, coords.data(), nullptr, nullptr, nullptr,
+                                  nullptr, nullptr, nullptr, nullptr, nullptr),
+                  "phg_mesh_arrays");
+            std::vector<double> fv((size_t)V), gv((size_t)V), lhv((size_t)V);
+            for (int i = 0; i < V; ++i) {
+                const double x = coords[(size_t)i * nz * 2], y = coords[(size_t)i * nz * 2 + 1];
+                fv[i] = bc_f(x, y, 0.25, 0, 0, 0, 0, 0);
+                gv[i] = bc_g(x, y, 0.25, 0, 0, 0, 0, 0);
+                lhv[i] = bc_cor(x, y);
+            }
+            dump(out, "bc_f.bin", fv);
+            dump(out, "bc_g.bin", gv);
+            dump(out, "bc_lh.bin", lhv);
+        }
+        Baroclin bc(m, bc_f, bc_g, bc_cor, 1e-3, 1.6e-2, 8e-2, 1.6e-2, 8e-2, 1.0);
+        ArrayDevice<double> b1((size_t)V), b2((size_t)V);
+        bc.calc(b1.data(), b2.data(), du.data(), dv.data(), nullptr, 0.25);
+        dump(out, "bc_u11.bin", from_dev(b1));
+        dump(out, "bc_u21.bin", from_dev(b2));
+        const phg_bc_info bci = bc.last_info();
+
         const std::string sp = out + "/summary.txt";
         FILE* f = std::fopen(sp.c_str(), "w");
         if (!f) return 2;
@@ -157,6 +197,7 @@ int main(int argc, char** argv)
         std::fprintf(f, "norm_u %.17g\nscalar_uv %.17g\ndist_uv %.17g\n", nu, suv, duv);
         std::fprintf(f, "bv_converged %d\nbv_iterations %d\n", bi.converged, bi.iterations);
         std::fprintf(f, "bv_bnd_throws %d\n", threw ? 1 : 0);
+        std::fprintf(f, "bc_converged %d\nbc_iterations %d\n", bci.converged, bci.iterations);
         std::fclose(f);
         std::printf("mirror_check level %d ok\n", level);
     } catch (const std::exception& e) {

@@ -1,0 +1,111 @@
+"""GPU Baroclin step (phg_baroclin_*) against the oracle restatement
+(oracle/baroclin.py: literal integrate_cb blocks, element-order right part,
+direct LU solve of the 4n x 4n system), through the C ABI.
+
+Bars: A_ (4n x 4n CSR) bit-identical to the oracle's; one step of two
+members (the reference driver's manufactured pair and a perturbed copy, with
+its forcing) within 1e-9 of the oracle step (SphereNorm distance relative to
+the state), with the block preconditioner and without it; a member's result
+bit-identical alone and beside another member; the manufactured-solution
+tendency error falls with refinement as the oracle's does."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytest.importorskip("scipy")
+
+import mke_amd as P
+from oracle import mesh_ref, oracle as O
+from oracle.baroclin import Baroclin as OB, manufactured
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+PRM = dict(sigma=1.6e-2, mu=8e-2, sigma1=1.6e-2, mu1=8e-2, alpha=1.0)
+TAU = 1e-3
+
+
+def _dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=DEV)
+
+
+def _setup(level):
+    text = mesh_ref.sphere_mesh_text(level)
+    a = mesh_ref.load_text(text)
+    om = O.Mesh(a)
+    xy = np.asarray(a["coords"]).reshape(len(om.p2io), -1, 2)[:, 0, :]
+    return P.Mesh.load(text), om, xy
+
+
+def test_baroclin_matrix_bitwise():
+    print("baroclin matrix", flush=True)
+    pm, om, xy = _setup(3)
+    bc = P.Baroclin(pm, _dev(np.zeros(len(xy))), tau=TAU, slots=1, **PRM)
+    rp, ci, v = bc.matrix().csr()
+    orp, oci, ov = O.baroclin_matrix(*O.baroclin_blocks(om, TAU, PRM["sigma"], PRM["mu"],
+                                                       PRM["sigma1"], PRM["mu1"], PRM["alpha"]))
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci) and np.array_equal(v, ov)
+
+
+def _states(xy, om):
+    u1, u2, f, g = manufactured(xy, 0.0, **PRM)
+    rng = np.random.default_rng(5)
+    bump = np.exp(-8.0 * ((xy[:, 0] - 0.3) ** 2 + (np.cos(xy[:, 1]) - 1.0) ** 2))
+    bump[om.outer] = 0.0
+    U1 = np.stack([u1, u1 + 0.05 * bump])
+    U2 = np.stack([u2, u2 - 0.03 * bump * rng.uniform(0.5, 1.0)])
+    return U1, U2, f, g
+
+
+@pytest.mark.parametrize("precondition", [1, 0])
+def test_baroclin_step_vs_oracle(precondition):
+    print("baroclin step precondition=%d" % precondition, flush=True)
+    pm, om, xy = _setup(3)
+    U1, U2, f, g = _states(xy, om)
+    bc = P.Baroclin(pm, _dev(np.zeros(len(xy))), tau=TAU, slots=2, **PRM)
+    D1, D2 = torch.zeros_like(_dev(U1)), torch.zeros_like(_dev(U2))
+    infos = bc.calc(D1, D2, _dev(U1), _dev(U2), _dev(f), _dev(g),
+                    opts=P.bc_opts(precondition=precondition, max_solver_iters=20000))
+    ob = OB(om, tau=TAU, lh=np.zeros(len(xy)), **PRM)
+    got1, got2 = D1.cpu().numpy(), D2.cpu().numpy()
+    for q in range(2):
+        r1, r2, it, nr = ob.calc(U1[q], U2[q], f, g)
+        assert infos[q]["converged"] and abs(infos[q]["iterations"] - it) <= 1, (infos[q], it)
+        assert ob.dist(got1[q], r1) <= 1e-9 * ob.bv.norm(r1), q
+        assert ob.dist(got2[q], r2) <= 1e-9 * ob.bv.norm(r2), q
+        assert ob.dist(got1[q], U1[q]) > 1e-4 * ob.bv.norm(U1[q])   # the step moved u1
+
+
+def test_baroclin_member_independent_of_batch():
+    print("baroclin independence", flush=True)
+    pm, om, xy = _setup(3)
+    U1, U2, f, g = _states(xy, om)
+    bc = P.Baroclin(pm, _dev(np.zeros(len(xy))), tau=TAU, slots=2, **PRM)
+    D1, D2 = torch.zeros_like(_dev(U1)), torch.zeros_like(_dev(U2))
+    bc.calc(D1, D2, _dev(U1), _dev(U2), _dev(f), _dev(g))
+    a1 = torch.zeros_like(D1[1:])
+    a2 = torch.zeros_like(D2[1:])
+    bc.calc(a1, a2, _dev(U1[1:]), _dev(U2[1:]), _dev(f), _dev(g))
+    # member 1 alone runs in slot 0 of the same 2-slot engine
+    assert torch.equal(a1[0], D1[1]) and torch.equal(a2[0], D2[1])
+
+
+def test_baroclin_manufactured_tendency_converges():
+    print("baroclin manufactured", flush=True)
+    errs = []
+    for level in (3, 4, 5):
+        pm, om, xy = _setup(level)
+        bc = P.Baroclin(pm, _dev(np.zeros(len(xy))), tau=TAU, slots=1, **PRM)
+        u1, u2, f, g = manufactured(xy, 0.0, **PRM)
+        r1, r2, _, _ = manufactured(xy, TAU, **PRM)
+        D1, D2 = torch.zeros_like(_dev(u1)), torch.zeros_like(_dev(u2))
+        info = bc.calc(D1, D2, _dev(u1), _dev(u2), _dev(f), _dev(g))[0]
+        assert info["converged"], info
+        N = O.Mesh(mesh_ref.load_text(mesh_ref.sphere_mesh_text(level))).assemble_full(O.KIND_MASS)
+        nrm = lambda d: float(np.sqrt(d @ O.spmv(*N, d)))  # noqa: E731
+        v1, v2 = D1.cpu().numpy(), D2.cpu().numpy()
+        errs.append((nrm((v1 - u1) - (r1 - u1)) / nrm(r1 - u1),
+                     nrm((v2 - u2) - (r2 - u2)) / nrm(r2 - u2)))
+    # the oracle gives 0.118 / 0.040 / 0.0072 (layer 1) and 0.190 / 0.079 /
+    # 0.037 (layer 2) at levels 3 / 4 / 5
+    assert errs[1][0] < errs[0][0] / 2.5 and errs[2][0] < errs[1][0] / 2.5, errs
+    assert errs[1][1] < errs[0][1] / 1.8 and errs[2][1] < errs[1][1] / 1.8, errs

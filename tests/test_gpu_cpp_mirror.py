@@ -141,3 +141,22 @@ def test_python_sphere_laplace_and_norm_equal_cpp(run):
     assert N.norm(du) == summ["norm_u"]
     assert N.scalar(du, dv) == summ["scalar_uv"]
     assert N.dist(du, dv) == summ["dist_uv"]
+
+
+def test_baroclin_equals_python(run):
+    """The C++ Baroclin (callbacks evaluated on the host at the zone-0 chart
+    coordinates) and the Python Baroclin given the same forcing vectors make
+    the same library call: same bits."""
+    summ, load = run
+    m = P.Mesh.sphere(LEVEL)
+    bc = P.Baroclin(m, torch.as_tensor(load("bc_lh.bin"), device=DEV), tau=1e-3, sigma=1.6e-2,
+                    mu=8e-2, sigma1=1.6e-2, mu1=8e-2, alpha=1.0)
+    u1 = torch.as_tensor(load("u.bin"), device=DEV).reshape(1, -1)
+    u2 = torch.as_tensor(load("v.bin"), device=DEV).reshape(1, -1)
+    o1, o2 = torch.zeros_like(u1), torch.zeros_like(u2)
+    info = bc.calc(o1, o2, u1, u2, torch.as_tensor(load("bc_f.bin"), device=DEV),
+                   torch.as_tensor(load("bc_g.bin"), device=DEV))[0]
+    assert summ["bc_converged"] == 1 and info["converged"]
+    assert info["iterations"] == summ["bc_iterations"]
+    assert np.array_equal(o1.cpu().numpy()[0], load("bc_u11.bin"))
+    assert np.array_equal(o2.cpu().numpy()[0], load("bc_u21.bin"))
