@@ -252,6 +252,52 @@ def l8_leg(steps, oracle_check, seed=20251203):
     return out
 
 
+def baroclin_leg(level=5, members=8, steps=3):
+    """Baroclin::calc on the device for `members` members at once: the
+    reference driver's parameters (tau 1e-3, sigma 1.6e-2, mu 8e-2, sigma1 =
+    sigma, mu1 = mu, alpha 1, no Coriolis) and its manufactured pair with its
+    forcing, each member perturbed by a different smooth bump; block-
+    preconditioned 4n x 4n solve.  Returns the variant dict."""
+    import numpy as np
+    import torch
+
+    import mke_amd as P
+    from oracle.baroclin import manufactured
+
+    prm = dict(sigma=1.6e-2, mu=8e-2, sigma1=1.6e-2, mu1=8e-2, alpha=1.0)
+    t = time.perf_counter()
+    mesh = P.Mesh.sphere(level)
+    a = mesh.arrays()
+    xy = a["coords"][:, 0, :]
+    u1, u2, f, g = manufactured(xy, 0.0, **prm)
+    U1 = np.stack([u1 * (1.0 + 0.01 * k) for k in range(members)])
+    U2 = np.stack([u2 * (1.0 - 0.01 * k) for k in range(members)])
+    dev = lambda x: torch.as_tensor(np.ascontiguousarray(x), device="cuda")  # noqa: E731
+    bc = P.Baroclin(mesh, dev(np.zeros(len(xy))), tau=1e-3, slots=members, **prm)
+    log("baroclin: L%d engine for %d members ready in %.1fs" % (level, members,
+                                                                time.perf_counter() - t))
+    D1, D2 = torch.zeros_like(dev(U1)), torch.zeros_like(dev(U2))
+    bc.calc(D1, D2, dev(U1), dev(U2), dev(f), dev(g))   # warm-up (graph capture)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        infos = bc.calc(D1, D2, dev(U1), dev(U2), dev(f), dev(g))
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    out = dict(member_steps_per_s=round(members / dt, 3), ms_per_call=round(dt * 1e3, 2),
+               level=level, members=members,
+               passes=[i["iterations"] for i in infos],
+               system_gmres_steps_per_pass=round(float(np.mean(
+                   [i["it_system"] / max(1, i["iterations"]) for i in infos])), 2),
+               jacobian_gmres_steps_per_pass=round(float(np.mean(
+                   [i["it_jacobian"] / max(1, i["iterations"]) for i in infos])), 2),
+               converged=all(i["converged"] for i in infos))
+    log("baroclin L%d: %.1f member-steps/s (%.1f ms per call of %d), passes %s" %
+        (level, out["member_steps_per_s"], out["ms_per_call"], members, out["passes"]))
+    bc.close()
+    return out
+
+
 def cpu_baseline(rp, ci, v, b, budget_s):
     """Oracle CSR GMRES(30)-MGS (OpenMP) on the host cores, same A and b."""
     from oracle import oracle as O
@@ -336,6 +382,17 @@ def main():
     log("fp64-mgs: %.2f solves/s, p50 %.2f ms, p99 %.2f ms, %d iters, rel %.2e" %
         (solves_s, p50, p99, iters, infos[-1]["rel_residual"]))
     x64 = x.cpu().numpy()
+
+    # ---- roofline of the FP64 SpMV (Solver::mult_vector), right after the
+    #      timed region: median of 5 runs of 400 back-to-back launches ----
+    y = torch.zeros_like(b)
+    reps = sorted(kernel_avg_ms(lambda: A.mult_vector(y, b, stream=stream), 400, stream)
+                  for _ in range(5))
+    spmv_ms = reps[2]
+    bspmv = spmv_bytes(n, nnz, 8, 8)
+    achieved = bspmv / (spmv_ms * 1e-3) / 1e9
+    log("spmv fp64: %.2f us per launch (%.0f GB/s; runs %s us)" %
+        (spmv_ms * 1e3, achieved, [round(r * 1e3, 2) for r in reps]))
 
     rp, ci, v = A.csr()
     bh = b.cpu().numpy()
@@ -453,16 +510,14 @@ def main():
     if not args.no_variants and not args.no_l8:
         variants.update(l8_leg(max(2, args.steps // 10), not args.no_cpu))
 
+    # ---- Baroclin: two-layer implicit step, 8 members at level 5 ----
+    if not args.no_variants:
+        variants["baroclin_L5_8members"] = baroclin_leg()
+
     # ---- serving: ensemble BarVortex steps on mixed levels ----
     if not args.no_variants and args.serve_requests > 0:
         variants.update(serve_leg(args.serve_requests, args.serve_rate))
 
-    # ---- roofline of the FP64 SpMV (Solver::mult_vector) ----
-    y = torch.zeros_like(b)
-    spmv_ms = kernel_avg_ms(lambda: A.mult_vector(y, b, stream=stream), 400, stream)
-    bspmv = spmv_bytes(n, nnz, 8, 8)
-    achieved = bspmv / (spmv_ms * 1e-3) / 1e9
-    log("spmv fp64: %.2f us per launch (%.0f GB/s)" % (spmv_ms * 1e3, achieved))
     cold_ms = cold_gbs = None
     if args.cold_spmv:
         # cold-cache variant: round-robin over 40 copies (> 512 MB working set)

@@ -67,6 +67,21 @@ void phg_free(void *p);
 int phg_mesh_load(const char *text, size_t len, phg_mesh_t *out);
 /* phg_mesh_sphere_text + phg_mesh_load without the caller seeing the text. */
 int phg_mesh_sphere(int level, phg_mesh_t *out);
+/* Binary image of a loaded mesh (all Mesh::load arrays, with a checksum):
+ * no reference counterpart; it replaces re-running mke_mesh_sphere and
+ * Mesh::load (utils/icosahedron.cpp:481-624, src/mesh.cpp:323-519) at
+ * start-up.  load_file returns PHG_ERR_FORMAT for a damaged or foreign file. */
+int phg_mesh_save(phg_mesh_t m, const char *path);
+int phg_mesh_load_file(const char *path, phg_mesh_t *out);
+/* The level's sphere from dir/sphere_L<level>.phgmesh, generated and written
+ * there when absent or unreadable. */
+int phg_mesh_sphere_cached(int level, const char *dir, phg_mesh_t *out);
+/* print_function (src/phelm.cpp:248-309) of host values h_u (n_points):
+ * x y z f u v per point (%.16f; u, v the zone-0 chart coordinates; x, y, z
+ * = u, v, 0, or with sphere_xyz the unit-sphere map cos u cos v, cos u sin v,
+ * sin u), then the 1-based triangles.  Free *text with phg_free(). */
+int phg_print_function(phg_mesh_t m, const double *h_u, int sphere_xyz, char **text,
+                       size_t *len);
 /* Sizes: m.ps.size(), m.tr.size(), inner_size, outer_size, zone count. */
 int phg_mesh_info(phg_mesh_t m, int *n_points, int *n_triangles,
                   int *n_inner, int *n_outer, int *n_zones);

@@ -124,6 +124,70 @@ int phg_mesh_sphere(int level, phg_mesh_t* out)
     return phg_mesh_load(s.data(), s.size(), out);
 }
 
+int phg_mesh_save(phg_mesh_t m, const char* path)
+{
+    if (!m || !path) {
+        set_error("phg_mesh_save: null argument");
+        return PHG_ERR_ARG;
+    }
+    std::string err;
+    if (!save_mesh_binary(m->h, path, err)) {
+        set_error(err);
+        return PHG_ERR_ARG;
+    }
+    return PHG_OK;
+}
+
+int phg_mesh_load_file(const char* path, phg_mesh_t* out)
+{
+    if (!path || !out) {
+        set_error("phg_mesh_load_file: null argument");
+        return PHG_ERR_ARG;
+    }
+    std::unique_ptr<phg_mesh_s> m(new phg_mesh_s());
+    std::string err;
+    if (!load_mesh_binary(path, m->h, err)) {
+        set_error(err);
+        return PHG_ERR_FORMAT;
+    }
+    *out = m.release();
+    return PHG_OK;
+}
+
+int phg_mesh_sphere_cached(int level, const char* dir, phg_mesh_t* out)
+{
+    if (level < 0 || level > 10 || !dir || !out) {
+        set_error("phg_mesh_sphere_cached: bad argument (level 0..10, dir, out)");
+        return PHG_ERR_ARG;
+    }
+    const std::string path = std::string(dir) + "/sphere_L" + std::to_string(level) + ".phgmesh";
+    if (phg_mesh_load_file(path.c_str(), out) == PHG_OK) return PHG_OK;
+    int rc = phg_mesh_sphere(level, out);
+    if (rc) return rc;
+    std::string err;
+    (void)save_mesh_binary((*out)->h, path.c_str(), err);  // a cache: failure to write is not an error
+    return PHG_OK;
+}
+
+int phg_print_function(phg_mesh_t m, const double* h_u, int sphere_xyz, char** text, size_t* len)
+{
+    if (!m || !h_u || !text || !len) {
+        set_error("phg_print_function: null argument");
+        return PHG_ERR_ARG;
+    }
+    const std::string s = print_function_text(m->h, h_u, sphere_xyz != 0);
+    char* buf = (char*)std::malloc(s.size() + 1);
+    if (!buf) {
+        set_error("phg_print_function: out of memory");
+        return PHG_ERR_NOMEM;
+    }
+    std::memcpy(buf, s.data(), s.size());
+    buf[s.size()] = 0;
+    *text = buf;
+    *len = s.size();
+    return PHG_OK;
+}
+
 int phg_mesh_info(phg_mesh_t m, int* n_points, int* n_triangles, int* n_inner, int* n_outer,
                   int* n_zones)
 {

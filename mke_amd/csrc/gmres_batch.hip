@@ -135,6 +135,14 @@ struct BRow<double, double> {
                                                  const double* __restrict__ X, int KP, int q,
                                                  double s)
     {
+        if (width > ELL_MAXW) {  // long rows (block systems): the same sum, not unrolled
+            double r = 0.0;
+            for (int k = 0; k < width; ++k) {
+                const size_t o = (size_t)k * n + row;
+                r += val[o] * (X[(size_t)col[o] * KP + q] * s);
+            }
+            return r;
+        }
         int c[ELL_MAXW];
         double v[ELL_MAXW], xv[ELL_MAXW];
 #pragma unroll
@@ -160,6 +168,14 @@ struct BRow<float, MT> {
                                                 const float* __restrict__ X, int KP, int q,
                                                 float s)
     {
+        if (width > ELL_MAXW) {
+            float r = 0.0f;
+            for (int k = 0; k < width; ++k) {
+                const size_t o = (size_t)k * n + row;
+                r = fmaf(ell_val_f32(val[o]), X[(size_t)col[o] * KP + q] * s, r);
+            }
+            return r;
+        }
         int c[ELL_MAXW];
         MT v[ELL_MAXW];
         float xv[ELL_MAXW];
@@ -560,20 +576,28 @@ bk_residual(int n, int KP, int K, int width, const int* __restrict__ col,
         const int row = base + t * RB;
         w[t] = VT(0);
         if (act && row < n) {
-            int c[ELL_MAXW];
-            double v[ELL_MAXW], xv[ELL_MAXW];
-#pragma unroll
-            for (int k = 0; k < ELL_MAXW; ++k) {
-                const size_t o = (size_t)k * n + row;
-                c[k] = k < width ? col[o] : 0;
-                v[k] = k < width ? val[o] : 0.0;
-            }
-#pragma unroll
-            for (int k = 0; k < ELL_MAXW; ++k) xv[k] = k < width ? X[(size_t)c[k] * K + q] : 0.0;
             double s = 0.0;
+            if (width > ELL_MAXW) {  // long rows: the same sum, not unrolled
+                for (int k = 0; k < width; ++k) {
+                    const size_t o = (size_t)k * n + row;
+                    s += val[o] * X[(size_t)col[o] * K + q];
+                }
+            } else {
+                int c[ELL_MAXW];
+                double v[ELL_MAXW], xv[ELL_MAXW];
 #pragma unroll
-            for (int k = 0; k < ELL_MAXW; ++k)
-                if (k < width) s += v[k] * xv[k];
+                for (int k = 0; k < ELL_MAXW; ++k) {
+                    const size_t o = (size_t)k * n + row;
+                    c[k] = k < width ? col[o] : 0;
+                    v[k] = k < width ? val[o] : 0.0;
+                }
+#pragma unroll
+                for (int k = 0; k < ELL_MAXW; ++k)
+                    xv[k] = k < width ? X[(size_t)c[k] * K + q] : 0.0;
+#pragma unroll
+                for (int k = 0; k < ELL_MAXW; ++k)
+                    if (k < width) s += v[k] * xv[k];
+            }
             const double r = B[(size_t)row * K + q] - s;
             w[t] = (VT)r;
             W0[(size_t)row * KP + q] = w[t];
@@ -981,10 +1005,6 @@ int gmres_batch_solve_pc(phg_matrix_s* A, int K, const double* B, double* X,
     }
     int rc = ensure_ell64(A, s);
     if (rc) return rc;
-    if (A->ell64.width > ELL_MAXW) {
-        set_error("gmres_batch: rows longer than 8 entries are not supported");
-        return PHG_ERR_ARG;
-    }
     const int kp = next_pow2(K);
     if (pc) {
         if (o.precision != PHG_F64 || pc->rows() != A->n_rows) {

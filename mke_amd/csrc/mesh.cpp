@@ -10,6 +10,7 @@
 // order; this file must be built with -ffp-contract=off.
 //
 // load_mesh_text() follows Mesh::load (src/mesh.cpp:323-519) for P1 meshes.
+#include <cstdint>
 #include "mesh.h"
 
 #include <algorithm>
@@ -417,6 +418,160 @@ bool load_mesh_text(const char* text, size_t len, SphereMesh& m, std::string& er
             return false;
         }
     return true;
+}
+
+// ---------------------------------------------------------------- binary --
+namespace {
+
+const char MESH_MAGIC[8] = {'P', 'H', 'G', 'M', 'E', 'S', 'H', '1'};
+
+uint64_t fnv1a(const unsigned char* p, size_t n, uint64_t h)
+{
+    for (size_t i = 0; i < n; ++i) {
+        h ^= p[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+template <typename T>
+void put(std::string& out, const std::vector<T>& v)
+{
+    out.append(reinterpret_cast<const char*>(v.data()), v.size() * sizeof(T));
+}
+
+template <typename T>
+bool take(const std::string& in, size_t& at, std::vector<T>& v, size_t count)
+{
+    const size_t bytes = count * sizeof(T);
+    if (in.size() - at < bytes) return false;
+    v.resize(count);
+    if (bytes) std::memcpy(v.data(), in.data() + at, bytes);
+    at += bytes;
+    return true;
+}
+
+}  // namespace
+
+bool save_mesh_binary(const SphereMesh& m, const char* path, std::string& err)
+{
+    std::string body;
+    const int32_t head[5] = {m.n_points, m.n_triangles, m.n_zones, (int32_t)m.inner.size(),
+                             (int32_t)m.outer.size()};
+    body.append(reinterpret_cast<const char*>(head), sizeof head);
+    put(body, m.pr);
+    put(body, m.coords);
+    put(body, m.tri);
+    put(body, m.tri_zone);
+    put(body, m.adj_ptr);
+    put(body, m.adj);
+    put(body, m.is_boundary);
+    put(body, m.inner);
+    put(body, m.outer);
+    put(body, m.p2io);
+    const uint64_t sum = fnv1a(reinterpret_cast<const unsigned char*>(body.data()), body.size(),
+                               1469598103934665603ull);
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) {
+        err = "cannot write " + tmp;
+        return false;
+    }
+    bool ok = std::fwrite(MESH_MAGIC, 1, 8, f) == 8 &&
+              std::fwrite(&sum, sizeof sum, 1, f) == 1 &&
+              std::fwrite(body.data(), 1, body.size(), f) == body.size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), path) != 0) {
+        std::remove(tmp.c_str());
+        err = std::string("cannot write ") + path;
+        return false;
+    }
+    return true;
+}
+
+bool load_mesh_binary(const char* path, SphereMesh& m, std::string& err)
+{
+    FILE* f = std::fopen(path, "rb");
+    if (!f) {
+        err = std::string("cannot open ") + path;
+        return false;
+    }
+    std::string in;
+    char buf[1 << 16];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof buf, f)) > 0) in.append(buf, got);
+    std::fclose(f);
+    err = std::string("bad mesh image ") + path;
+    if (in.size() < 8 + 8 + 20 || std::memcmp(in.data(), MESH_MAGIC, 8) != 0) return false;
+    uint64_t sum = 0;
+    std::memcpy(&sum, in.data() + 8, sizeof sum);
+    if (fnv1a(reinterpret_cast<const unsigned char*>(in.data()) + 16, in.size() - 16,
+              1469598103934665603ull) != sum)
+        return false;
+    int32_t head[5];
+    std::memcpy(head, in.data() + 16, sizeof head);
+    const long V = head[0], T = head[1], nz = head[2], ni = head[3], no = head[4];
+    if (V < 0 || T < 0 || nz < 1 || ni < 0 || no < 0 || ni + no != V) return false;
+    SphereMesh r;
+    r.n_points = (int)V;
+    r.n_triangles = (int)T;
+    r.n_zones = (int)nz;
+    size_t at = 16 + sizeof head;
+    if (!take(in, at, r.pr, (size_t)V * 3) || !take(in, at, r.coords, (size_t)V * nz * 2) ||
+        !take(in, at, r.tri, (size_t)T * 3) || !take(in, at, r.tri_zone, (size_t)T) ||
+        !take(in, at, r.adj_ptr, (size_t)V + 1) ||
+        !take(in, at, r.adj, (size_t)(V + 1 > 0 && !r.adj_ptr.empty() ? r.adj_ptr[V] : 0)) ||
+        !take(in, at, r.is_boundary, (size_t)V) || !take(in, at, r.inner, (size_t)ni) ||
+        !take(in, at, r.outer, (size_t)no) || !take(in, at, r.p2io, (size_t)V) || at != in.size())
+        return false;
+    // structural checks: indices in range, adjacency consistent
+    if (r.adj_ptr[0] != 0 || r.adj_ptr[V] != 3 * T) return false;
+    for (int v : r.tri)
+        if (v < 0 || v >= V) return false;
+    for (long i = 0; i < V; ++i)
+        if (r.adj_ptr[i + 1] < r.adj_ptr[i]) return false;
+    for (int t : r.adj)
+        if (t < 0 || t >= T) return false;
+    for (int t : r.tri_zone)
+        if (t < 0 || t >= nz) return false;
+    for (int p : r.inner)
+        if (p < 0 || p >= V || r.is_boundary[p]) return false;
+    for (int p : r.outer)
+        if (p < 0 || p >= V || !r.is_boundary[p]) return false;
+    for (long i = 0; i < V; ++i) {
+        const int q = r.p2io[i];
+        if (q < 0 || q >= (r.is_boundary[i] ? no : ni)) return false;
+    }
+    m = std::move(r);
+    err.clear();
+    return true;
+}
+
+std::string print_function_text(const SphereMesh& m, const double* u, bool sphere_xyz)
+{
+    std::string out;
+    char line[256];
+    std::snprintf(line, sizeof line, "# points %d\n", m.n_points);
+    out += line;
+    const int nz = m.n_zones;
+    for (int i = 0; i < m.n_points; ++i) {
+        const double a = m.coords[(size_t)i * nz * 2], b = m.coords[(size_t)i * nz * 2 + 1];
+        const double x = sphere_xyz ? std::cos(a) * std::cos(b) : a;
+        const double y = sphere_xyz ? std::cos(a) * std::sin(b) : b;
+        const double z = sphere_xyz ? std::sin(a) : 0.0;
+        std::snprintf(line, sizeof line, "%.16f %.16f %.16f %.16f %.16f %.16f \n", x, y, z, u[i],
+                      a, b);
+        out += line;
+    }
+    std::snprintf(line, sizeof line, "# triangles %d\n", m.n_triangles);
+    out += line;
+    for (int t = 0; t < m.n_triangles; ++t) {
+        std::snprintf(line, sizeof line, "%d %d %d\n", m.tri[3 * t] + 1, m.tri[3 * t + 1] + 1,
+                      m.tri[3 * t + 2] + 1);
+        out += line;
+    }
+    out += "# end \n";
+    return out;
 }
 
 }  // namespace phg

@@ -27,4 +27,15 @@ std::string sphere_mesh_text(int level);
 // ("bad file format, line = N") on malformed input, like the reference.
 bool load_mesh_text(const char* text, size_t len, SphereMesh& m, std::string& err);
 
+// Binary image of a loaded mesh (every array above, little-endian, with a
+// checksum): written once, read back in a fraction of the text path's time.
+bool save_mesh_binary(const SphereMesh& m, const char* path, std::string& err);
+bool load_mesh_binary(const char* path, SphereMesh& m, std::string& err);
+
+// print_function (src/phelm.cpp:248-309): "# points", one line per point
+// (x y z f u v, %.16f) with u, v the zone-0 chart coordinates and x, y, z
+// either u, v, 0 (no coordinate maps) or the unit-sphere map cos u cos v,
+// cos u sin v, sin u (sphere_xyz); "# triangles" (1-based); "# end".
+std::string print_function_text(const SphereMesh& m, const double* u, bool sphere_xyz);
+
 }  // namespace phg

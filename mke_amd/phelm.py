@@ -81,6 +81,10 @@ SIGNATURES = {
     "phg_gmres_batch_tune": (_I, [_P, _I, C.POINTER(GmresOpts), _P, _PI, _PD]),
     "phg_gmres_batch_shape": (_I, [_I, _I, _I]),
     "phg_gmres_batch_set_shape": (_I, [_I, _I, _I, _I]),
+    "phg_mesh_save": (_I, [_P, C.c_char_p]),
+    "phg_mesh_load_file": (_I, [C.c_char_p, C.POINTER(_P)]),
+    "phg_mesh_sphere_cached": (_I, [_I, C.c_char_p, C.POINTER(_P)]),
+    "phg_print_function": (_I, [_P, _PD, _I, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]),
     "phg_mg_create": (_I, [_P, _I, _D, _D, _P, C.POINTER(_P)]),
     "phg_mg_destroy": (None, [_P]),
     "phg_mg_matrix": (_I, [_P, C.POINTER(_P)]),
@@ -214,6 +218,39 @@ class Mesh:
         h = C.c_void_p()
         check(lib().phg_mesh_sphere(level, C.byref(h)), "phg_mesh_sphere")
         return cls(h)
+
+    @classmethod
+    def load_file(cls, path):
+        """A mesh from its binary image (save)."""
+        h = C.c_void_p()
+        check(lib().phg_mesh_load_file(os.fsencode(path), C.byref(h)), "phg_mesh_load_file")
+        return cls(h)
+
+    @classmethod
+    def sphere_cached(cls, level, directory):
+        """The level's sphere from directory/sphere_L<level>.phgmesh, written
+        there on first use."""
+        h = C.c_void_p()
+        check(lib().phg_mesh_sphere_cached(level, os.fsencode(directory), C.byref(h)),
+              "phg_mesh_sphere_cached")
+        return cls(h)
+
+    def save(self, path):
+        """Writes the binary image of the mesh."""
+        check(lib().phg_mesh_save(self.h, os.fsencode(path)), "phg_mesh_save")
+
+    def print_function(self, u, sphere_xyz=False):
+        """print_function (src/phelm.cpp:248-309) text of the host values u."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if u.shape != (self.n_points,):
+            raise PhelmError("print_function: u must have n_points values")
+        buf, n = C.c_void_p(), C.c_size_t()
+        check(lib().phg_print_function(self.h, _pd(u), int(sphere_xyz), C.byref(buf),
+                                       C.byref(n)), "phg_print_function")
+        try:
+            return C.string_at(buf, n.value).decode()
+        finally:
+            lib().phg_free(buf)
 
     def arrays(self):
         V, T, nz = self.n_points, self.n_triangles, self.n_zones
