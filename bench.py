@@ -252,6 +252,40 @@ def l8_leg(steps, oracle_check, seed=20251203):
     return out
 
 
+def barvortex_leg(level, members=8, steps=2):
+    """One standalone BarVortex engine (no serving threads): `members` RH
+    wave-4 members with seeded noise stepped together `steps` times (tau =
+    300 s, the serving workload's level-8 sub-step).  For profiling a pass."""
+    import numpy as np
+    import torch
+
+    import mke_amd as P
+    from tests._common import rh_state
+
+    mesh = P.Mesh.sphere(level)
+    a = mesh.arrays()
+    pr = a["pr"]
+    lh = torch.as_tensor(2.0 * np.clip(pr[:, 2], -1, 1), device="cuda")
+    U = torch.as_tensor(np.stack([rh_state(pr, seed=100 + k) for k in range(members)]),
+                        device="cuda")
+    bv = P.BarVortex(mesh, lh, tau=300.0 * 7.292e-5, sigma=1.6e-2, mu=8e-5, k1=1.0, k2=1.0,
+                     theta=0.5, slots=members)
+    U1 = torch.zeros_like(U)
+    bv.calc(U1, U)   # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        infos = bv.calc(U1, U)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    passes = [i["iterations"] for i in infos]
+    log("barvortex L%d: %.1f ms per step of %d members, passes %s, gmres per pass %s" %
+        (level, dt * 1e3, members, passes,
+         {k: round(float(np.mean([i[k] / max(1, i["iterations"]) for i in infos])), 1)
+          for k in ("it_jacobian", "it_helmholtz", "it_poisson")}))
+    return dict(ms_per_step=round(dt * 1e3, 2), level=level, members=members, passes=passes)
+
+
 def baroclin_leg(level=5, members=8, steps=3):
     """Baroclin::calc on the device for `members` members at once: the
     reference driver's parameters (tau 1e-3, sigma 1.6e-2, mu 8e-2, sigma1 =
@@ -332,6 +366,8 @@ def main():
                     help="skip the level-8 mixed-precision leg")
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the ensemble serving leg and print its results (profiling)")
+    ap.add_argument("--barvortex-only", type=int, default=0, metavar="LEVEL",
+                    help="run only a standalone 8-member BarVortex step at LEVEL (profiling)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 256 MiB device copy (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -349,6 +385,9 @@ def main():
     torch.cuda.set_device(0)
     if args.serve_only:
         print(json.dumps({"serve_only": serve_leg(args.serve_requests, args.serve_rate)}))
+        return
+    if args.barvortex_only:
+        print(json.dumps({"barvortex_only": barvortex_leg(args.barvortex_only)}))
         return
     stream = torch.cuda.current_stream()
     dev = "cuda:0"
@@ -479,17 +518,20 @@ def main():
     #      Helmholtz operator, bit-identical fine matrix), 1 and 64 right-hand sides ----
     if not args.no_variants:
         mg = P.Multigrid(mesh, P.HELMHOLTZ, cm, cl, stream=stream)
-        for name, k in (("fp64_mg_gmres_cgs2", 1), ("batched64_f64_mg_gmres_cgs2", 64)):
+        for name, k, prec in (("fp64_mg_gmres_cgs2", 1, P.F64),
+                              ("mixed_f32_mg_gmres_cgs2", 1, P.F32),
+                              ("batched64_f64_mg_gmres_cgs2", 64, P.F64),
+                              ("batched64_f32_mg_gmres_cgs2", 64, P.F32)):
             Bk = b.reshape(-1, 1).clone() if k == 1 else B64
             Xk = torch.zeros_like(Bk)
-            mg.solve_batch(Xk, Bk, rtol=1e-10, stream=stream)  # warmup (graph capture)
+            mg.solve_batch(Xk, Bk, rtol=1e-10, precision=prec, stream=stream)  # warmup
             torch.cuda.synchronize()
             reps = max(3, args.steps // (1 if k == 1 else 5))
             lat_k = []
             t0 = time.perf_counter()
             for _ in range(reps):
                 t1 = time.perf_counter()
-                minfo = mg.solve_batch(Xk, Bk, rtol=1e-10, stream=stream)
+                minfo = mg.solve_batch(Xk, Bk, rtol=1e-10, precision=prec, stream=stream)
                 torch.cuda.synchronize()
                 lat_k.append((time.perf_counter() - t1) * 1e3)
             wk = time.perf_counter() - t0

@@ -272,8 +272,10 @@ int phg_mg_matrix(phg_mg_t M, phg_matrix_t *out);
 int phg_mg_levels(phg_mg_t M);
 /* phg_gmres_batch with M's fine operator, right-preconditioned by one
  * V-cycle per Arnoldi step (flexible form: the preconditioned vectors are
- * kept for the update).  opts->precision must be PHG_F64.  Calls sharing
- * one M must not overlap in time (its level vectors are shared). */
+ * kept for the update).  opts->precision PHG_F64, or PHG_F32 / PHG_BF16:
+ * Krylov vectors and V-cycle in FP32 (level operators FP32) under FP64
+ * refinement.  Calls sharing one M must not overlap in time (its level
+ * vectors are shared). */
 int phg_gmres_batch_mg(phg_mg_t M, int nrhs, const double *d_B, double *d_X,
                        const phg_gmres_opts *opts, phg_gmres_info *infos,
                        phg_stream_t s);

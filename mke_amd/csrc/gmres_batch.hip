@@ -699,8 +699,9 @@ void precond_apply<double>(BatchPrecond* pc, int KP, const double* v, double* z,
     pc->apply(KP, v, z, s);
 }
 template <>
-void precond_apply<float>(BatchPrecond*, int, const float*, float*, hipStream_t)
+void precond_apply<float>(BatchPrecond* pc, int KP, const float* v, float* z, hipStream_t s)
 {
+    pc->apply_f32(KP, v, z, s);
 }
 
 // The fused first-pass kernel (default on; PHG_BATCH_FUSED=0 selects the
@@ -1007,11 +1008,11 @@ int gmres_batch_solve_pc(phg_matrix_s* A, int K, const double* B, double* X,
     if (rc) return rc;
     const int kp = next_pow2(K);
     if (pc) {
-        if (o.precision != PHG_F64 || pc->rows() != A->n_rows) {
-            set_error("gmres_batch: a preconditioner needs PHG_F64 and a matrix of its size");
+        if ((o.precision != PHG_F64 && !pc->f32()) || pc->rows() != A->n_rows) {
+            set_error("gmres_batch: the preconditioner does not fit the matrix or the precision");
             return PHG_ERR_ARG;
         }
-        rc = pc->reserve(kp);
+        rc = o.precision == PHG_F64 ? pc->reserve(kp) : pc->reserve_f32(kp);
         if (rc) return rc;
     }
     BatchWork* w = nullptr;
@@ -1024,11 +1025,11 @@ int gmres_batch_solve_pc(phg_matrix_s* A, int K, const double* B, double* X,
     if (o.precision == PHG_F32) {
         rc = ensure_ell32(A, s);
         if (rc) return rc;
-        return batch_run<float, float>(A, A->ell32, w, K, B, X, o, infos, nullptr, s);
+        return batch_run<float, float>(A, A->ell32, w, K, B, X, o, infos, pc, s);
     }
     rc = ensure_ell16(A, s);
     if (rc) return rc;
-    return batch_run<float, uint16_t>(A, A->ell16, w, K, B, X, o, infos, nullptr, s);
+    return batch_run<float, uint16_t>(A, A->ell16, w, K, B, X, o, infos, pc, s);
 }
 
 int batch_shape_get(int n, int K, int prec)

@@ -47,6 +47,8 @@ struct MgLevel {
     DevBuf<int> rrp, ridx;        // restriction: coarse row j <- fine rows
     DevBuf<double> rw;
     DevBuf<double> x0, x1, b, r;  // n x kp work vectors
+    // the same in FP32 for the mixed-precision form (made on first use)
+    DevBuf<float> dinvf, rwf, x0f, x1f, bf, rf;
     ~MgLevel()
     {
         delete A;
@@ -62,6 +64,12 @@ struct MgPrecond : BatchPrecond {
     int rows() const override { return mg_rows(M); }
     int reserve(int kp) override { return mg_reserve(M, kp); }
     void apply(int KP, const double* v, double* z, hipStream_t s) override { mg_apply(M, KP, v, z, s); }
+    bool f32() const override { return true; }
+    int reserve_f32(int kp) override { return mg_reserve_f32(M, kp); }
+    void apply_f32(int KP, const float* v, float* z, hipStream_t s) override
+    {
+        mg_apply_f32(M, KP, v, z, s);
+    }
 };
 }  // namespace phg
 
@@ -70,32 +78,36 @@ struct phg_mg_s {
     double c_mass = 1.0, c_lap = 0.0;
     std::vector<std::unique_ptr<MgLevel>> lv;  // lv[0] = finest
     DevBuf<double> cinv;                       // coarsest operator's inverse
-    int nc = 0;
+    DevBuf<float> cinvf;                       // the same in FP32
+    int nc = 0, kpf = 0;
     phg::MgPrecond pc;
 };
 
 namespace {
 
+// The V-cycle kernels in the vector type T (double, or float for the
+// mixed-precision form) over matrix values MT.
+template <typename T, typename MT>
 __global__ void mg_jacobi(int n, int KP, int width, const int* __restrict__ col,
-                          const double* __restrict__ val, const double* __restrict__ dinv,
-                          const double* __restrict__ xin, const double* __restrict__ b,
-                          double* __restrict__ xout, double w)
+                          const MT* __restrict__ val, const T* __restrict__ dinv,
+                          const T* __restrict__ xin, const T* __restrict__ b, T* __restrict__ xout,
+                          T w)
 {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)n * KP) return;
     const int row = (int)(e / KP), q = (int)(e % KP);
-    double s = 0.0, x = 0.0;
+    T s = T(0), x = T(0);
     if (xin) {
         int c[ELL_MAXW];
-        double v[ELL_MAXW], xv[ELL_MAXW];
+        T v[ELL_MAXW], xv[ELL_MAXW];
 #pragma unroll
         for (int k = 0; k < ELL_MAXW; ++k) {
             const size_t o = (size_t)k * n + row;
             c[k] = k < width ? col[o] : 0;
-            v[k] = k < width ? val[o] : 0.0;
+            v[k] = k < width ? (T)val[o] : T(0);
         }
 #pragma unroll
-        for (int k = 0; k < ELL_MAXW; ++k) xv[k] = k < width ? xin[(size_t)c[k] * KP + q] : 0.0;
+        for (int k = 0; k < ELL_MAXW; ++k) xv[k] = k < width ? xin[(size_t)c[k] * KP + q] : T(0);
 #pragma unroll
         for (int k = 0; k < ELL_MAXW; ++k)
             if (k < width) s += v[k] * xv[k];
@@ -104,73 +116,82 @@ __global__ void mg_jacobi(int n, int KP, int width, const int* __restrict__ col,
     xout[e] = x + w * dinv[row] * (b[e] - s);
 }
 
+template <typename T, typename MT>
 __global__ void mg_residual(int n, int KP, int width, const int* __restrict__ col,
-                            const double* __restrict__ val, const double* __restrict__ x,
-                            const double* __restrict__ b, double* __restrict__ r)
+                            const MT* __restrict__ val, const T* __restrict__ x,
+                            const T* __restrict__ b, T* __restrict__ r)
 {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)n * KP) return;
     const int row = (int)(e / KP), q = (int)(e % KP);
     int c[ELL_MAXW];
-    double v[ELL_MAXW], xv[ELL_MAXW];
+    T v[ELL_MAXW], xv[ELL_MAXW];
 #pragma unroll
     for (int k = 0; k < ELL_MAXW; ++k) {
         const size_t o = (size_t)k * n + row;
         c[k] = k < width ? col[o] : 0;
-        v[k] = k < width ? val[o] : 0.0;
+        v[k] = k < width ? (T)val[o] : T(0);
     }
 #pragma unroll
-    for (int k = 0; k < ELL_MAXW; ++k) xv[k] = k < width ? x[(size_t)c[k] * KP + q] : 0.0;
-    double s = 0.0;
+    for (int k = 0; k < ELL_MAXW; ++k) xv[k] = k < width ? x[(size_t)c[k] * KP + q] : T(0);
+    T s = T(0);
 #pragma unroll
     for (int k = 0; k < ELL_MAXW; ++k)
         if (k < width) s += v[k] * xv[k];
     r[e] = b[e] - s;
 }
 
+template <typename T>
 __global__ void mg_restrict(int nc, int KP, const int* __restrict__ rrp,
-                            const int* __restrict__ ridx, const double* __restrict__ rw,
-                            const double* __restrict__ rf, double* __restrict__ bc)
+                            const int* __restrict__ ridx, const T* __restrict__ rw,
+                            const T* __restrict__ rf, T* __restrict__ bc)
 {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)nc * KP) return;
     const int j = (int)(e / KP), q = (int)(e % KP);
-    double s = 0.0;
+    T s = T(0);
     for (int t = rrp[j]; t < rrp[j + 1]; ++t) s += rw[t] * rf[(size_t)ridx[t] * KP + q];
     bc[e] = s;
 }
 
+template <typename T>
 __global__ void mg_prolong_add(int nf, int KP, const int* __restrict__ p0,
-                               const int* __restrict__ p1, const double* __restrict__ xc,
-                               double* __restrict__ xf)
+                               const int* __restrict__ p1, const T* __restrict__ xc,
+                               T* __restrict__ xf)
 {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)nf * KP) return;
     const int i = (int)(e / KP), q = (int)(e % KP);
     const int a = p0[i], c = p1[i];
-    double v;
-    if (c == -2) v = a >= 0 ? xc[(size_t)a * KP + q] : 0.0;  // the vertex itself
-    else v = 0.5 * ((a >= 0 ? xc[(size_t)a * KP + q] : 0.0) + (c >= 0 ? xc[(size_t)c * KP + q] : 0.0));
+    T v;
+    if (c == -2) v = a >= 0 ? xc[(size_t)a * KP + q] : T(0);  // the vertex itself
+    else v = T(0.5) * ((a >= 0 ? xc[(size_t)a * KP + q] : T(0)) + (c >= 0 ? xc[(size_t)c * KP + q] : T(0)));
     xf[e] += v;
+}
+
+__global__ void mg_to_f32(size_t n, const double* __restrict__ a, float* __restrict__ b)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
 }
 
 // x = inv b on the coarsest level: one workgroup per row i; thread t takes
 // column q = t % KP and the j = t / KP, t / KP + 256 / KP, ... terms, and the
 // 256 / KP partial sums of a column are added in a fixed order (a column's
 // sum does not involve the other columns).
+template <typename T>
 __global__ void __launch_bounds__(256)
-mg_dense(int nc, int KP, const double* __restrict__ inv, const double* __restrict__ b,
-         double* __restrict__ x)
+mg_dense(int nc, int KP, const T* __restrict__ inv, const T* __restrict__ b, T* __restrict__ x)
 {
-    __shared__ double part[256];
+    __shared__ T part[256];
     const int i = blockIdx.x, t = threadIdx.x;
     const int q = t % KP, g = t / KP, ng = 256 / KP;
-    double s = 0.0;
+    T s = T(0);
     for (int j = g; j < nc; j += ng) s += inv[(size_t)i * nc + j] * b[(size_t)j * KP + q];
     part[t] = s;
     __syncthreads();
     if (t < KP) {
-        double a = 0.0;
+        T a = T(0);
         for (int k = 0; k < ng; ++k) a += part[k * KP + t];
         x[(size_t)i * KP + t] = a;
     }
@@ -327,47 +348,128 @@ int mg_reserve(phg_mg_s* M, int kp)
     return PHG_OK;
 }
 
+// Level buffers of one precision.
+template <typename T>
+struct MgT;
+template <>
+struct MgT<double> {
+    using MT = double;
+    static const Ell<double>& E(MgLevel& l) { return l.A->ell64; }
+    static double* dinv(MgLevel& l) { return l.dinv.p; }
+    static double* rw(MgLevel& l) { return l.rw.p; }
+    static double* x0(MgLevel& l) { return l.x0.p; }
+    static double* x1(MgLevel& l) { return l.x1.p; }
+    static double* b(MgLevel& l) { return l.b.p; }
+    static double* r(MgLevel& l) { return l.r.p; }
+    static double* cinv(phg_mg_s* M) { return M->cinv.p; }
+};
+template <>
+struct MgT<float> {
+    using MT = float;
+    static const Ell<float>& E(MgLevel& l) { return l.A->ell32; }
+    static float* dinv(MgLevel& l) { return l.dinvf.p; }
+    static float* rw(MgLevel& l) { return l.rwf.p; }
+    static float* x0(MgLevel& l) { return l.x0f.p; }
+    static float* x1(MgLevel& l) { return l.x1f.p; }
+    static float* b(MgLevel& l) { return l.bf.p; }
+    static float* r(MgLevel& l) { return l.rf.p; }
+    static float* cinv(phg_mg_s* M) { return M->cinvf.p; }
+};
+
 // z = V-cycle(v): v, z are n x KP (finest level); enqueued on s.
-void mg_apply(phg_mg_s* M, int KP, const double* v, double* z, hipStream_t s)
+template <typename T>
+void mg_apply_t(phg_mg_s* M, int KP, const T* v, T* z, hipStream_t s)
 {
+    using B = MgT<T>;
+    using MT = typename B::MT;
+    const T om = (T)MG_OMEGA;
     const int nlev = (int)M->lv.size();
     // descend: level 0 takes b = v (read directly), result to z
-    const double* bk = v;
+    const T* bk = v;
     for (int k = 0; k < nlev - 1; ++k) {
         MgLevel& l = *M->lv[k];
-        const Ell<double>& E = l.A->ell64;
+        const auto& E = B::E(l);
         const size_t w = (size_t)l.n * KP;
-        double* xa = (k == 0) ? z : l.x0.p;
-        hipLaunchKernelGGL(mg_jacobi, grid_for(w), dim3(256), 0, s, l.n, KP, E.width, E.col.p,
-                           E.val.p, l.dinv.p, (const double*)nullptr, bk, l.x1.p, MG_OMEGA);
-        hipLaunchKernelGGL(mg_jacobi, grid_for(w), dim3(256), 0, s, l.n, KP, E.width, E.col.p,
-                           E.val.p, l.dinv.p, l.x1.p, bk, xa, MG_OMEGA);
-        hipLaunchKernelGGL(mg_residual, grid_for(w), dim3(256), 0, s, l.n, KP, E.width, E.col.p,
-                           E.val.p, xa, bk, l.r.p);
+        T* xa = (k == 0) ? z : B::x0(l);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(mg_jacobi<T, MT>), grid_for(w), dim3(256), 0, s, l.n,
+                           KP, E.width, E.col.p, E.val.p, B::dinv(l), (const T*)nullptr, bk,
+                           B::x1(l), om);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(mg_jacobi<T, MT>), grid_for(w), dim3(256), 0, s, l.n,
+                           KP, E.width, E.col.p, E.val.p, B::dinv(l), B::x1(l), bk, xa, om);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(mg_residual<T, MT>), grid_for(w), dim3(256), 0, s, l.n,
+                           KP, E.width, E.col.p, E.val.p, xa, bk, B::r(l));
         MgLevel& c = *M->lv[k + 1];
-        hipLaunchKernelGGL(mg_restrict, grid_for((size_t)c.n * KP), dim3(256), 0, s, c.n, KP,
-                           l.rrp.p, l.ridx.p, l.rw.p, l.r.p, c.b.p);
-        bk = c.b.p;
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(mg_restrict<T>), grid_for((size_t)c.n * KP), dim3(256),
+                           0, s, c.n, KP, l.rrp.p, l.ridx.p, B::rw(l), B::r(l), B::b(c));
+        bk = B::b(c);
     }
     MgLevel& cl = *M->lv[nlev - 1];
-    double* xcoarse = (nlev == 1) ? z : cl.x0.p;
-    hipLaunchKernelGGL(mg_dense, dim3((unsigned)M->nc), dim3(256), 0, s, M->nc, KP, M->cinv.p, bk,
-                       xcoarse);
+    T* xcoarse = (nlev == 1) ? z : B::x0(cl);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(mg_dense<T>), dim3((unsigned)M->nc), dim3(256), 0, s, M->nc,
+                       KP, B::cinv(M), bk, xcoarse);
     // ascend
     for (int k = nlev - 2; k >= 0; --k) {
         MgLevel& l = *M->lv[k];
         MgLevel& c = *M->lv[k + 1];
-        const Ell<double>& E = l.A->ell64;
+        const auto& E = B::E(l);
         const size_t w = (size_t)l.n * KP;
-        const double* bl = (k == 0) ? v : l.b.p;
-        double* xa = (k == 0) ? z : l.x0.p;
-        hipLaunchKernelGGL(mg_prolong_add, grid_for(w), dim3(256), 0, s, l.n, KP, l.p0.p, l.p1.p,
-                           c.x0.p, xa);
-        hipLaunchKernelGGL(mg_jacobi, grid_for(w), dim3(256), 0, s, l.n, KP, E.width, E.col.p,
-                           E.val.p, l.dinv.p, xa, bl, l.x1.p, MG_OMEGA);
-        hipLaunchKernelGGL(mg_jacobi, grid_for(w), dim3(256), 0, s, l.n, KP, E.width, E.col.p,
-                           E.val.p, l.dinv.p, l.x1.p, bl, xa, MG_OMEGA);
+        const T* bl = (k == 0) ? v : B::b(l);
+        T* xa = (k == 0) ? z : B::x0(l);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(mg_prolong_add<T>), grid_for(w), dim3(256), 0, s, l.n, KP,
+                           l.p0.p, l.p1.p, B::x0(c), xa);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(mg_jacobi<T, MT>), grid_for(w), dim3(256), 0, s, l.n,
+                           KP, E.width, E.col.p, E.val.p, B::dinv(l), xa, bl, B::x1(l), om);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(mg_jacobi<T, MT>), grid_for(w), dim3(256), 0, s, l.n,
+                           KP, E.width, E.col.p, E.val.p, B::dinv(l), B::x1(l), bl, xa, om);
     }
+}
+
+void mg_apply(phg_mg_s* M, int KP, const double* v, double* z, hipStream_t s)
+{
+    mg_apply_t<double>(M, KP, v, z, s);
+}
+
+void mg_apply_f32(phg_mg_s* M, int KP, const float* v, float* z, hipStream_t s)
+{
+    mg_apply_t<float>(M, KP, v, z, s);
+}
+
+// FP32 copies of every level's operator values, inverse diagonal and
+// restriction weights, the coarse inverse, and FP32 work vectors for kp
+// columns.
+int mg_reserve_f32(phg_mg_s* M, int kp)
+{
+    if (!M->cinvf.p) {
+        for (auto& l : M->lv) {
+            int rc = ensure_ell32(l->A, nullptr);
+            if (rc) return rc;
+            PHG_HIP(l->dinvf.alloc(l->n));
+            if (l->n)
+                hipLaunchKernelGGL(mg_to_f32, grid_for(l->n), dim3(256), 0, nullptr,
+                                   (size_t)l->n, l->dinv.p, l->dinvf.p);
+            if (l->rw.n) {
+                PHG_HIP(l->rwf.alloc(l->rw.n));
+                hipLaunchKernelGGL(mg_to_f32, grid_for(l->rw.n), dim3(256), 0, nullptr, l->rw.n,
+                                   l->rw.p, l->rwf.p);
+            }
+        }
+        const size_t nci = (size_t)M->nc * M->nc;
+        PHG_HIP(M->cinvf.alloc(nci));
+        hipLaunchKernelGGL(mg_to_f32, grid_for(nci), dim3(256), 0, nullptr, nci, M->cinv.p,
+                           M->cinvf.p);
+        PHG_HIP(hipGetLastError());
+        PHG_HIP(hipDeviceSynchronize());
+    }
+    if (M->kpf >= kp) return PHG_OK;
+    for (auto& l : M->lv) {
+        const size_t sz = (size_t)l->n * kp;
+        PHG_HIP(l->x0f.alloc(sz));
+        PHG_HIP(l->x1f.alloc(sz));
+        PHG_HIP(l->bf.alloc(sz));
+        PHG_HIP(l->rf.alloc(sz));
+    }
+    M->kpf = kp;
+    return PHG_OK;
 }
 
 int mg_rows(const phg_mg_s* M) { return M->lv.empty() ? 0 : M->lv[0]->n; }

@@ -110,6 +110,8 @@ int assemble_boundary_device(phg_mesh_s* M, int kind, double c_mass, double c_la
 // multigrid preconditioner (mg.hip)
 int mg_reserve(phg_mg_s* M, int kp);
 void mg_apply(phg_mg_s* M, int KP, const double* v, double* z, hipStream_t s);
+int mg_reserve_f32(phg_mg_s* M, int kp);
+void mg_apply_f32(phg_mg_s* M, int KP, const float* v, float* z, hipStream_t s);
 int mg_rows(const phg_mg_s* M);
 // Right preconditioner of the batched GMRES: z = P^-1 v for n x KP FP64
 // vectors, enqueued on s (captured into the cycle graph, so a fixed kernel
@@ -119,6 +121,10 @@ struct BatchPrecond {
     virtual int rows() const = 0;
     virtual int reserve(int kp) = 0;
     virtual void apply(int KP, const double* v, double* z, hipStream_t s) = 0;
+    // FP32 form for the mixed-precision solves (optional)
+    virtual bool f32() const { return false; }
+    virtual int reserve_f32(int) { return PHG_ERR_ARG; }
+    virtual void apply_f32(int, const float*, float*, hipStream_t) {}
 };
 BatchPrecond* mg_precond(phg_mg_s* M);
 int gmres_batch_solve(phg_matrix_s* A, int K, const double* B, double* X,

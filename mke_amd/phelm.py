@@ -489,9 +489,13 @@ class Multigrid:
         self.levels = lib().phg_mg_levels(self.h)
 
     def solve_batch(self, X, B, restart=30, rtol=1e-10, max_iters=10000, ortho=CGS2,
-                    use_x0=False, graph=True, stream=None):
+                    use_x0=False, graph=True, precision=F64, inner_rtol=1e-5, stream=None):
+        """K right-preconditioned solves (one V-cycle per step); precision F32
+        or BF16 runs the Krylov vectors and the V-cycle in FP32 under FP64
+        refinement."""
         K = int(B.shape[1]) if B.dim() == 2 else 1
-        o = _opts(restart, rtol, max_iters, ortho, F64, use_x0=use_x0, graph=graph)
+        o = _opts(restart, rtol, max_iters, ortho, precision, inner_rtol=inner_rtol,
+                  use_x0=use_x0, graph=graph)
         infos = (GmresInfo * K)()
         check(lib().phg_gmres_batch_mg(self.h, K, _ptr(B), _ptr(X), C.byref(o), infos,
                                        _stream(stream)), "phg_gmres_batch_mg")

@@ -81,3 +81,25 @@ def test_mg_batch_columns_independent_and_helmholtz():
     for q in range(4):
         r = B[:, q] - O.spmv(rp, ci, v, Xh[:, q])
         assert np.linalg.norm(r) / np.linalg.norm(B[:, q]) <= 1e-10 * 1.000001
+
+
+@pytest.mark.parametrize("prec", [P.F32, P.BF16])
+def test_mg_mixed_precision_helmholtz_and_poisson(prec):
+    """FP32 Krylov vectors and V-cycle under FP64 refinement reach the same
+    true FP64 residual (checked with the oracle's SpMV) in few steps."""
+    print("mg mixed prec=%d" % prec, flush=True)
+    level = 5
+    pm = P.Mesh.load(oracle_mesh(level)[0])
+    B = _rhs(level, 4)
+    cm, cl = P.helmholtz_coeffs(tau=1.0, sigma=0.0, theta=1.0, mu=1e-2)
+    for kind, a, b, okind in ((P.HELMHOLTZ, cm, cl, O.KIND_HELMHOLTZ),
+                              (P.LAPLACE, 1.0, 0.0, O.KIND_LAPLACE)):
+        mg = P.Multigrid(pm, kind, a, b)
+        X = torch.zeros((len(B), 4), dtype=torch.float64, device=DEV)
+        infos = mg.solve_batch(X, torch.as_tensor(B, device=DEV), rtol=1e-10, precision=prec)
+        assert all(i["converged"] and i["iterations"] <= 60 for i in infos), infos
+        rp, ci, v = oracle_matrix(level, okind, a, b)
+        Xh = X.cpu().numpy()
+        for q in range(4):
+            r = B[:, q] - O.spmv(rp, ci, v, Xh[:, q])
+            assert np.linalg.norm(r) / np.linalg.norm(B[:, q]) <= 1e-10 * 1.000001, (kind, q)
