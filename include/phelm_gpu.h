@@ -180,6 +180,9 @@ typedef struct {
     double inner_rtol; /* relative reduction per refinement step (mixed)   */
     int use_x0;      /* 0: start from x = 0; 1: start from the given x      */
     int graph;       /* 1: replay each restart cycle as one captured hipGraph */
+    int diag_scale;  /* 1: right-precondition with the inverse diagonal, x = D^-1 y
+                        (the batched FP64 solves without another preconditioner;
+                        the stopping test stays on ||b - A x|| / ||b||)    */
 } phg_gmres_opts;
 
 typedef struct {

@@ -536,11 +536,12 @@ bk_solve_y(int K, BCol* __restrict__ cols)
         for (int i = 0; i < k; ++i) c->y[i] = y[i];
 }
 
-// X += V y per column (FP64 accumulation); X is n x K (the caller's layout).
+// X += D (V y) per column (FP64 accumulation; D = dscale, or 1 when null: the
+// diagonal right preconditioner); X is n x K (the caller's layout).
 template <typename VT, int RPT>
 __global__ void __launch_bounds__(BB_BS)
 bk_xupdate(int n, int KP, int K, const VT* __restrict__ V, size_t bs, double* __restrict__ X,
-           const BCol* __restrict__ cols)
+           const BCol* __restrict__ cols, const double* __restrict__ dscale)
 {
     const int q = threadIdx.x & (KP - 1);
     const int RB = BB_BS / KP;
@@ -554,7 +555,7 @@ bk_xupdate(int n, int KP, int K, const VT* __restrict__ V, size_t bs, double* __
         if (row >= n) break;
         double s = 0.0;
         for (int i = 0; i < k; ++i) s += dd(V[(size_t)i * bs + (size_t)row * KP + q]) * c->y[i];
-        X[(size_t)row * K + q] += s;
+        X[(size_t)row * K + q] += dscale ? dscale[row] * s : s;
     }
 }
 
@@ -811,7 +812,9 @@ void enqueue_batch_cycle(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K,
     }
     hipLaunchKernelGGL(bk_solve_y, dim3(cb), block, 0, s, K, cols);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_xupdate<VT, RPT>), grid, block, 0, s, n, KP, K,
-                       pc ? (const VT*)w->Z.p : (const VT*)V, bs, X, cols);
+                       pc ? (const VT*)w->Z.p : (const VT*)V, bs, X, cols,
+                       (!pc && (const void*)E.val.p == (const void*)A->ell64d.val.p)
+                           ? A->dinv.p : (const double*)nullptr);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_residual<VT, RPT>), grid, block, 0, s, n, KP, K,
                        A->ell64.width, A->ell64.col.p, A->ell64.val.p, B, X, W0, part, G, cols);
     hipLaunchKernelGGL(bk_cycle_start, dim3(cb), block, 0, s, KP, K, part, G, cols, w->sum.p,
@@ -1020,8 +1023,14 @@ int gmres_batch_solve_pc(phg_matrix_s* A, int K, const double* B, double* X,
     rc = prepare_batch(A, slot, o.precision == PHG_F64 ? PHG_F64 : PHG_F32, o.restart, kp,
                        shape_rpt(A->n_rows, kp, o.precision), pc != nullptr, &w);
     if (rc) return rc;
-    if (o.precision == PHG_F64)
+    if (o.precision == PHG_F64) {
+        if (o.diag_scale && !pc) {  // right preconditioning by D^-1: solve (A D^-1) y = b, x = D^-1 y
+            rc = ensure_ell64_diag(A, s);
+            if (rc) return rc;
+            return batch_run<double, double>(A, A->ell64d, w, K, B, X, o, infos, nullptr, s);
+        }
         return batch_run<double, double>(A, A->ell64, w, K, B, X, o, infos, pc, s);
+    }
     if (o.precision == PHG_F32) {
         rc = ensure_ell32(A, s);
         if (rc) return rc;

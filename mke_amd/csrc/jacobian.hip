@@ -201,6 +201,7 @@ int jacobian_calc2_strided(phg_jacobian_s* J, int np, const double* d_u, const d
         o.ortho = PHG_CGS1;
         o.restart = 32;
         o.rtol = 1e-13;
+        o.diag_scale = 1;
     }
     const size_t gat = (size_t)(n + os) * P;
     hipLaunchKernelGGL(jk_gather, dim3((unsigned)((gat + 255) / 256)), dim3(256), 0, s, n, os, np,

@@ -94,6 +94,8 @@ struct phg_matrix_s {
     phg::Ell<double> ell64;
     phg::Ell<float> ell32;
     phg::Ell<uint16_t> ell16;      // bfloat16 bit patterns
+    phg::Ell<double> ell64d;       // columns scaled by dinv (diagonal right preconditioning)
+    phg::DevBuf<double> dinv;      // 1 / diagonal
     std::unique_ptr<phg::GmresWork, phg::GmresWorkDeleter> work[3];  // per precision
     phg::BlockWork* bwork = nullptr;  // block GMRES workspace
     phg::BatchWork* batch[2] = {nullptr, nullptr};  // batched GMRES: FP64, mixed
@@ -139,6 +141,8 @@ int batch_shape_get(int n, int K, int prec);
 int batch_shape_set(int n, int K, int prec, int rpt);
 int ensure_mesh_device(phg_mesh_s* m);
 int ensure_ell64(phg_matrix_s* A, hipStream_t s);
+// A D^-1 (D = diag A) as an ELL copy, and D^-1 (square matrices with a non-zero diagonal)
+int ensure_ell64_diag(phg_matrix_s* A, hipStream_t s);
 int ensure_ell32(phg_matrix_s* A, hipStream_t s);
 int ensure_ell16(phg_matrix_s* A, hipStream_t s);
 // matrix with host structure given, device values uploaded from host

@@ -63,7 +63,7 @@ __global__ void csr_to_ell(int n, int width, const int* __restrict__ rp,
     }
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 ell_spmv_f64(int n, int width, const int* __restrict__ col, const double* __restrict__ val,
              const double* __restrict__ x, double* __restrict__ y)
 {
@@ -73,7 +73,7 @@ ell_spmv_f64(int n, int width, const int* __restrict__ col, const double* __rest
 }
 
 template <typename V>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 ell_spmv_f32(int n, int width, const int* __restrict__ col, const V* __restrict__ val,
              const float* __restrict__ x, float* __restrict__ y)
 {
@@ -187,8 +187,70 @@ int build_ell(phg_matrix_s* A, Ell<V>& E, hipStream_t s)
 }  // namespace
 
 int ensure_ell64(phg_matrix_s* A, hipStream_t s) { return build_ell(A, A->ell64, s); }
+
+namespace {
+__global__ void diag_inverse(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                             const double* __restrict__ val, double* __restrict__ d)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k)
+        if (ci[k] == i) a = val[k];
+    d[i] = a != 0.0 ? 1.0 / a : 1.0;
+}
+
+// E2 = E with every value times d[column]
+__global__ void ell_scale_cols(size_t m, const int* __restrict__ col,
+                               const double* __restrict__ val, const double* __restrict__ d,
+                               int* __restrict__ col2, double* __restrict__ val2)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    col2[e] = col[e];
+    val2[e] = val[e] * d[col[e]];
+}
+}  // namespace
+
+int ensure_ell64_diag(phg_matrix_s* A, hipStream_t s)
+{
+    if (A->ell64d.width) return PHG_OK;
+    if (A->n_rows != A->n_cols) {
+        set_error("diagonal scaling needs a square matrix");
+        return PHG_ERR_ARG;
+    }
+    int rc = ensure_ell64(A, s);
+    if (rc) return rc;
+    const int n = A->n_rows;
+    const size_t m = (size_t)A->ell64.width * n;
+    PHG_HIP(A->dinv.alloc(n));
+    PHG_HIP(A->ell64d.col.alloc(m));
+    PHG_HIP(A->ell64d.val.alloc(m));
+    if (n) {
+        hipLaunchKernelGGL(diag_inverse, dim3((n + 255) / 256), dim3(256), 0, s, n, A->rp.p,
+                           A->ci.p, A->val.p, A->dinv.p);
+        hipLaunchKernelGGL(ell_scale_cols, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m,
+                           A->ell64.col.p, A->ell64.val.p, A->dinv.p, A->ell64d.col.p,
+                           A->ell64d.val.p);
+        PHG_HIP(hipGetLastError());
+    }
+    PHG_HIP(hipStreamSynchronize(s));
+    A->ell64d.width = A->ell64.width;
+    return PHG_OK;
+}
 int ensure_ell32(phg_matrix_s* A, hipStream_t s) { return build_ell(A, A->ell32, s); }
 int ensure_ell16(phg_matrix_s* A, hipStream_t s) { return build_ell(A, A->ell16, s); }
+
+// Threads per block of the one-row-per-thread SpMV: the power of two nearest
+// n / 1280 within 64..1024, so a launch is about 1280 workgroups (5 per CU)
+// at every mesh level (measured at levels 7 and 8: 128 and 512 give the
+// shortest launches, 12-14 % below 256).
+int spmv_block(int n)
+{
+    int bs = 64;
+    while (bs < 1024 && (double)n / (bs * 2) >= 1280.0 / 1.41421356) bs *= 2;
+    return bs;
+}
 
 int spmv_f64(phg_matrix_s* A, const double* x, double* y, hipStream_t s)
 {
@@ -196,7 +258,8 @@ int spmv_f64(phg_matrix_s* A, const double* x, double* y, hipStream_t s)
     if (rc) return rc;
     const int n = A->n_rows;
     if (n == 0) return PHG_OK;
-    hipLaunchKernelGGL(ell_spmv_f64, dim3((n + 255) / 256), dim3(256), 0, s, n,
+    const int bs = spmv_block(n);
+    hipLaunchKernelGGL(ell_spmv_f64, dim3((n + bs - 1) / bs), dim3(bs), 0, s, n,
                        A->ell64.width, A->ell64.col.p, A->ell64.val.p, x, y);
     PHG_HIP(hipGetLastError());
     return PHG_OK;
@@ -208,14 +271,16 @@ int spmv_f32(phg_matrix_s* A, int prec, const float* x, float* y, hipStream_t s)
     if (prec == PHG_BF16) {
         int rc = ensure_ell16(A, s);
         if (rc) return rc;
+        const int bs = spmv_block(n);
         if (n)
-            hipLaunchKernelGGL(ell_spmv_f32<uint16_t>, dim3((n + 255) / 256), dim3(256), 0, s,
+            hipLaunchKernelGGL(ell_spmv_f32<uint16_t>, dim3((n + bs - 1) / bs), dim3(bs), 0, s,
                                n, A->ell16.width, A->ell16.col.p, A->ell16.val.p, x, y);
     } else {
         int rc = ensure_ell32(A, s);
         if (rc) return rc;
+        const int bs = spmv_block(n);
         if (n)
-            hipLaunchKernelGGL(ell_spmv_f32<float>, dim3((n + 255) / 256), dim3(256), 0, s, n,
+            hipLaunchKernelGGL(ell_spmv_f32<float>, dim3((n + bs - 1) / bs), dim3(bs), 0, s, n,
                                A->ell32.width, A->ell32.col.p, A->ell32.val.p, x, y);
     }
     PHG_HIP(hipGetLastError());

@@ -31,7 +31,8 @@ _PI, _PD = C.POINTER(C.c_int), C.POINTER(C.c_double)
 
 class GmresOpts(C.Structure):
     _fields_ = [("restart", _I), ("max_iters", _I), ("rtol", _D), ("ortho", _I),
-                ("precision", _I), ("inner_rtol", _D), ("use_x0", _I), ("graph", _I)]
+                ("precision", _I), ("inner_rtol", _D), ("use_x0", _I), ("graph", _I),
+                ("diag_scale", _I)]
 
 
 class GmresInfo(C.Structure):
@@ -394,7 +395,8 @@ class Matrix:
                     max_rel_residual=info.max_rel_residual, converged=bool(info.converged))
 
     def solve_batch(self, X, B, restart=30, rtol=1e-10, max_iters=200000, ortho=CGS2,
-                    precision=F64, inner_rtol=1e-5, use_x0=False, graph=True, stream=None):
+                    precision=F64, inner_rtol=1e-5, use_x0=False, graph=True, diag_scale=False,
+                    stream=None):
         """K independent solves A x_q = b_q sharing the matrix (1 <= K <= 64):
         X, B are n x K FP64 device tensors (row-major).  Returns one info
         dict per column."""
@@ -403,6 +405,7 @@ class Matrix:
         lib().phg_gmres_default_opts(C.byref(o))
         o.restart, o.max_iters, o.rtol, o.ortho = restart, max_iters, rtol, ortho
         o.precision, o.inner_rtol, o.use_x0, o.graph = precision, inner_rtol, int(use_x0), int(graph)
+        o.diag_scale = int(diag_scale)
         infos = (GmresInfo * K)()
         rc = lib().phg_gmres_batch(self.h, K, _ptr(B), _ptr(X), C.byref(o), infos, _stream(stream))
         check(rc, "phg_gmres_batch")
@@ -753,12 +756,13 @@ class SphereJacobian:
                 "diff1_rp_", "diff2_rp_", "diff1_cos_rp_", "diff2_cos_rp_")
 
     def __init__(self, mesh, stream=None, restart=32, rtol=1e-13, ortho=CGS1,
-                 precision=F64):
+                 precision=F64, diag_scale=True):
         self.m = mesh
         self.h = C.c_void_p()
         check(lib().phg_jacobian_create(mesh.h, _stream(stream), C.byref(self.h)),
               "phg_jacobian_create")
-        self.opts = _opts(restart, rtol, 100000, ortho, precision)
+        self.opts = _opts(restart, rtol, 200000, ortho, precision)
+        self.opts.diag_scale = int(diag_scale)   # the library's default mass-solve options
         for which, name in enumerate(self.MATRICES):
             mh = C.c_void_p()
             check(lib().phg_jacobian_matrix(self.h, which, C.byref(mh)))
