@@ -64,12 +64,18 @@ def time_solves(A, b, x, steps, warmup, stream, **kw):
 
 def kernel_avg_ms(fn, launches, stream):
     """Average duration of one launch of fn: `launches` back-to-back launches
-    on `stream` between two HIP events on that stream."""
+    on `stream` between two HIP events on that stream.  A spin kernel queued
+    first holds the stream while the host enqueues every launch, so the events
+    time the device running them back to back rather than the host's issue
+    rate (a launch through ctypes costs the host a few microseconds, about
+    the SpMV's own duration)."""
     import torch
     for _ in range(10):
         fn()
     torch.cuda.synchronize()
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(stream):
+        torch.cuda._sleep(20_000_000)
     s.record(stream)
     for _ in range(launches):
         fn()
