@@ -312,8 +312,9 @@ int phg_jacobian_calc2t(phg_jacobian_t J, int npairs, const double *d_u,
  *      examples/barvortex_impl.h:108-310) --------------------------------- */
 /* BarVortexConf without the callbacks: the Coriolis/orography term lh is
  * passed as a mesh function (proj(lh, m, conf.coriolis),
- * barvortex_impl.h:121-122); the right-hand-side callback conf.rp is not
- * supported (no forcing). */
+ * barvortex_impl.h:121-122), and the right-part callback conf.rp as the
+ * mesh function of its values at the step's t (phg_barvortex_calc_rp,
+ * phg_barvortex_load_rp). */
 typedef struct {
     double tau, sigma, mu, k1, k2, theta;
 } phg_bv_conf;
@@ -368,15 +369,49 @@ int phg_barvortex_slots(phg_barvortex_t B);
  * other.  infos: nmembers host entries (may be NULL). */
 int phg_barvortex_calc(phg_barvortex_t B, int nmembers, const double *d_u, double *d_u1,
                        const phg_bv_opts *opts, phg_bv_info *infos, phg_stream_t s);
+/* The same with the right part conf.rp (barvortex_impl.h:233-244): d_rp
+ * holds, member after member, rp(x, y, t, mu, sigma) at every mesh point
+ * (x, y: zone-0 chart coordinates, MeshPoint::x()/y()) for the step's t; it
+ * is added to the fixed right side FC.  d_rp NULL = no right part. */
+int phg_barvortex_calc_rp(phg_barvortex_t B, int nmembers, const double *d_u,
+                          const double *d_rp, double *d_u1, const phg_bv_opts *opts,
+                          phg_bv_info *infos, phg_stream_t s);
 /* The same step, slot by slot, for a caller that admits and retires members
  * while others run (continuous batching): load a free slot with u, then call
  * iterate (starts loaded slots, runs one fixed-point pass of the running
  * ones; state[k] = 0 free, 1 running, 2 done), and store a done slot's u1,
  * which frees it.  A member's result does not depend on the other slots. */
 int phg_barvortex_load(phg_barvortex_t B, int slot, const double *d_u, phg_stream_t s);
+/* load with the slot's right-part values (as phg_barvortex_calc_rp); NULL
+ * d_rp = phg_barvortex_load. */
+int phg_barvortex_load_rp(phg_barvortex_t B, int slot, const double *d_u,
+                          const double *d_rp, phg_stream_t s);
 int phg_barvortex_iterate(phg_barvortex_t B, const phg_bv_opts *opts, int *state,
                           phg_bv_info *infos, phg_stream_t s);
 int phg_barvortex_store(phg_barvortex_t B, int slot, double *d_u1, phg_stream_t s);
+/* Linearised and adjoint steps with zero boundary values, for nmembers
+ * (<= slots) members at once; no step may be in flight on B.  Every solve
+ * starts from zero; infos (may be NULL) report the GMRES steps by solve and
+ * converged = all of the member's solves converged.
+ *   calc_L   (barvortex_impl.h:326-400), L_step:  u1 = the step linearised
+ *            about z, with the Jacobian terms explicit:
+ *            laplace^-1 idt A_^-1 idt [ (1/tau - (1-theta) sigma) L(u)
+ *            + (1-theta) mu L(L(u)) - (k2 J(u, lh) + k1 J(z, L(u)) + k1 J(u, L(z))) ]
+ *   calc_L_1 (:402-467), L_1_step: its inverse form with Ab_
+ *            (integrate_backward_cb, :83-103); as in the reference, the
+ *            terms are ADDED to d_u1's content (L_1_step passes zeros)
+ *   calc_LT  (:469-535), LT_step: the adjoint form, with J^T = calc1t.
+ * d_u/d_v, d_z, d_u1/d_v1: nmembers mesh functions each, one after the
+ * other. */
+int phg_barvortex_calc_L(phg_barvortex_t B, int nmembers, const double *d_u,
+                         const double *d_z, double *d_u1, const phg_bv_opts *opts,
+                         phg_bv_info *infos, phg_stream_t s);
+int phg_barvortex_calc_L_1(phg_barvortex_t B, int nmembers, const double *d_u,
+                           const double *d_z, double *d_u1, const phg_bv_opts *opts,
+                           phg_bv_info *infos, phg_stream_t s);
+int phg_barvortex_calc_LT(phg_barvortex_t B, int nmembers, const double *d_v,
+                          const double *d_z, double *d_v1, const phg_bv_opts *opts,
+                          phg_bv_info *infos, phg_stream_t s);
 
 /* ---- Baroclin implicit step (examples/baroclin.h:60-292,
  *      examples/baroclin.cpp:69-181, 445-606) ------------------------------ */

@@ -443,24 +443,43 @@ private:
     ArrayDevice<T> tmp_, diff_;
 };
 
-// BarVortexConf without the callbacks: the Coriolis/orography term is given
-// as a mesh function lh (proj(lh, m, conf.coriolis),
-// examples/barvortex_impl.h:121-122); no right-hand-side forcing.
+// BarVortexConf (examples/barvortex.h:62-96) with the reference's callbacks:
+// rp(phi, lambda, t, mu, sigma) is the right part, coriolis(phi, lambda) the
+// Coriolis + orography term l + h; either may be null.  Both are evaluated at
+// the mesh points' zone-0 chart coordinates, as the reference does.
 struct BarVortexConf {
+    typedef double (*rp_t)(double phi, double lambda, double t, double mu, double sigma);
+    typedef double (*coriolis_t)(double phi, double lambda);
     double tau, sigma, mu, k1, k2, theta;
+    rp_t rp = nullptr;
+    coriolis_t coriolis = nullptr;
 };
 
-// BarVortex (examples/barvortex.h:62-155): calc() is one implicit step
-// (examples/barvortex_impl.h:171-310) on the device.  Boundary (pole)
-// values are zero, as the reference drivers pass them; bnd_u / bnd_w must
-// be null.
+// BarVortex (examples/barvortex.h:62-240): calc() is one implicit step
+// (examples/barvortex_impl.h:171-310) on the device, calc_L / calc_L_1 /
+// calc_LT the linearised, inverse-linearised and adjoint steps
+// (:326-535) and S_step / L_step / L_1_step / LT_step their wrappers
+// (:537-571).  Boundary (pole) values are zero, as the reference drivers
+// pass them: bnd_u / bnd_w may be null or point to zeros (device pointers);
+// non-zero values are rejected.
 class BarVortex {
 public:
-    BarVortex(const Mesh& m, const BarVortexConf& conf, const double* d_lh)
+    // the Coriolis term as a device mesh function (the callbacks of conf are
+    // not used for it)
+    BarVortex(const Mesh& m, const BarVortexConf& conf, const double* d_lh) : conf_(conf)
     {
-        phg_bv_conf c{conf.tau, conf.sigma, conf.mu, conf.k1, conf.k2, conf.theta};
-        check(phg_barvortex_create(m.handle(), &c, d_lh, 1, nullptr, &h_), "BarVortex");
-        phg_bv_default_opts(&opts_);
+        init(m, d_lh);
+    }
+    // BarVortex(m, conf) of the reference: lh = proj(conf.coriolis)
+    BarVortex(const Mesh& m, const BarVortexConf& conf) : conf_(conf)
+    {
+        read_xy(m);
+        std::vector<double> lh(x_.size(), 0.0);
+        if (conf.coriolis)
+            for (size_t i = 0; i < lh.size(); ++i) lh[i] = conf.coriolis(x_[i], y_[i]);
+        ArrayDevice<double> dlh(lh.size());
+        dlh.upload(lh.data());
+        init(m, dlh.data());
     }
     BarVortex(const BarVortex&) = delete;
     BarVortex& operator=(const BarVortex&) = delete;
@@ -470,18 +489,112 @@ public:
     }
     void calc(double* Ans, const double* F, const double* bnd_u, const double* bnd_w, double t)
     {
-        (void)t;
-        if (bnd_u || bnd_w)
-            throw Error("BarVortex::calc: only zero boundary values are supported", PHG_ERR_ARG);
-        check(phg_barvortex_calc(h_, 1, F, Ans, &opts_, &info_, nullptr), "BarVortex::calc");
+        check_zero_bnd(bnd_u, bnd_w, "BarVortex::calc");
+        const double* rp = nullptr;
+        if (conf_.rp) {
+            const size_t V = x_.size();
+            std::vector<double> h(V);
+            for (size_t i = 0; i < V; ++i) h[i] = conf_.rp(x_[i], y_[i], t, conf_.mu, conf_.sigma);
+            rp_.upload(h.data());
+            rp = rp_.data();
+        }
+        check(phg_barvortex_calc_rp(h_, 1, F, rp, Ans, &opts_, &info_, nullptr),
+              "BarVortex::calc");
+    }
+    void calc_L(double* u1, const double* u, const double* z, const double* bnd_u,
+                const double* bnd_w, double)
+    {
+        check_zero_bnd(bnd_u, bnd_w, "BarVortex::calc_L");
+        check(phg_barvortex_calc_L(h_, 1, u, z, u1, &opts_, &info_, nullptr), "BarVortex::calc_L");
+    }
+    // adds to u1's content, as the reference does
+    void calc_L_1(double* u1, const double* u, const double* z, const double* bnd_u,
+                  const double* bnd_w, double)
+    {
+        check_zero_bnd(bnd_u, bnd_w, "BarVortex::calc_L_1");
+        check(phg_barvortex_calc_L_1(h_, 1, u, z, u1, &opts_, &info_, nullptr),
+              "BarVortex::calc_L_1");
+    }
+    void calc_LT(double* v1, const double* v, const double* z, const double* bnd_u,
+                 const double* bnd_w, double)
+    {
+        check_zero_bnd(bnd_u, bnd_w, "BarVortex::calc_LT");
+        check(phg_barvortex_calc_LT(h_, 1, v, z, v1, &opts_, &info_, nullptr),
+              "BarVortex::calc_LT");
+    }
+    void S_step(double* Ans, const double* F) { calc(Ans, F, nullptr, nullptr, 0.0); }
+    void L_step(double* Ans, const double* F, const double* z)
+    {
+        ArrayDevice<double> tmp(x_.size());
+        calc_L(tmp.data(), F, z, nullptr, nullptr, 0.0);
+        copy_dev(Ans, tmp.data());
+    }
+    void L_1_step(double* Ans, const double* F, const double* z)
+    {
+        ArrayDevice<double> tmp(x_.size());  // zeros
+        calc_L_1(tmp.data(), F, z, nullptr, nullptr, 0.0);
+        copy_dev(Ans, tmp.data());
+    }
+    void LT_step(double* Ans, const double* F, const double* z)
+    {
+        ArrayDevice<double> tmp(x_.size());
+        calc_LT(tmp.data(), F, z, nullptr, nullptr, 0.0);
+        copy_dev(Ans, tmp.data());
     }
     phg_bv_opts& opts() { return opts_; }
     const phg_bv_info& last_info() const { return info_; }
 
 private:
+    void read_xy(const Mesh& m)
+    {
+        int V = 0, T = 0, ni = 0, no = 0, nz = 0;
+        check(phg_mesh_info(m.handle(), &V, &T, &ni, &no, &nz), "phg_mesh_info");
+        std::vector<double> coords((size_t)V * nz * 2);
+        check(phg_mesh_arrays(m.handle(), nullptr, coords.data(), nullptr, nullptr, nullptr,
+                              nullptr, nullptr, nullptr, nullptr, nullptr),
+              "phg_mesh_arrays");
+        x_.resize((size_t)V);
+        y_.resize((size_t)V);
+        for (int i = 0; i < V; ++i) {
+            x_[i] = coords[(size_t)i * nz * 2];
+            y_[i] = coords[(size_t)i * nz * 2 + 1];
+        }
+        os_ = no;
+    }
+    void init(const Mesh& m, const double* d_lh)
+    {
+        if (x_.empty()) read_xy(m);
+        phg_bv_conf c{conf_.tau, conf_.sigma, conf_.mu, conf_.k1, conf_.k2, conf_.theta};
+        check(phg_barvortex_create(m.handle(), &c, d_lh, 1, nullptr, &h_), "BarVortex");
+        phg_bv_default_opts(&opts_);
+        rp_.resize(x_.size());
+    }
+    void check_zero_bnd(const double* bnd_u, const double* bnd_w, const char* what) const
+    {
+        for (const double* b : {bnd_u, bnd_w}) {
+            if (!b || os_ == 0) continue;
+            std::vector<double> h((size_t)os_);
+            check_hip(hipMemcpy(h.data(), b, h.size() * sizeof(double), hipMemcpyDeviceToHost),
+                      what);
+            for (double v : h)
+                if (v != 0.0)
+                    throw Error(std::string(what) + ": non-zero boundary values are not supported",
+                                PHG_ERR_ARG);
+        }
+    }
+    void copy_dev(double* dst, const double* src) const
+    {
+        check_hip(hipMemcpy(dst, src, x_.size() * sizeof(double), hipMemcpyDeviceToDevice),
+                  "BarVortex copy");
+    }
+
+    BarVortexConf conf_;
     phg_barvortex_t h_ = nullptr;
     phg_bv_opts opts_{};
     phg_bv_info info_{};
+    std::vector<double> x_, y_;
+    int os_ = 0;
+    ArrayDevice<double> rp_;
 };
 
 // Baroclin (examples/baroclin.h:60-292): the constructor takes the

@@ -49,6 +49,7 @@ struct phg_barvortex_s {
     phg_matrix_s* idt = nullptr;    // SphereLaplace::idt_ (MASS)
     phg_matrix_s* bnd3 = nullptr;   // SphereLaplace::bnd3_ (slaplace_integrate_cb, boundary)
     phg_matrix_s* A = nullptr;      // BarVortex::A_ (Helmholtz)
+    phg_matrix_s* Ab = nullptr;     // BarVortex::Ab_ (backward; built on first calc_L_1)
     phg_matrix_s* norm = nullptr;   // SphereNorm::NORM_ (full mass, V x V)
     phg_jacobian_t J = nullptr;
     phg_mg_t mg = nullptr;          // Poisson: laplace_ and its V-cycle
@@ -56,6 +57,7 @@ struct phg_barvortex_s {
     DevBuf<double> U, W, FC, Un, Wn, Un1, T1, T2;  // V x S
     DevBuf<double> JAC, F, RP, X, Y, Z, Q;          // n x S
     DevBuf<double> Ub;                              // outer x S
+    DevBuf<double> RPF;                             // V x S right parts (conf.rp values)
     DevBuf<double> part, nrm;       // reductions
     DevBuf<int> dmask;
     int G = 0;
@@ -63,11 +65,14 @@ struct phg_barvortex_s {
     std::vector<int> it_mass, it_jac, it_helm, it_pois;  // GMRES steps per slot and solve
     std::vector<double> nr0, rel, rel_prev, rel_prev2, rel_min;
     std::vector<int> stalled, diverged;
+    std::vector<int> has_rp;                        // slot loaded with a right part
+    std::vector<int> solve_fail;                    // a solve of the slot did not converge
     ~phg_barvortex_s()
     {
         delete idt;
         delete bnd3;
         delete A;
+        delete Ab;
         delete norm;
         if (J) phg_jacobian_destroy(J);
         if (mg) phg_mg_destroy(mg);
@@ -145,6 +150,8 @@ int batched(phg_barvortex_s* B, phg_matrix_s* A, const double* rhs, double* x, d
     const int rc = gmres_batch_solve(A, B->S, rhs, x, &o, inf.data(), s, pc);
     if (rc < 0) return rc;
     count_iters(inf, B->S, acc, 1);
+    for (int k = 0; k < B->S; ++k)
+        if (!inf[k].converged) B->solve_fail[k] = 1;
     return PHG_OK;
 }
 
@@ -205,6 +212,21 @@ int init_slots(phg_barvortex_s* B, const std::vector<int>& mask, const phg_bv_op
     std::vector<double> nr;
     rc = norms(B, B->U.p, nullptr, nr, s);                       // nr0 = ||u||
     if (rc) return rc;
+    // FC += rp(x, y, t, mu, sigma) for the slots loaded with a right part
+    // (barvortex_impl.h:233-244)
+    std::vector<int> forced(S, 0);
+    bool any_forced = false;
+    for (int k = 0; k < S; ++k) {
+        forced[k] = mask[k] && B->has_rp[k];
+        any_forced = any_forced || forced[k];
+    }
+    if (any_forced) {
+        rc = upload_mask(B, forced, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(bv_axpby, g256(V * S), dim3(BV_BS), 0, s, V, S, 1.0, B->FC.p, 1.0,
+                           B->RPF.p, B->FC.p, B->dmask.p, 0);
+        PHG_HIP(hipGetLastError());
+    }
     for (int k = 0; k < S; ++k)
         if (mask[k]) {
             B->nr0[k] = nr[k];
@@ -292,6 +314,219 @@ int iterate_slots(phg_barvortex_s* B, const std::vector<int>& mask, const phg_bv
                        B->dmask.p);
     PHG_HIP(hipGetLastError());
     return PHG_OK;
+}
+
+// ---- linearised and adjoint steps (barvortex_impl.h:326-571), bnd = 0 ----
+
+// Checks that no step is in flight, selects slots 0..nm-1 and clears every
+// work vector, so that the other slots' columns stay zero (zero right-hand
+// sides, which the batched solver skips).
+int lin_begin(phg_barvortex_s* B, int nm, hipStream_t s)
+{
+    const int S = B->S;
+    for (int k = 0; k < S; ++k)
+        if (B->status[k] != SLOT_FREE) {
+            set_error("phg_barvortex linearised step: slots in use by a step");
+            return PHG_ERR_ARG;
+        }
+    std::vector<int> mask(S, 0);
+    for (int k = 0; k < nm; ++k) mask[k] = 1;
+    int rc = upload_mask(B, mask, s);
+    if (rc) return rc;
+    const size_t V = B->V, n = B->n;
+    for (auto* b : {&B->U, &B->W, &B->FC, &B->Un, &B->Wn, &B->Un1, &B->T1, &B->T2})
+        PHG_HIP(hipMemsetAsync(b->p, 0, V * S * sizeof(double), s));
+    for (auto* b : {&B->JAC, &B->F, &B->RP, &B->X, &B->Y, &B->Z, &B->Q})
+        PHG_HIP(hipMemsetAsync(b->p, 0, n * S * sizeof(double), s));
+    for (auto* v : {&B->it_mass, &B->it_jac, &B->it_helm, &B->it_pois}) v->assign(S, 0);
+    B->solve_fail.assign(S, 0);
+    return PHG_OK;
+}
+
+void cols_in(phg_barvortex_s* B, int nm, const double* d, double* M, hipStream_t s)
+{
+    for (int k = 0; k < nm; ++k)
+        hipLaunchKernelGGL(bv_col_in, g256(B->V), dim3(BV_BS), 0, s, B->V, B->S, k,
+                           d + (size_t)k * B->V, M);
+}
+
+void cols_out(phg_barvortex_s* B, int nm, const double* M, double* d, hipStream_t s)
+{
+    for (int k = 0; k < nm; ++k)
+        hipLaunchKernelGGL(bv_col_out, g256(B->V), dim3(BV_BS), 0, s, B->V, B->S, k, M,
+                           d + (size_t)k * B->V);
+}
+
+// out = SphereJacobian::calc1 (sign 1) or calc1t (sign -1) of (a, b) with
+// bnd = 0 (src/sjacobian.cpp:456-468); the mass solves start from zero.
+int jac_full(phg_barvortex_s* B, const double* a, const double* b, double sign, double* out,
+             const phg_bv_opts& o, hipStream_t s)
+{
+    const int S = B->S;
+    phg_gmres_opts jo = solve_opts(o.rtol_mass, o.precision, PHG_CGS1, 0, o.max_solver_iters);
+    jo.restart = 32;
+    jo.diag_scale = 1;
+    std::vector<phg_gmres_info> inf((size_t)8 * S);
+    int rc = jacobian_calc2_strided(B->J, S, a, b, S, 1, B->JAC.p, S, 1, sign, &jo, inf.data(), s);
+    if (rc < 0) return rc;
+    count_iters(inf, S, B->it_jac, 8);
+    for (int k = 0; k < S; ++k)
+        for (int j = 0; j < 8; ++j)
+            if (!inf[(size_t)k * 8 + j].converged) B->solve_fail[k] = 1;
+    hipLaunchKernelGGL(bv_p2u, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
+                       B->mesh->d_isb.p, B->mesh->d_p2io.p, B->JAC.p, out, B->dmask.p);
+    PHG_HIP(hipGetLastError());
+    return PHG_OK;
+}
+
+// out = phelm::solve(M, idt u2p(in)) with bnd = 0 (mesh.h:719-746): the
+// Helmholtz-type A_ / Ab_ solves of the linearised steps, from zero.
+int solve_full(phg_barvortex_s* B, phg_matrix_s* M, const double* in, double* out,
+               const phg_bv_opts& o, hipStream_t s)
+{
+    const int S = B->S, n = B->n;
+    hipLaunchKernelGGL(bv_u2p, g256((size_t)n * S), dim3(BV_BS), 0, s, n, S, B->mesh->d_inner.p,
+                       in, B->Y.p, B->dmask.p);
+    spmm(B, B->idt, B->Y.p, B->RP.p, 0, s);
+    zero_cols(B, n, B->RP.p, s);
+    PHG_HIP(hipGetLastError());
+    int rc = batched(B, M, B->RP.p, B->X.p, o.rtol_helmholtz, o.precision, PHG_CGS2, 0, nullptr,
+                     o.max_solver_iters, B->it_helm, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(bv_p2u, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
+                       B->mesh->d_isb.p, B->mesh->d_p2io.p, B->X.p, out, B->dmask.p);
+    PHG_HIP(hipGetLastError());
+    return PHG_OK;
+}
+
+// out = SphereLaplace::solve(in, bnd = 0) (src/impl/slaplace_impl.h:84-130):
+// p2u(laplace^-1 idt u2p(in)), multigrid-preconditioned, from zero.
+int poisson_full(phg_barvortex_s* B, const double* in, double* out, const phg_bv_opts& o,
+                 hipStream_t s)
+{
+    const int S = B->S, n = B->n;
+    hipLaunchKernelGGL(bv_u2p, g256((size_t)n * S), dim3(BV_BS), 0, s, n, S, B->mesh->d_inner.p,
+                       in, B->Y.p, B->dmask.p);
+    spmm(B, B->idt, B->Y.p, B->Q.p, 0, s);
+    zero_cols(B, n, B->Q.p, s);
+    PHG_HIP(hipGetLastError());
+    phg_matrix_t lap = nullptr;
+    phg_mg_matrix(B->mg, &lap);
+    int rc = batched(B, lap, B->Q.p, B->Z.p, o.rtol_poisson, PHG_F64, PHG_CGS2, 0, B->mg,
+                     o.max_solver_iters, B->it_pois, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(bv_p2u, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
+                       B->mesh->d_isb.p, B->mesh->d_p2io.p, B->Z.p, out, B->dmask.p);
+    PHG_HIP(hipGetLastError());
+    return PHG_OK;
+}
+
+void lin_sum(phg_barvortex_s* B, const double* base, const double* P1, double c1,
+             const double* P2, double c2, const double* A, double ka, const double* Bv,
+             double kb, const double* C, double kc, double sj, int zb, double* r, hipStream_t s)
+{
+    hipLaunchKernelGGL(bv_lin_sum, g256((size_t)B->V * B->S), dim3(BV_BS), 0, s, B->V, B->S, base,
+                       P1, c1, P2, c2, A, ka, Bv, kb, C, kc, sj, B->mesh->d_isb.p, zb, r,
+                       B->dmask.p);
+}
+
+// calc_L (which 0) or calc_L_1 (which 1) of members 0..nm-1.
+int lin_forward(phg_barvortex_s* B, int which, int nm, const double* d_u, const double* d_z,
+                double* d_u1, const phg_bv_opts& o, hipStream_t s)
+{
+    const phg_bv_conf& c = B->conf;
+    const int S = B->S;
+    if (which == 1 && !B->Ab) {
+        // Ab_ = (1/tau - sigma (1-theta)) M + (1-theta) mu L (integrate_backward_cb)
+        int rc = assemble_device(B->mesh, PHG_HELMHOLTZ, 1.0 / c.tau - c.sigma * (1.0 - c.theta),
+                                 (1.0 - c.theta) * c.mu, s, &B->Ab);
+        if (rc) return rc;
+    }
+    int rc = lin_begin(B, nm, s);
+    if (rc) return rc;
+    cols_in(B, nm, d_u, B->U.p, s);                                   // u
+    cols_in(B, nm, d_z, B->T2.p, s);                                  // z
+    PHG_HIP(hipGetLastError());
+    if ((rc = calc1(B, B->T2.p, B->Wn.p, o, B->it_mass, s))) return rc;   // z_lapl = L(z)
+    if ((rc = calc1(B, B->U.p, B->W.p, o, B->it_mass, s))) return rc;    // pt1 = L(u)
+    hipLaunchKernelGGL(bv_bcast, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S, B->lh.p,
+                       B->T1.p, B->dmask.p);
+    if ((rc = jac_full(B, B->U.p, B->T1.p, 1.0, B->Un.p, o, s))) return rc;    // J(u, lh)
+    if ((rc = jac_full(B, B->T2.p, B->W.p, 1.0, B->Un1.p, o, s))) return rc;   // J(z, pt1)
+    if ((rc = jac_full(B, B->U.p, B->Wn.p, 1.0, B->FC.p, o, s))) return rc;    // J(u, z_lapl)
+    if ((rc = calc1(B, B->W.p, B->T1.p, o, B->it_mass, s))) return rc;   // L(pt1)
+    const double* base = nullptr;
+    double c1, c2, sj;
+    if (which == 0) {
+        c1 = 1.0 / c.tau - (1.0 - c.theta) * c.sigma;
+        c2 = (1.0 - c.theta) * c.mu;
+        sj = -1.0;
+    } else {
+        c1 = 1.0 / c.tau + c.theta * c.sigma;
+        c2 = -c.theta * c.mu;
+        sj = 1.0;
+        cols_in(B, nm, d_u1, B->T2.p, s);                             // u1 accumulates
+        base = B->T2.p;
+    }
+    lin_sum(B, base, B->W.p, c1, B->T1.p, c2, B->Un.p, c.k2, B->Un1.p, c.k1, B->FC.p, c.k1, sj, 0,
+            B->T2.p, s);
+    PHG_HIP(hipGetLastError());
+    if ((rc = solve_full(B, which == 0 ? B->A : B->Ab, B->T2.p, B->Wn.p, o, s))) return rc;
+    if ((rc = poisson_full(B, B->Wn.p, B->Un.p, o, s))) return rc;
+    cols_out(B, nm, B->Un.p, d_u1, s);
+    PHG_HIP(hipGetLastError());
+    return PHG_OK;
+}
+
+// calc_LT of members 0..nm-1.
+int lin_adjoint(phg_barvortex_s* B, int nm, const double* d_v, const double* d_z, double* d_v1,
+                const phg_bv_opts& o, hipStream_t s)
+{
+    const phg_bv_conf& c = B->conf;
+    const int S = B->S;
+    int rc = lin_begin(B, nm, s);
+    if (rc) return rc;
+    cols_in(B, nm, d_v, B->U.p, s);                                   // v
+    cols_in(B, nm, d_z, B->T2.p, s);                                  // z
+    PHG_HIP(hipGetLastError());
+    if ((rc = calc1(B, B->T2.p, B->Wn.p, o, B->it_mass, s))) return rc;   // lz = L(z)
+    if ((rc = poisson_full(B, B->U.p, B->W.p, o, s))) return rc;         // v1 = L^-1 v
+    if ((rc = solve_full(B, B->A, B->W.p, B->W.p, o, s))) return rc;     // v1 = A^-1 idt v1
+    if ((rc = calc1(B, B->W.p, B->Un.p, o, B->it_mass, s))) return rc;   // L(v1)
+    if ((rc = calc1(B, B->Un.p, B->Un1.p, o, B->it_mass, s))) return rc; // L(L(v1))
+    hipLaunchKernelGGL(bv_bcast, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S, B->lh.p,
+                       B->T1.p, B->dmask.p);
+    if ((rc = jac_full(B, B->W.p, B->T1.p, -1.0, B->FC.p, o, s))) return rc;   // JT(v1, lh)
+    if ((rc = jac_full(B, B->T2.p, B->W.p, -1.0, B->U.p, o, s))) return rc;    // JT(z, v1)
+    if ((rc = calc1(B, B->U.p, B->T1.p, o, B->it_mass, s))) return rc;   // p_lapl
+    if ((rc = jac_full(B, B->W.p, B->Wn.p, -1.0, B->U.p, o, s))) return rc;    // JT(v1, lz)
+    lin_sum(B, nullptr, B->Un.p, 1.0 / c.tau - (1.0 - c.theta) * c.sigma, B->Un1.p,
+            (1.0 - c.theta) * c.mu, B->U.p, c.k1, B->T1.p, c.k1, B->FC.p, c.k2, -1.0, 1, B->T2.p,
+            s);
+    cols_out(B, nm, B->T2.p, d_v1, s);
+    PHG_HIP(hipGetLastError());
+    return PHG_OK;
+}
+
+int lin_finish(phg_barvortex_s* B, int nm, phg_bv_info* infos, hipStream_t s)
+{
+    PHG_HIP(hipStreamSynchronize(s));
+    bool all = true;
+    for (int k = 0; k < nm; ++k) {
+        const bool ok = !B->solve_fail[k];
+        all = all && ok;
+        if (!infos) continue;
+        phg_bv_info& f = infos[k];
+        std::memset(&f, 0, sizeof(f));
+        f.iterations = 1;
+        f.converged = ok;
+        f.it_mass = B->it_mass[k];
+        f.it_jacobian = B->it_jac[k];
+        f.it_helmholtz = B->it_helm[k];
+        f.it_poisson = B->it_pois[k];
+        f.solver_iterations = f.it_mass + f.it_jacobian + f.it_helmholtz + f.it_poisson;
+    }
+    return all ? PHG_OK : PHG_ERR_NOCONV;
 }
 
 }  // namespace
@@ -390,6 +625,8 @@ int phg_barvortex_create(phg_mesh_t m, const phg_bv_conf* conf, const double* d_
     B->rel_min.assign(S, 1.0);
     B->stalled.assign(S, 0);
     B->diverged.assign(S, 0);
+    B->has_rp.assign(S, 0);
+    B->solve_fail.assign(S, 0);
     PHG_HIP(hipStreamSynchronize(s));
     *out = B.release();
     return PHG_OK;
@@ -409,7 +646,25 @@ int phg_barvortex_load(phg_barvortex_t B, int slot, const double* d_u, phg_strea
     hipLaunchKernelGGL(bv_col_in, g256(B->V), dim3(BV_BS), 0, s, B->V, B->S, slot, d_u, B->U.p);
     PHG_HIP(hipGetLastError());
     B->status[slot] = SLOT_NEW;
+    B->has_rp[slot] = 0;
     for (auto* v : {&B->it_mass, &B->it_jac, &B->it_helm, &B->it_pois}) (*v)[slot] = 0;
+    return PHG_OK;
+}
+
+int phg_barvortex_load_rp(phg_barvortex_t B, int slot, const double* d_u, const double* d_rp,
+                          phg_stream_t stream)
+{
+    int rc = phg_barvortex_load(B, slot, d_u, stream);
+    if (rc || !d_rp) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if (!B->RPF.p) {
+        PHG_HIP(B->RPF.alloc((size_t)B->V * B->S));
+        PHG_HIP(hipMemsetAsync(B->RPF.p, 0, (size_t)B->V * B->S * sizeof(double), s));
+    }
+    hipLaunchKernelGGL(bv_col_in, g256(B->V), dim3(BV_BS), 0, s, B->V, B->S, slot, d_rp,
+                       B->RPF.p);
+    PHG_HIP(hipGetLastError());
+    B->has_rp[slot] = 1;
     return PHG_OK;
 }
 
@@ -503,6 +758,13 @@ int phg_barvortex_store(phg_barvortex_t B, int slot, double* d_u1, phg_stream_t 
 int phg_barvortex_calc(phg_barvortex_t B, int nmembers, const double* d_u, double* d_u1,
                        const phg_bv_opts* opts, phg_bv_info* infos, phg_stream_t stream)
 {
+    return phg_barvortex_calc_rp(B, nmembers, d_u, nullptr, d_u1, opts, infos, stream);
+}
+
+int phg_barvortex_calc_rp(phg_barvortex_t B, int nmembers, const double* d_u, const double* d_rp,
+                          double* d_u1, const phg_bv_opts* opts, phg_bv_info* infos,
+                          phg_stream_t stream)
+{
     if (!B || !d_u || !d_u1 || nmembers < 1 || nmembers > B->S) {
         set_error("phg_barvortex_calc: bad argument (1 <= nmembers <= slots)");
         return PHG_ERR_ARG;
@@ -514,7 +776,8 @@ int phg_barvortex_calc(phg_barvortex_t B, int nmembers, const double* d_u, doubl
         }
     int rc;
     for (int k = 0; k < nmembers; ++k) {
-        rc = phg_barvortex_load(B, k, d_u + (size_t)k * B->V, stream);
+        rc = phg_barvortex_load_rp(B, k, d_u + (size_t)k * B->V,
+                                   d_rp ? d_rp + (size_t)k * B->V : nullptr, stream);
         if (rc) return rc;
     }
     std::vector<int> state(B->S);
@@ -535,6 +798,54 @@ int phg_barvortex_calc(phg_barvortex_t B, int nmembers, const double* d_u, doubl
     }
     PHG_HIP(hipStreamSynchronize((hipStream_t)stream));
     return all ? PHG_OK : PHG_ERR_NOCONV;
+}
+
+int phg_barvortex_calc_L(phg_barvortex_t B, int nmembers, const double* d_u, const double* d_z,
+                         double* d_u1, const phg_bv_opts* opts, phg_bv_info* infos,
+                         phg_stream_t stream)
+{
+    if (!B || !d_u || !d_z || !d_u1 || nmembers < 1 || nmembers > B->S) {
+        set_error("phg_barvortex_calc_L: bad argument (1 <= nmembers <= slots)");
+        return PHG_ERR_ARG;
+    }
+    phg_bv_opts o;
+    if (opts) o = *opts;
+    else phg_bv_default_opts(&o);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = lin_forward(B, 0, nmembers, d_u, d_z, d_u1, o, s);
+    return rc ? rc : lin_finish(B, nmembers, infos, s);
+}
+
+int phg_barvortex_calc_L_1(phg_barvortex_t B, int nmembers, const double* d_u, const double* d_z,
+                           double* d_u1, const phg_bv_opts* opts, phg_bv_info* infos,
+                           phg_stream_t stream)
+{
+    if (!B || !d_u || !d_z || !d_u1 || nmembers < 1 || nmembers > B->S) {
+        set_error("phg_barvortex_calc_L_1: bad argument (1 <= nmembers <= slots)");
+        return PHG_ERR_ARG;
+    }
+    phg_bv_opts o;
+    if (opts) o = *opts;
+    else phg_bv_default_opts(&o);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = lin_forward(B, 1, nmembers, d_u, d_z, d_u1, o, s);
+    return rc ? rc : lin_finish(B, nmembers, infos, s);
+}
+
+int phg_barvortex_calc_LT(phg_barvortex_t B, int nmembers, const double* d_v, const double* d_z,
+                          double* d_v1, const phg_bv_opts* opts, phg_bv_info* infos,
+                          phg_stream_t stream)
+{
+    if (!B || !d_v || !d_z || !d_v1 || nmembers < 1 || nmembers > B->S) {
+        set_error("phg_barvortex_calc_LT: bad argument (1 <= nmembers <= slots)");
+        return PHG_ERR_ARG;
+    }
+    phg_bv_opts o;
+    if (opts) o = *opts;
+    else phg_bv_default_opts(&o);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = lin_adjoint(B, nmembers, d_v, d_z, d_v1, o, s);
+    return rc ? rc : lin_finish(B, nmembers, infos, s);
 }
 
 }  // extern "C"

@@ -176,4 +176,37 @@ __global__ void bv_set_cols(size_t rows, int S, double v, double* __restrict__ x
     if (mask[e % S]) x[e] = v;
 }
 
+// r[p][s] = lh[p] for the masked columns, 0 elsewhere
+__global__ void bv_bcast(int V, int S, const double* __restrict__ lh, double* __restrict__ r,
+                         const int* __restrict__ mask)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)V * S) return;
+    r[e] = mask[e % S] ? lh[e / S] : 0.0;
+}
+
+// Right side of the linearised and adjoint steps, in the reference's order:
+//   jt = ((ka A + kb B) + kc C) * sj
+//   r  = (((base or 0) + c1 P1) + c2 P2) + jt,  0 on boundary points if zb
+// (calc_L: A, B, C = J(u, lh), J(z, L u), J(u, L z), ka = k2, kb = kc = k1;
+// calc_LT: A, B, C = J^T(v1, L z), L J^T(z, v1), J^T(v1, lh), ka = kb = k1,
+// kc = k2.)  Masked columns only.
+__global__ void bv_lin_sum(int V, int S, const double* __restrict__ base,
+                           const double* __restrict__ P1, double c1,
+                           const double* __restrict__ P2, double c2,
+                           const double* __restrict__ A, double ka,
+                           const double* __restrict__ B, double kb,
+                           const double* __restrict__ C, double kc, double sj,
+                           const int* __restrict__ isb, int zb, double* __restrict__ r,
+                           const int* __restrict__ mask)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)V * S) return;
+    if (!mask[e % S]) return;
+    const double jt = ((ka * A[e] + kb * B[e]) + kc * C[e]) * sj;
+    const double p1 = P1[e] * c1, p2 = P2[e] * c2;
+    const double v = (((base ? base[e] : 0.0) + p1) + p2) + jt;
+    r[e] = zb && isb[e / S] ? 0.0 : v;
+}
+
 }  // namespace

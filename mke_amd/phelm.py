@@ -101,6 +101,11 @@ SIGNATURES = {
     "phg_barvortex_load": (_I, [_P, _I, _P, _P]),
     "phg_barvortex_iterate": (_I, [_P, _P, _P, _P, _P]),
     "phg_barvortex_store": (_I, [_P, _I, _P, _P]),
+    "phg_barvortex_calc_rp": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
+    "phg_barvortex_load_rp": (_I, [_P, _I, _P, _P, _P]),
+    "phg_barvortex_calc_L": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
+    "phg_barvortex_calc_L_1": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
+    "phg_barvortex_calc_LT": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     "phg_bc_default_opts": (None, [_P]),
     "phg_baroclin_create": (_I, [_P, _P, _P, _I, _P, C.POINTER(_P)]),
     "phg_baroclin_destroy": (None, [_P]),
@@ -266,6 +271,12 @@ class Mesh:
                                     _pi(out["outer"]), _pi(out["p2io"])), "phg_mesh_arrays")
         out["outer"] = out["outer"][:self.outer_size]
         return out
+
+    def zone0_xy(self):
+        """(x, y) = MeshPoint::x(), y() (src/mesh.h:126-148) of every point:
+        its zone-0 chart coordinates, where the reference evaluates its
+        callbacks (proj, conf.rp)."""
+        return self.arrays()["coords"][:, 0, :].copy()
 
     def csr_structure(self):
         rp = np.zeros(self.inner_size + 1, np.int32)
@@ -564,18 +575,76 @@ class BarVortex:
                                          _stream(stream), C.byref(self.h)), "phg_barvortex_create")
         self.slots = slots
 
-    def calc(self, u1, u, opts=None, stream=None):
-        """u, u1: (nmembers, n_points) FP64 device tensors (or (n_points,))."""
+    def calc(self, u1, u, opts=None, stream=None, rp=None):
+        """u, u1: (nmembers, n_points) FP64 device tensors (or (n_points,));
+        rp: the right part conf.rp(x, y, t, mu, sigma) at every point for the
+        step's t, same shape as u (or None)."""
         k = 1 if u.dim() == 1 else int(u.shape[0])
+        if rp is not None and tuple(rp.shape) != tuple(u.shape):
+            raise PhelmError("BarVortex.calc: rp must have the shape of u")
         infos = (BvInfo * k)()
-        rc = lib().phg_barvortex_calc(self.h, k, _ptr(u), _ptr(u1),
-                                      C.byref(opts) if opts is not None else None, infos,
-                                      _stream(stream))
+        rc = lib().phg_barvortex_calc_rp(self.h, k, _ptr(u), _ptr(rp) if rp is not None else None,
+                                         _ptr(u1), C.byref(opts) if opts is not None else None,
+                                         infos, _stream(stream))
         check(rc, "phg_barvortex_calc")
         return [_bv_info(i) for i in infos]
 
-    def load(self, slot, u, stream=None):
-        check(lib().phg_barvortex_load(self.h, slot, _ptr(u), _stream(stream)), "phg_barvortex_load")
+    def rp_values(self, fn, t):
+        """conf.rp evaluated as BarVortex::calc does (barvortex_impl.h:233-244):
+        fn(x, y, t, mu, sigma) at the zone-0 chart coordinates, a host array."""
+        if not hasattr(self, "_xy"):
+            self._xy = self.m.zone0_xy()
+        x, y = self._xy[:, 0], self._xy[:, 1]
+        return np.asarray(fn(x, y, t, self.conf.mu, self.conf.sigma), dtype=np.float64)
+
+    def _lin(self, name, out, a, z, opts, stream):
+        k = 1 if a.dim() == 1 else int(a.shape[0])
+        if tuple(z.shape) != tuple(a.shape) or tuple(out.shape) != tuple(a.shape):
+            raise PhelmError("BarVortex.%s: arguments must have one shape" % name)
+        infos = (BvInfo * k)()
+        rc = getattr(lib(), "phg_barvortex_" + name)(
+            self.h, k, _ptr(a), _ptr(z), _ptr(out), C.byref(opts) if opts is not None else None,
+            infos, _stream(stream))
+        check(rc, "phg_barvortex_" + name)
+        return [_bv_info(i) for i in infos]
+
+    def calc_L(self, u1, u, z, opts=None, stream=None):
+        """BarVortex::calc_L(u1, u, z, 0, 0, t) (examples/barvortex_impl.h:326-400)."""
+        return self._lin("calc_L", u1, u, z, opts, stream)
+
+    def calc_L_1(self, u1, u, z, opts=None, stream=None):
+        """BarVortex::calc_L_1 (examples/barvortex_impl.h:402-467): adds to
+        u1's content, as the reference does."""
+        return self._lin("calc_L_1", u1, u, z, opts, stream)
+
+    def calc_LT(self, v1, v, z, opts=None, stream=None):
+        """BarVortex::calc_LT(v1, v, z, 0, 0, t) (examples/barvortex_impl.h:469-535)."""
+        return self._lin("calc_LT", v1, v, z, opts, stream)
+
+    def S_step(self, ans, F, opts=None, stream=None):
+        """BarVortex::S_step (examples/barvortex_impl.h:538-542): calc(ans, F, 0, 0, 0)."""
+        return self.calc(ans, F, opts, stream)
+
+    def L_step(self, ans, F, z, opts=None, stream=None):
+        """BarVortex::L_step (examples/barvortex_impl.h:548-554)."""
+        return self.calc_L(ans, F, z, opts, stream)
+
+    def L_1_step(self, ans, F, z, opts=None, stream=None):
+        """BarVortex::L_1_step (examples/barvortex_impl.h:556-562): calc_L_1
+        into zeros (a new buffer, so ans may be F or z)."""
+        import torch
+        tmp = torch.zeros_like(F)
+        infos = self.calc_L_1(tmp, F, z, opts, stream)
+        ans.copy_(tmp)
+        return infos
+
+    def LT_step(self, ans, F, z, opts=None, stream=None):
+        """BarVortex::LT_step (examples/barvortex_impl.h:564-570)."""
+        return self.calc_LT(ans, F, z, opts, stream)
+
+    def load(self, slot, u, stream=None, rp=None):
+        check(lib().phg_barvortex_load_rp(self.h, slot, _ptr(u), _ptr(rp) if rp is not None else None,
+                                          _stream(stream)), "phg_barvortex_load")
 
     def iterate(self, opts=None, stream=None):
         state = (C.c_int * self.slots)()

@@ -41,10 +41,14 @@ class BarVortex:
         self.bnd3 = mesh.assemble_boundary(O.KIND_LAPLACE)
         cm, cl = 1.0 / tau + sigma * theta, -theta * mu      # integrate_cb
         self.A = mesh.assemble(O.KIND_HELMHOLTZ, cm, cl)
+        # Ab_: integrate_backward_cb (barvortex_impl.h:83-103)
+        self.Ab = mesh.assemble(O.KIND_HELMHOLTZ, 1.0 / tau - sigma * (1.0 - theta),
+                                (1.0 - theta) * mu)
         self.norm_m = mesh.assemble_full(O.KIND_MASS)
         self.J = SphereJacobian(mesh)
         self.lu_idt = spla.splu(_csr(self.idt, n).tocsc())
         self.lu_A = spla.splu(_csr(self.A, n).tocsc())
+        self.lu_Ab = spla.splu(_csr(self.Ab, n).tocsc())
         self.lu_lap = spla.splu(_csr(self.lap, n).tocsc())
         self.V = len(mesh.p2io)
 
@@ -70,7 +74,10 @@ class BarVortex:
     def norm(self, u):
         return math.sqrt(float(np.dot(u, O.spmv(*self.norm_m, u))))
 
-    def calc(self, u, max_iters=1000, tol=1e-14):
+    def calc(self, u, max_iters=1000, tol=1e-14, forcing=None):
+        """One step; forcing: the right-part callback conf.rp(x, y, t, mu, sigma)
+        evaluated at every mesh point for this step's t (or None), added to
+        FC as barvortex_impl.h:233-244 does."""
         u = np.asarray(u, dtype=np.float64)
         th, k1, k2 = self.theta, self.k1, self.k2
         nr0 = self.norm(u)
@@ -78,6 +85,8 @@ class BarVortex:
         dw = self.calc1(w)
         FC = (1.0 / self.tau) * w + (self.mu * (1.0 - th)) * dw
         FC = 1.0 * FC + (-self.sigma * (1.0 - th)) * w
+        if forcing is not None:
+            FC = FC + np.asarray(forcing, dtype=np.float64)
         u_n = np.zeros(self.V)
         w_n = np.zeros(self.V)
         it = 0
@@ -97,3 +106,104 @@ class BarVortex:
             if rel < tol or math.isnan(nr):
                 break
         return u_n, it, rel
+
+
+    # ---- linearised and adjoint steps (barvortex_impl.h:326-571), bnd = 0 --
+    def jac1(self, u, v):
+        """SphereJacobian::calc1 with bnd = 0 (src/sjacobian.cpp:456-461)."""
+        return self.p2u(self.J.calc2(u, v))
+
+    def jac1t(self, u, v):
+        """SphereJacobian::calc1t with bnd = 0 (src/sjacobian.cpp:463-468)."""
+        return self.p2u(self.J.calc2t(u, v))
+
+    def _solve_full(self, lu, u1):
+        """phelm::solve of idt u2p(u1) (mesh.h:719-746) with bnd = 0."""
+        return self.p2u(lu.solve(O.spmv(*self.idt, u1[self.m.inner])))
+
+    def _lin_terms(self, u, z):
+        z_lapl = self.calc1(z)
+        pt1 = self.calc1(u)
+        jac1 = self.jac1(u, self.lh)
+        jac2 = self.jac1(z, pt1)
+        jac3 = self.jac1(u, z_lapl)
+        pt3 = np.zeros(self.V)
+        pt3 = pt3 + self.k2 * jac1                          # vec_sum2
+        pt3 = pt3 + self.k1 * jac2
+        pt3 = pt3 + self.k1 * jac3
+        return pt1, pt3
+
+    def calc_L(self, u, z):
+        """BarVortex::calc_L (barvortex_impl.h:326-400)."""
+        th, mu, sigma, tau = self.theta, self.mu, self.sigma, self.tau
+        pt1, pt3 = self._lin_terms(np.asarray(u, dtype=np.float64), np.asarray(z, dtype=np.float64))
+        pt3 = pt3 * -1.0
+        pt2 = self.calc1(pt1)
+        pt2 = pt2 * ((1.0 - th) * mu)
+        pt1 = pt1 * (1.0 / tau - (1.0 - th) * sigma)
+        u1 = np.zeros(self.V)
+        u1 = u1 + pt1
+        u1 = u1 + pt2
+        u1 = u1 + pt3
+        u1 = self._solve_full(self.lu_A, u1)
+        return self.poisson(u1)
+
+    def calc_L_1(self, u1, u, z):
+        """BarVortex::calc_L_1 (barvortex_impl.h:402-467): adds to u1 as the
+        reference does (its caller L_1_step passes zeros)."""
+        th, mu, sigma, tau = self.theta, self.mu, self.sigma, self.tau
+        pt1, pt3 = self._lin_terms(np.asarray(u, dtype=np.float64), np.asarray(z, dtype=np.float64))
+        pt2 = self.calc1(pt1)
+        pt2 = pt2 * (-th * mu)
+        pt1 = pt1 * (1.0 / tau + th * sigma)
+        r = np.asarray(u1, dtype=np.float64)
+        r = r + pt1
+        r = r + pt2
+        r = r + pt3
+        r = self._solve_full(self.lu_Ab, r)
+        return self.poisson(r)
+
+    def calc_LT(self, v, z):
+        """BarVortex::calc_LT (barvortex_impl.h:469-535)."""
+        th, mu, sigma, tau = self.theta, self.mu, self.sigma, self.tau
+        k1, k2 = self.k1, self.k2
+        v = np.asarray(v, dtype=np.float64)
+        lz = self.calc1(np.asarray(z, dtype=np.float64))
+        v1 = self.poisson(v)
+        v1 = self._solve_full(self.lu_A, v1)
+        pt1 = self.calc1(v1)
+        pt1 = pt1 * (1.0 / tau - (1.0 - th) * sigma)
+        pt2 = self.calc1(v1)
+        pt2 = self.calc1(pt2)
+        pt2 = pt2 * ((1.0 - th) * mu)
+        tmp = self.jac1t(v1, self.lh)
+        p_lapl = self.jac1t(z, v1)
+        p_lapl = self.calc1(p_lapl)
+        h1 = self.jac1t(v1, lz)
+        h1 = k1 * h1 + k1 * p_lapl                          # vec_sum1
+        h1 = h1 + k2 * tmp                                  # vec_sum2
+        h1 = h1 * -1.0
+        out = np.zeros(self.V)
+        out = out + pt1
+        out = out + pt2
+        out = out + h1
+        out[self.m.outer] = 0.0                             # set_bnd(v1, 0)
+        return out
+
+
+def an1(x, y, t):
+    """The reference driver's analytic solution (examples/test_barvortex.cpp:216-219)."""
+    return x * np.sin(y + t) * np.cos(x) ** 4
+
+
+def rp1(x, y, t, mu, sigma):
+    """The forcing that makes an1 a solution without Coriolis
+    (examples/test_barvortex.cpp:226-242), term for term."""
+    c, s = np.cos(x), np.sin(x)
+    sy, cy = np.sin(y + t), np.cos(y + t)
+    return (390 * mu * sy * x * c ** 2 + 147 * mu * sy * s * c - 400 * mu * sy * x * c ** 4
+            - 360 * mu * sy * s * c ** 3 - 20 * sigma * sy * c ** 4 * x
+            + 15 * sigma * sy * c ** 2 * x - 9 * sigma * sy * c ** 3 * s
+            + 30 * cy * c ** 4 * sy * x * x * s + 9 * cy * c ** 6 * sy * s - 45 * mu * sy * x
+            - 9 * cy * c ** 5 * sy * x - 20 * x * cy * c ** 4 - 9 * cy * c ** 3 * s
+            + 15 * x * cy * c ** 2)

@@ -8,7 +8,10 @@
 // by add() vs the GPU-assembled one, same x), solve_x.bin (Solver::solve of
 // the Helmholtz system), lap_calc2.bin, lap_calc1.bin, lap_solve.bin
 // (SphereLaplace), jac_calc2.bin, jac_calc2t.bin (SphereJacobian),
-// bv_u1.bin (one BarVortex::calc step), bc_u11.bin / bc_u21.bin (one
+// bv_u1.bin (one BarVortex::calc step), bv_rp_u1.bin (one step of the
+// reference-style BarVortex(m, conf) with the rp and coriolis callbacks at
+// t = 0.5, zero boundary arrays passed), bv_L.bin / bv_L1.bin / bv_LT.bin
+// (L_step, L_1_step of bv_L, LT_step about psi), bc_u11.bin / bc_u21.bin (one
 // Baroclin::calc step of (u, v) with the forcing written to bc_f.bin,
 // bc_g.bin and the Coriolis term bc_lh.bin), inputs u.bin, v.bin, x.bin,
 // psi.bin and summary.txt (scalars).
@@ -45,6 +48,15 @@ static double bc_g(double x, double y, double t, double, double, double, double,
     return 0.05 * std::cos(x) * std::sin(2.0 * y - t);
 }
 static double bc_cor(double phi, double) { return 2.0 * std::sin(phi); }
+// BarVortexConf callbacks
+static double bv_rp(double x, double y, double t, double mu, double sigma)
+{
+    return (1.0 + mu + sigma) * std::cos(x) * std::sin(2.0 * y + t);
+}
+static double bv_cor(double phi, double lambda)
+{
+    return 2.0 * std::sin(phi) + 0.1 * std::cos(2.0 * lambda) * std::sin(2.0 * phi);
+}
 
 static ArrayDevice<double> to_dev(const std::vector<double>& h)
 {
@@ -153,10 +165,51 @@ int main(int argc, char** argv)
         dump(out, "bv_u1.bin", from_dev(u1));
         const phg_bv_info bi = bv.last_info();
         bool threw = false;
+        ArrayDevice<double> nzb(1), zb(1);
+        {
+            const double one = 1.0;
+            nzb.upload(&one);
+        }
         try {
-            bv.calc(u1.data(), dpsi.data(), dpsi.data(), nullptr, 0.0);
+            bv.calc(u1.data(), dpsi.data(), nzb.data(), nullptr, 0.0);
         } catch (const Error& e) {
             threw = e.code() == PHG_ERR_ARG;
+        }
+
+        // BarVortex(m, conf) with the reference's callbacks; zero boundary
+        // arrays, as examples/test_barvortex.cpp passes them
+        BarVortexConf rc{600.0 * 7.292e-5, 1.6e-2, 8e-5, 1.0, 1.0, 0.5};
+        rc.rp = bv_rp;
+        rc.coriolis = bv_cor;
+        BarVortex bvr(m, rc);
+        ArrayDevice<double> ur((size_t)V);
+        bvr.calc(ur.data(), dpsi.data(), zb.data(), zb.data(), 0.5);
+        dump(out, "bv_rp_u1.bin", from_dev(ur));
+        const phg_bv_info bri = bvr.last_info();
+        ArrayDevice<double> dL((size_t)V), dL1((size_t)V), dLT((size_t)V);
+        bvr.L_step(dL.data(), du.data(), dpsi.data());
+        bvr.L_1_step(dL1.data(), dL.data(), dpsi.data());
+        bvr.LT_step(dLT.data(), dv.data(), dpsi.data());
+        dump(out, "bv_L.bin", from_dev(dL));
+        dump(out, "bv_L1.bin", from_dev(dL1));
+        dump(out, "bv_LT.bin", from_dev(dLT));
+        {
+            // the callbacks at the zone-0 chart coordinates, as the class
+            // evaluates them
+            std::vector<double> lhr((size_t)V);
+            int VV = 0, TT = 0, ni = 0, no = 0, nz = 0;
+            check(phg_mesh_info(m.handle(), &VV, &TT, &ni, &no, &nz), "phg_mesh_info");
+            std::vector<double> cc((size_t)VV * nz * 2), rpv((size_t)V);
+            check(phg_mesh_arrays(m.handle(), nullptr, cc.data(), nullptr, nullptr, nullptr,
+                                  nullptr, nullptr, nullptr, nullptr, nullptr),
+                  "phg_mesh_arrays");
+            for (int i = 0; i < V; ++i) {
+                const double x = cc[(size_t)i * nz * 2], y = cc[(size_t)i * nz * 2 + 1];
+                lhr[i] = bv_cor(x, y);
+                rpv[i] = bv_rp(x, y, 0.5, rc.mu, rc.sigma);
+            }
+            dump(out, "bv_rp_lh.bin", lhr);
+            dump(out, "bv_rp_f.bin", rpv);
         }
 
         // Baroclin: one step of (u, v) with the callbacks above at t = 0.25
@@ -197,6 +250,8 @@ int main(int argc, char** argv)
         std::fprintf(f, "norm_u %.17g\nscalar_uv %.17g\ndist_uv %.17g\n", nu, suv, duv);
         std::fprintf(f, "bv_converged %d\nbv_iterations %d\n", bi.converged, bi.iterations);
         std::fprintf(f, "bv_bnd_throws %d\n", threw ? 1 : 0);
+        std::fprintf(f, "bv_rp_converged %d\nbv_rp_iterations %d\n", bri.converged,
+                     bri.iterations);
         std::fprintf(f, "bc_converged %d\nbc_iterations %d\n", bci.converged, bci.iterations);
         std::fclose(f);
         std::printf("mirror_check level %d ok\n", level);

@@ -1,6 +1,8 @@
 """GPU BarVortex::calc (phg_barvortex_*) against the oracle restatement
 (oracle/barvortex.py, direct solves) and the member-independence property the
-serving loop relies on.
+serving loop relies on; the right part conf.rp on the reference driver's
+analytic case; the linearised, inverse and adjoint steps (calc_L, calc_L_1,
+calc_LT) against the oracle's.
 
 Bars: the GPU step's u1 within 1e-9 (SphereNorm-relative) of the oracle's
 in at most 2 more fixed-point passes (it may end earlier by the stall rule, once
@@ -110,3 +112,80 @@ def test_barvortex_mixed_precision_vs_oracle():
     ref, it, rel = ob.calc(U[0], max_iters=100)
     err = ob.norm(U1.cpu().numpy()[0] - ref) / ob.norm(ref)
     assert err <= 1e-9, err
+
+
+def test_barvortex_forced_step_vs_oracle():
+    """The reference driver's an1 / rp1 run (examples/test_barvortex.cpp:489-558)
+    at level 4: 10 steps of tau = 1e-3 with the right part, no Coriolis."""
+    print("barvortex forced an1/rp1 level=4", flush=True)
+    from oracle.barvortex import BarVortex as OB, an1, rp1
+    text, om = oracle_mesh(4)
+    pm = P.Mesh.load(text)
+    conf = dict(tau=1e-3, sigma=1.6e-2, mu=8e-5, k1=1.0, k2=1.0, theta=0.5)
+    bv = P.BarVortex(pm, _dev(np.zeros(pm.n_points)), slots=1, **conf)
+    ob = OB(om, lh=np.zeros(pm.n_points), **conf)
+    xy = pm.zone0_xy()
+    x, y = xy[:, 0], xy[:, 1]
+    u = an1(x, y, 0.0)
+    ug, uo, t = _dev(u).reshape(1, -1), u.copy(), 0.0
+    while t < 0.01:
+        rp = bv.rp_values(rp1, t)
+        assert np.array_equal(rp, rp1(x, y, t, conf["mu"], conf["sigma"]))
+        out = torch.zeros_like(ug)
+        info = bv.calc(out, ug, rp=_dev(rp).reshape(1, -1))[0]
+        assert info["converged"], info
+        ug = out
+        uo, it, rel = ob.calc(uo, forcing=rp)
+        if t == 0.0:        # one step: the single-step bar
+            assert ob.norm(ug.cpu().numpy()[0] - uo) <= 1e-9 * ob.norm(uo)
+        t += conf["tau"]
+    got = ug.cpu().numpy()[0]
+    # ten steps, each within 1e-9 of the oracle's from the same start
+    assert ob.norm(got - uo) <= 1e-8 * ob.norm(uo)
+    e_gpu, e_or = ob.norm(got - an1(x, y, t)), ob.norm(uo - an1(x, y, t))
+    assert abs(e_gpu - e_or) <= 1e-6 * e_or, (e_gpu, e_or)
+
+
+def _lin_fields(pr, om, seeds):
+    x, y, z = pr[:, 0], pr[:, 1], pr[:, 2]
+    out = []
+    for sd in seeds:
+        r = np.random.default_rng(sd)
+        f = (r.standard_normal() * x * y + r.standard_normal() * (x * x - y * y) * z
+             + r.standard_normal() * np.sin(2 * x + y) * z + 0.3 * r.standard_normal() * y)
+        out.append(f - f[om.outer[0]])
+    return np.stack(out)
+
+
+def test_barvortex_linear_steps_vs_oracle():
+    print("barvortex calc_L / calc_L_1 / calc_LT level=3", flush=True)
+    from oracle.barvortex import BarVortex as OB
+    level = 3
+    om, pr, Z, lh = _states(level, 2)
+    pm = P.Mesh.load(oracle_mesh(level)[0])
+    bv = P.BarVortex(pm, _dev(lh), slots=2, **CONF)
+    ob = OB(om, lh=lh, **CONF)
+    U, W, Vv = _lin_fields(pr, om, (1, 2)), _lin_fields(pr, om, (3, 4)), _lin_fields(pr, om, (5, 6))
+    opts = P.bv_opts(rtol_poisson=1e-12)
+    L = torch.zeros_like(_dev(U))
+    for i in bv.calc_L(L, _dev(U), _dev(Z), opts=opts):
+        assert i["converged"] and i["it_jacobian"] > 0 and i["it_helmholtz"] > 0, i
+    L1 = _dev(W).clone()
+    bv.calc_L_1(L1, _dev(U), _dev(Z), opts=opts)
+    LT = torch.zeros_like(_dev(Vv))
+    bv.calc_LT(LT, _dev(Vv), _dev(Z), opts=opts)
+    for q in range(2):
+        for got, ref in ((L, ob.calc_L(U[q], Z[q])), (L1, ob.calc_L_1(W[q], U[q], Z[q])),
+                         (LT, ob.calc_LT(Vv[q], Z[q]))):
+            g = got.cpu().numpy()[q]
+            assert ob.norm(g - ref) <= 1e-9 * ob.norm(ref), (q, ob.norm(g - ref) / ob.norm(ref))
+    # member 1 alone (slot 0 of the same engine) gives the same bits
+    alone = torch.zeros_like(L[:1])
+    bv.calc_L(alone, _dev(U[1:]), _dev(Z[1:]), opts=opts)
+    assert torch.equal(alone[0], L[1])
+    # L_1_step inverts L_step when the Jacobian terms are off
+    b0 = P.BarVortex(pm, _dev(lh), slots=1, **dict(CONF, k1=0.0, k2=0.0))
+    a, back = torch.zeros_like(L[:1]), torch.zeros_like(L[:1])
+    b0.L_step(a, _dev(U[:1]), _dev(Z[:1]), opts=opts)
+    b0.L_1_step(back, a, _dev(Z[:1]), opts=opts)
+    assert ob.norm(back.cpu().numpy()[0] - U[0]) <= 1e-9 * ob.norm(U[0])

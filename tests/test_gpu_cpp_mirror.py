@@ -120,6 +120,30 @@ def test_sphere_jacobian_and_barvortex_equal_python(run):
     assert np.array_equal(u1.cpu().numpy()[0], load("bv_u1.bin"))
 
 
+def test_barvortex_callbacks_and_linear_steps_equal_python(run):
+    """BarVortex(m, conf) with the rp / coriolis callbacks and zero boundary
+    arrays, and L_step / L_1_step / LT_step, equal the Python host class
+    given the same mesh functions, bit for bit."""
+    summ, load = run
+    assert summ["bv_rp_converged"] == 1
+    m = P.Mesh.sphere(LEVEL)
+    bv = P.BarVortex(m, torch.as_tensor(load("bv_rp_lh.bin"), device=DEV), tau=600.0 * 7.292e-5,
+                     sigma=1.6e-2, mu=8e-5, k1=1.0, k2=1.0, theta=0.5, slots=1)
+    psi = torch.as_tensor(load("psi.bin"), device=DEV).reshape(1, -1)
+    u1 = torch.zeros_like(psi)
+    bv.calc(u1, psi, rp=torch.as_tensor(load("bv_rp_f.bin"), device=DEV).reshape(1, -1))
+    assert np.array_equal(u1.cpu().numpy()[0], load("bv_rp_u1.bin"))
+    du = torch.as_tensor(load("u.bin"), device=DEV).reshape(1, -1)
+    dv = torch.as_tensor(load("v.bin"), device=DEV).reshape(1, -1)
+    a, b, c = (torch.zeros_like(psi) for _ in range(3))
+    bv.L_step(a, du, psi)
+    bv.L_1_step(b, a, psi)
+    bv.LT_step(c, dv, psi)
+    assert np.array_equal(a.cpu().numpy()[0], load("bv_L.bin"))
+    assert np.array_equal(b.cpu().numpy()[0], load("bv_L1.bin"))
+    assert np.array_equal(c.cpu().numpy()[0], load("bv_LT.bin"))
+
+
 def test_python_sphere_laplace_and_norm_equal_cpp(run):
     """The Python SphereLaplace / SphereNorm methods make the same library
     calls with the same options as the C++ classes: same bits."""
