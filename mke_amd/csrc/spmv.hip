@@ -72,6 +72,36 @@ ell_spmv_f64(int n, int width, const int* __restrict__ col, const double* __rest
     y[i] = ell_row_f64(n, width, i, col, val, x, 1.0, false);
 }
 
+// The same product for matrices larger than the L2s can hold (level 8 and
+// up, rows of at most ELL_MAXW entries): the column/value stream is read
+// with non-temporal loads, so it does not evict the gathered x, and XCD x
+// takes the contiguous x-th eighth of the blocks, so its L2 fetches about
+// an eighth of x plus the stencil's halo.  Same summation order as
+// ell_row_f64 (bitwise the same y).
+__global__ void __launch_bounds__(256)
+ell_spmv_f64_stream(int n, int width, const int* __restrict__ col,
+                    const double* __restrict__ val, const double* __restrict__ x,
+                    double* __restrict__ y)
+{
+    const int i = xcd_block((int)blockIdx.x, (int)gridDim.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    int c[ELL_MAXW];
+    double v[ELL_MAXW], xv[ELL_MAXW];
+#pragma unroll
+    for (int k = 0; k < ELL_MAXW; ++k) {
+        const size_t o = (size_t)k * n + i;
+        c[k] = k < width ? __builtin_nontemporal_load(col + o) : 0;
+        v[k] = k < width ? __builtin_nontemporal_load(val + o) : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < ELL_MAXW; ++k) xv[k] = k < width ? x[c[k]] : 0.0;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < ELL_MAXW; ++k)
+        if (k < width) s += v[k] * xv[k];
+    y[i] = s;
+}
+
 template <typename V>
 __global__ void __launch_bounds__(1024)
 ell_spmv_f32(int n, int width, const int* __restrict__ col, const V* __restrict__ val,
@@ -245,6 +275,13 @@ int ensure_ell16(phg_matrix_s* A, hipStream_t s) { return build_ell(A, A->ell16,
 // n / 1280 within 64..1024, so a launch is about 1280 workgroups (5 per CU)
 // at every mesh level (measured at levels 7 and 8: 128 and 512 give the
 // shortest launches, 12-14 % below 256).
+// Rows from which spmv_f64 uses the streaming kernel (between levels 7 and
+// 8: measured at level 8, 256 threads, non-temporal matrix loads and
+// contiguous XCD eighths give 10.6 us repeated and 12.2 us when the matrix
+// streams from HBM, against 12.4 / 14.7 us for the plain kernel; at level 7,
+// whose 14 MB stay cached, they make the launch slower).
+constexpr int SPMV_STREAM_ROWS = 1 << 18;
+
 int spmv_block(int n)
 {
     int bs = 64;
@@ -258,6 +295,12 @@ int spmv_f64(phg_matrix_s* A, const double* x, double* y, hipStream_t s)
     if (rc) return rc;
     const int n = A->n_rows;
     if (n == 0) return PHG_OK;
+    if (n >= SPMV_STREAM_ROWS && A->ell64.width <= ELL_MAXW) {
+        hipLaunchKernelGGL(ell_spmv_f64_stream, dim3((n + 255) / 256), dim3(256), 0, s, n,
+                           A->ell64.width, A->ell64.col.p, A->ell64.val.p, x, y);
+        PHG_HIP(hipGetLastError());
+        return PHG_OK;
+    }
     const int bs = spmv_block(n);
     hipLaunchKernelGGL(ell_spmv_f64, dim3((n + bs - 1) / bs), dim3(bs), 0, s, n,
                        A->ell64.width, A->ell64.col.p, A->ell64.val.p, x, y);

@@ -102,6 +102,19 @@ def test_spmv_bitwise(level):
     assert np.array_equal(y.cpu().numpy(), O.spmv(rp, ci, v, x))
 
 
+def test_spmv_bitwise_streaming_kernel():
+    """Level 8 (655,361 rows) takes the streaming SpMV kernel (non-temporal
+    matrix loads, XCD-contiguous blocks): same bits as the CSR loop."""
+    print("spmv level=8", flush=True)
+    A = P.Matrix.assemble(P.Mesh.sphere(8), P.HELMHOLTZ, CM, CL)
+    rp, ci, v = A.csr()
+    x = np.random.default_rng(8).standard_normal(A.n_rows)
+    y = torch.zeros(A.n_rows, dtype=torch.float64, device=DEV)
+    A.mult_vector(y, _dev(x))
+    torch.cuda.synchronize()
+    assert np.array_equal(y.cpu().numpy(), O.spmv(rp, ci, v, x))
+
+
 def test_spmv_f32_bf16_and_spmm():
     pm, _ = _mesh(4)
     A = P.Matrix.assemble(pm, P.HELMHOLTZ, CM, CL)
