@@ -297,7 +297,8 @@ int phg_jacobian_matrix(phg_jacobian_t J, int which, phg_matrix_t *out);
  * pairs at once: u_k = d_u + k * n_points, v_k = d_v + k * n_points (full
  * mesh vectors), J(u_k, v_k) at the inner points to d_ans + k * n_inner.
  * The 8 * npairs mass solves run as one phg_gmres_batch with opts (NULL:
- * CGS1, restart 32, rtol 1e-13); infos (8 * npairs entries) may be NULL. */
+ * CGS1, restart 8, rtol 1e-13, diagonal scaling); infos (8 * npairs entries)
+ * may be NULL. */
 int phg_jacobian_calc2(phg_jacobian_t J, int npairs, const double *d_u,
                        const double *d_v, double *d_ans,
                        const phg_gmres_opts *opts, phg_gmres_info *infos,

@@ -286,6 +286,7 @@ int calc1(phg_baroclin_s* B, const double* u, double* out, const phg_bc_opts& o,
     PHG_HIP(hipGetLastError());
     phg_gmres_opts go = solve_opts(o.rtol_mass, PHG_CGS1, 0, o.max_solver_iters);
     go.diag_scale = 1;
+    go.restart = MASS_RESTART;
     std::vector<phg_gmres_info> inf(S);
     int rc = gmres_batch_solve(B->blk[5], S, B->Q.p, B->X.p, &go, inf.data(), s);
     if (rc < 0) return rc;
@@ -540,7 +541,7 @@ int phg_baroclin_calc(phg_baroclin_t B, int nmembers, const double* d_u1, const 
                        B->W1.p, B->W2.p, B->U1.p, B->U2.p, B->ANS.p, B->dmask.p);
     PHG_HIP(hipGetLastError());
     phg_gmres_opts jo = solve_opts(o.rtol_mass, PHG_CGS1, 0, o.max_solver_iters);
-    jo.restart = 32;
+    jo.restart = MASS_RESTART;
     jo.diag_scale = 1;
     phg_gmres_opts so = solve_opts(o.rtol_system, PHG_CGS2, 1, o.max_solver_iters);
     const int P = BC_PAIRS * S;

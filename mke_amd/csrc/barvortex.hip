@@ -142,10 +142,11 @@ void zero_cols(phg_barvortex_s* B, size_t rows, double* x, hipStream_t s)
 
 int batched(phg_barvortex_s* B, phg_matrix_s* A, const double* rhs, double* x, double rtol,
             int precision, int ortho, int use_x0, phg_mg_s* pc, int max_iters,
-            std::vector<int>& acc, hipStream_t s, int diag_scale = 0)
+            std::vector<int>& acc, hipStream_t s, int diag_scale = 0, int restart = 30)
 {
     phg_gmres_opts o = solve_opts(rtol, precision, ortho, use_x0, max_iters);
     o.diag_scale = diag_scale;
+    o.restart = restart;
     std::vector<phg_gmres_info> inf(B->S);
     const int rc = gmres_batch_solve(A, B->S, rhs, x, &o, inf.data(), s, pc);
     if (rc < 0) return rc;
@@ -173,7 +174,7 @@ int calc1(phg_barvortex_s* B, const double* u, double* out, const phg_bv_opts& o
     zero_cols(B, n, B->Q.p, s);
     PHG_HIP(hipGetLastError());
     int rc = batched(B, B->idt, B->Q.p, B->X.p, o.rtol_mass, o.precision, PHG_CGS1, 0, nullptr,
-                     o.max_solver_iters, acc, s, 1);
+                     o.max_solver_iters, acc, s, 1, MASS_RESTART);
     if (rc) return rc;
     hipLaunchKernelGGL(bv_p2u, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
                        B->mesh->d_isb.p, B->mesh->d_p2io.p, B->X.p, out, B->dmask.p);
@@ -269,7 +270,7 @@ int iterate_slots(phg_barvortex_s* B, const std::vector<int>& mask, const phg_bv
         rc = jacobian_clear_pairs(B->J, S, idle, s);
         if (rc) return rc;
         phg_gmres_opts jo = solve_opts(o.rtol_mass, o.precision, PHG_CGS1, 1, o.max_solver_iters);
-        jo.restart = 32;
+        jo.restart = MASS_RESTART;
         jo.diag_scale = 1;
         std::vector<phg_gmres_info> inf((size_t)8 * S);
         rc = jacobian_calc2_strided(B->J, S, B->T2.p, B->T1.p, S, 1, B->JAC.p, S, 1, 1.0, &jo,
@@ -364,7 +365,7 @@ int jac_full(phg_barvortex_s* B, const double* a, const double* b, double sign, 
 {
     const int S = B->S;
     phg_gmres_opts jo = solve_opts(o.rtol_mass, o.precision, PHG_CGS1, 0, o.max_solver_iters);
-    jo.restart = 32;
+    jo.restart = MASS_RESTART;
     jo.diag_scale = 1;
     std::vector<phg_gmres_info> inf((size_t)8 * S);
     int rc = jacobian_calc2_strided(B->J, S, a, b, S, 1, B->JAC.p, S, 1, sign, &jo, inf.data(), s);

@@ -42,6 +42,16 @@ constexpr int BB_MAXK = 64;
 constexpr int BB_RPT = 16;  // largest rows per thread of a launch shape
 constexpr int BB_LB = 4;   // basis vectors per load batch
 
+// A restart cycle runs its Arnoldi steps in chunks ending at steps 4, 8, 16,
+// 24 and 32; after each chunk but the last the host reads how many columns
+// are still running and, when none is, ends the cycle there.  The steps
+// skipped would have found every column stopped: same results, without the
+// launches (and the preconditioner applications) of a full cycle, which
+// matters for solves that stop after a few steps (multigrid-preconditioned,
+// warm-started).
+constexpr int BB_NCHUNK = 5;
+__host__ __device__ constexpr int bb_chunk_end(int c) { return c < 2 ? 4 * (c + 1) : 8 * c; }
+
 struct BCol {
     double H[(BB_MAXM + 1) * BB_MAXM];  // unrotated Hessenberg, H[i * BB_MAXM + j]
     double R[BB_MAXM * BB_MAXM];        // rotated, upper triangular
@@ -64,17 +74,28 @@ struct BatchWork {
     DevBuf<BCol> cols;    // kp
     DevBuf<BSum> sum;     // kp
     BSum* host = nullptr;
+    DevBuf<int> nact;      // columns still running in the current cycle
+    int* hnact = nullptr;  // its host copy (pinned)
     hipStream_t cap = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // one graph per chunk of Arnoldi steps of a cycle, then the cycle's end
+    hipGraphExec_t exec[BB_NCHUNK + 1] = {};
+    bool captured = false;
     const void* key[4] = {nullptr, nullptr, nullptr, nullptr};
     int key_m = 0, key_ortho = -1, key_k = 0, key_maxit = 0;
     const void* key_pc = nullptr;
     double key_rtol = 0.0, key_inner = 0.0;
+    void drop_graphs()
+    {
+        for (auto& e : exec)
+            if (e) (void)hipGraphExecDestroy(e), e = nullptr;
+        captured = false;
+    }
     ~BatchWork()
     {
-        if (exec) (void)hipGraphExecDestroy(exec);
+        drop_graphs();
         if (cap) (void)hipStreamDestroy(cap);
         if (host) (void)hipHostFree(host);
+        if (hnact) (void)hipHostFree(hnact);
     }
 };
 
@@ -751,26 +772,39 @@ int prepare_batch(phg_matrix_s* A, int slot, int prec, int m, int kp, int rpt, b
         PHG_HIP(b->cols.alloc(kp));
         PHG_HIP(b->sum.alloc(kp));
         PHG_HIP(hipHostMalloc((void**)&b->host, sizeof(BSum) * kp, hipHostMallocDefault));
+        PHG_HIP(b->nact.alloc(1));
+        PHG_HIP(hipHostMalloc((void**)&b->hnact, sizeof(int), hipHostMallocDefault));
         w = b.release();
     }
     if (w->rpt != rpt) {  // another launch shape: new block count, new graph
         w->rpt = rpt;
         w->G = blocks_for(n, kp, rpt);
-        if (w->exec) {
-            PHG_HIP(hipGraphExecDestroy(w->exec));
-            w->exec = nullptr;
-        }
+        w->drop_graphs();
     }
     if (pc && !w->Z.p) PHG_HIP(w->Z.alloc((size_t)w->m * w->n * w->kp * (prec == PHG_F64 ? 8 : 4)));
     *out = w;
     return PHG_OK;
 }
 
-template <typename VT, typename MT, int RPT>
-void enqueue_batch_cycle(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K,
-                         const double* B, double* X, int m, int passes, double rtol,
-                         double inner_rtol, int max_iters, BatchPrecond* pc, hipStream_t s)
+// Number of columns still running in the cycle, for the host's early exit.
+__global__ void bk_count_active(int K, const BCol* __restrict__ cols, int* __restrict__ out)
 {
+    const int lane = threadIdx.x;
+    int a = 0;
+    for (int q = lane; q < K; q += 64) a += cols[q].active ? 1 : 0;
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    if (lane == 0) out[0] = a;
+}
+
+// Arnoldi steps j0 .. j1-1 of a cycle of m steps (chunk c: steps up to
+// bb_chunk_end(c)); a chunk that is not the cycle's last ends by counting
+// the columns still running.
+template <typename VT, typename MT, int RPT>
+void enqueue_batch_steps(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, int j0, int j1,
+                         int m, int passes, double rtol, double inner_rtol, int max_iters,
+                         BatchPrecond* pc, hipStream_t s)
+{
+    (void)A;
     const int n = w->n, KP = w->kp, G = w->G;
     const size_t bs = (size_t)n * KP;
     VT* V = (VT*)w->V.p;
@@ -780,7 +814,7 @@ void enqueue_batch_cycle(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K,
     BCol* cols = w->cols.p;
     const dim3 grid(G), block(BB_BS);
     const int cb = (K + 3) / 4;  // blocks of one wavefront per column
-    for (int j = 0; j < m; ++j) {
+    for (int j = j0; j < j1; ++j) {
         VT* src = (j & 1) ? W1 : W0;
         VT* dst = (j & 1) ? W0 : W1;
         if (pc) {
@@ -810,6 +844,24 @@ void enqueue_batch_cycle(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K,
         hipLaunchKernelGGL(bk_givens, dim3(cb), block, 0, s, KP, K, j, m, part, G, cols, rtol,
                            inner_rtol, max_iters);
     }
+    if (j1 < m) hipLaunchKernelGGL(bk_count_active, dim3(1), dim3(64), 0, s, K, cols, w->nact.p);
+}
+
+// The end of a cycle: y, the x update, the true residual and the next
+// cycle's start (the host summary).
+template <typename VT, typename MT, int RPT>
+void enqueue_batch_finish(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K,
+                          const double* B, double* X, double rtol, int max_iters,
+                          BatchPrecond* pc, hipStream_t s)
+{
+    const int n = w->n, KP = w->kp, G = w->G;
+    const size_t bs = (size_t)n * KP;
+    VT* V = (VT*)w->V.p;
+    VT* W0 = (VT*)w->W.p;
+    double* part = w->part.p;
+    BCol* cols = w->cols.p;
+    const dim3 grid(G), block(BB_BS);
+    const int cb = (K + 3) / 4;
     hipLaunchKernelGGL(bk_solve_y, dim3(cb), block, 0, s, K, cols);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(bk_xupdate<VT, RPT>), grid, block, 0, s, n, KP, K,
                        pc ? (const VT*)w->Z.p : (const VT*)V, bs, X, cols,
@@ -848,26 +900,35 @@ int batch_run_t(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const do
                        o.rtol, o.max_iters);
     PHG_HIP(hipGetLastError());
 
+    int nch = 0;  // chunks of the cycle
+    while (nch < BB_NCHUNK && (nch == 0 || bb_chunk_end(nch - 1) < m)) ++nch;
+    auto chunk_lo = [&](int c) { return c == 0 ? 0 : std::min(m, bb_chunk_end(c - 1)); };
+    auto chunk_hi = [&](int c) { return c == nch - 1 ? m : std::min(m, bb_chunk_end(c)); };
     if (o.graph) {
         const void* key[4] = {B, X, E.val.p, A->ell64.val.p};
-        const bool same = w->exec && w->key_m == m && w->key_ortho == o.ortho && w->key_k == K &&
-                          w->key_rtol == o.rtol && w->key_inner == inner &&
+        const bool same = w->captured && w->key_m == m && w->key_ortho == o.ortho &&
+                          w->key_k == K && w->key_rtol == o.rtol && w->key_inner == inner &&
                           w->key_maxit == o.max_iters && w->key_pc == (const void*)pc &&
                           !memcmp(key, w->key, sizeof key);
         if (!same) {
-            if (w->exec) {
-                PHG_HIP(hipGraphExecDestroy(w->exec));
-                w->exec = nullptr;
-            }
+            w->drop_graphs();
             if (!w->cap) PHG_HIP(hipStreamCreateWithFlags(&w->cap, hipStreamNonBlocking));
-            hipGraph_t g = nullptr;
-            PHG_HIP(hipStreamBeginCapture(w->cap, hipStreamCaptureModeThreadLocal));
-            enqueue_batch_cycle<VT, MT, RPT>(A, E, w, K, B, X, m, passes, o.rtol, inner, o.max_iters,
-                                        pc, w->cap);
-            PHG_HIP(hipStreamEndCapture(w->cap, &g));
-            hipError_t e = hipGraphInstantiate(&w->exec, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            PHG_HIP(e);
+            for (int c = 0; c <= nch; ++c) {
+                hipGraph_t g = nullptr;
+                PHG_HIP(hipStreamBeginCapture(w->cap, hipStreamCaptureModeThreadLocal));
+                if (c < nch)
+                    enqueue_batch_steps<VT, MT, RPT>(A, E, w, K, chunk_lo(c), chunk_hi(c), m,
+                                                     passes, o.rtol, inner, o.max_iters, pc,
+                                                     w->cap);
+                else
+                    enqueue_batch_finish<VT, MT, RPT>(A, E, w, K, B, X, o.rtol, o.max_iters, pc,
+                                                      w->cap);
+                PHG_HIP(hipStreamEndCapture(w->cap, &g));
+                hipError_t e = hipGraphInstantiate(&w->exec[c], g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                PHG_HIP(e);
+            }
+            w->captured = true;
             memcpy(w->key, key, sizeof key);
             w->key_m = m;
             w->key_ortho = o.ortho;
@@ -884,11 +945,24 @@ int batch_run_t(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const do
         bool any = false;
         for (int q = 0; q < K; ++q) any = any || w->host[q].active;
         if (!any) break;
+        for (int c = 0; c < nch; ++c) {
+            if (o.graph) {
+                PHG_HIP(hipGraphLaunch(w->exec[c], s));
+            } else {
+                enqueue_batch_steps<VT, MT, RPT>(A, E, w, K, chunk_lo(c), chunk_hi(c), m, passes,
+                                                 o.rtol, inner, o.max_iters, pc, s);
+                PHG_HIP(hipGetLastError());
+            }
+            if (c < nch - 1) {
+                PHG_HIP(hipMemcpyAsync(w->hnact, w->nact.p, sizeof(int), hipMemcpyDeviceToHost, s));
+                PHG_HIP(hipStreamSynchronize(s));
+                if (*w->hnact == 0) break;
+            }
+        }
         if (o.graph) {
-            PHG_HIP(hipGraphLaunch(w->exec, s));
+            PHG_HIP(hipGraphLaunch(w->exec[nch], s));
         } else {
-            enqueue_batch_cycle<VT, MT, RPT>(A, E, w, K, B, X, m, passes, o.rtol, inner, o.max_iters,
-                                        pc, s);
+            enqueue_batch_finish<VT, MT, RPT>(A, E, w, K, B, X, o.rtol, o.max_iters, pc, s);
             PHG_HIP(hipGetLastError());
         }
     }

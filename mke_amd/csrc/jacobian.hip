@@ -199,7 +199,7 @@ int jacobian_calc2_strided(phg_jacobian_s* J, int np, const double* d_u, const d
     else {
         phg_gmres_default_opts(&o);
         o.ortho = PHG_CGS1;
-        o.restart = 32;
+        o.restart = MASS_RESTART;
         o.rtol = 1e-13;
         o.diag_scale = 1;
     }

@@ -824,7 +824,7 @@ class SphereJacobian:
     MATRICES = ("idt_", "diff1_", "diff2_", "diff1_cos_", "diff2_cos_",
                 "diff1_rp_", "diff2_rp_", "diff1_cos_rp_", "diff2_cos_rp_")
 
-    def __init__(self, mesh, stream=None, restart=32, rtol=1e-13, ortho=CGS1,
+    def __init__(self, mesh, stream=None, restart=8, rtol=1e-13, ortho=CGS1,
                  precision=F64, diag_scale=True):
         self.m = mesh
         self.h = C.c_void_p()
