@@ -529,6 +529,14 @@ def main():
                        block_iterations=binfo["iterations"], cycles=binfo["cycles"],
                        max_rel_residual=binfo["max_rel_residual"], converged=binfo["converged"],
                        x0=X64[:, 0].cpu().numpy())
+        # block Gram-Schmidt on MFMA: step j of a 30-step cycle does V^T W and
+        # W -= V H over j+1 basis blocks, 2 * 8192 n (j+1) FP32 flops
+        its, full = binfo["iterations"], binfo["iterations"] // 30
+        rem = its - 30 * full
+        gs_flops = 16384.0 * n * (full * 465 + rem * (rem + 1) // 2)
+        batched["gs_flops_per_solve"] = gs_flops
+        batched["gs_tflops_over_solve"] = round(gs_flops / (wb / nb) / 1e12, 2)
+        batched["gs_mfma_frac_over_solve"] = round(gs_flops / (wb / nb) / 157.3e12, 4)
         log("batched64: %.1f solves/s (%.1f ms per 64), %d block iters, max rel %.2e" %
             (batched["solves_per_s"], batched["batch_ms"], binfo["iterations"],
              binfo["max_rel_residual"]))
