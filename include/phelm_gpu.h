@@ -465,6 +465,28 @@ int phg_baroclin_calc(phg_baroclin_t B, int nmembers, const double *d_u1,
                       const double *d_u2, const double *d_f, const double *d_g,
                       double *d_u11, double *d_u21, const phg_bc_opts *opts,
                       phg_bc_info *infos, phg_stream_t s);
+/* The linearised steps about (z1, z2), zero boundary values, for nmembers
+ * (<= slots) members; all arrays hold nmembers mesh functions one after the
+ * other.  The Jacobian arguments are the reference code's (two of them,
+ * J(L z1, avg w2) and J(L z2, avg u1), differ from its comments).
+ *   calc_L   (examples/baroclin.cpp:607-802, L_step): one pass with A_ and the
+ *            explicit right side w/tau - 0.5 (1-theta) sigma (w1 -+ w2) + ...;
+ *   calc_L_1 (:804-986, L_1_step): up to 10 passes with Ab_
+ *            (integrate_backward_cb, :184-244, assembled on the first call)
+ *            until max(dist) < 1e-8.
+ * infos: passes, last distance, converged (every solve converged and, for
+ * calc_L_1, the distance below 1e-8).  The reference's calc_LT has an
+ * empty body (:988-993), so its LT_step (:1009-1014) leaves the outputs as
+ * they were: there is no device counterpart, and the C++ class's LT_step
+ * does the same. */
+int phg_baroclin_calc_L(phg_baroclin_t B, int nmembers, const double *d_u1,
+                        const double *d_u2, const double *d_z1, const double *d_z2,
+                        double *d_u11, double *d_u21, const phg_bc_opts *opts,
+                        phg_bc_info *infos, phg_stream_t s);
+int phg_baroclin_calc_L_1(phg_baroclin_t B, int nmembers, const double *d_u1,
+                          const double *d_u2, const double *d_z1, const double *d_z2,
+                          double *d_u11, double *d_u21, const phg_bc_opts *opts,
+                          phg_bc_info *infos, phg_stream_t s);
 
 /* ---- serving: implicit-step requests of ensemble members on several mesh
  *      levels, continuously batched (no counterpart in the reference, which

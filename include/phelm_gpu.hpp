@@ -658,6 +658,41 @@ public:
                                 g_ ? gv_.data() : nullptr, u11, u21, &opts_, &info_, nullptr),
               "Baroclin::calc");
     }
+    // the linearised steps about (z1, z2) (baroclin.cpp:607-986); bnd as calc
+    void calc_L(double* u11, double* u21, const double* u1, const double* u2, const double* z1,
+                const double* z2, const double* bnd, double)
+    {
+        (void)bnd;
+        check(phg_baroclin_calc_L(h_, 1, u1, u2, z1, z2, u11, u21, &opts_, &info_, nullptr),
+              "Baroclin::calc_L");
+    }
+    void calc_L_1(double* u11, double* u21, const double* u1, const double* u2,
+                  const double* z1, const double* z2, const double* bnd, double)
+    {
+        (void)bnd;
+        check(phg_baroclin_calc_L_1(h_, 1, u1, u2, z1, z2, u11, u21, &opts_, &info_, nullptr),
+              "Baroclin::calc_L_1");
+    }
+    // the reference's calc_LT has an empty body (baroclin.cpp:988-993)
+    void calc_LT(double*, double*, const double*, const double*, const double*, const double*,
+                 const double*, double)
+    {
+    }
+    void L_step(double* u11, double* u21, const double* u1, const double* u2, const double* z1,
+                const double* z2)
+    {
+        calc_L(u11, u21, u1, u2, z1, z2, nullptr, 0.0);
+    }
+    void L_1_step(double* u11, double* u21, const double* u1, const double* u2,
+                  const double* z1, const double* z2)
+    {
+        calc_L_1(u11, u21, u1, u2, z1, z2, nullptr, 0.0);
+    }
+    void LT_step(double* u11, double* u21, const double* u1, const double* u2, const double* z1,
+                 const double* z2)
+    {
+        calc_LT(u11, u21, u1, u2, z1, z2, nullptr, 0.0);
+    }
     phg_bc_opts& opts() { return opts_; }
     const phg_bc_info& last_info() const { return info_; }
 

@@ -147,6 +147,92 @@ __global__ void bc_fg(int n, int S, const int* __restrict__ inner, const double*
     G[e] = g + GC[o];
 }
 
+// Jacobian pair inputs of member s for the linearised steps, the ten pairs
+// of baroclin.cpp:714-770 (calc_L) and :898-954 (calc_L_1) in the code's
+// order, half h = 0: pairs 0-4, h = 1: pairs 5-9.  m(a) = ka a + kb a_n
+// (vec_sum1), zl = L(z1) + lh (vec_sum):
+//   0 (m u1, zl)  1 (z1, m w1)  2 (z2, m w2)  3 (m u2, L z2)  4 (L z1, m w2)
+//   5 (L z2, m u1)  6 (m u2, zl)  7 (z2, m w1)  8 (z1, m u2)  9 (m u1, z2)
+__global__ void bc_lin_pairs(int V, int S, int s, int h, double ka, double kb,
+                             const double* __restrict__ u1, const double* __restrict__ u1n,
+                             const double* __restrict__ u2, const double* __restrict__ u2n,
+                             const double* __restrict__ w1, const double* __restrict__ w1n,
+                             const double* __restrict__ w2, const double* __restrict__ w2n,
+                             const double* __restrict__ z1, const double* __restrict__ z2,
+                             const double* __restrict__ dz1, const double* __restrict__ dz2,
+                             const double* __restrict__ lh, double* __restrict__ PU,
+                             double* __restrict__ PV)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)V * BC_PAIRS) return;
+    const int p = (int)(e / BC_PAIRS), t = h * BC_PAIRS + (int)(e % BC_PAIRS);
+    const size_t o = (size_t)p * S + s;
+    const double mu1 = ka * u1[o] + kb * u1n[o], mu2 = ka * u2[o] + kb * u2n[o];
+    const double mw1 = ka * w1[o] + kb * w1n[o], mw2 = ka * w2[o] + kb * w2n[o];
+    const double zl = dz1[o] + lh[p];
+    double a, b;
+    switch (t) {
+    case 0: a = mu1, b = zl; break;
+    case 1: a = z1[o], b = mw1; break;
+    case 2: a = z2[o], b = mw2; break;
+    case 3: a = mu2, b = dz2[o]; break;
+    case 4: a = dz1[o], b = mw2; break;
+    case 5: a = dz2[o], b = mu1; break;
+    case 6: a = mu2, b = zl; break;
+    case 7: a = z2[o], b = mw1; break;
+    case 8: a = z1[o], b = mu2; break;
+    default: a = mu1, b = z2[o]; break;
+    }
+    PU[e] = a;
+    PV[e] = b;
+}
+
+// F and G of the linearised steps from the ten Jacobians (JA: pairs 0-4, JB:
+// pairs 5-9, each n x 5S), in the code's vec_sum1 / vec_sum order, plus
+// FC, GC at the inner points.  which 0 (calc_L): F = ((-j0 - j1) - j2) - j3,
+// G = ((((-j4 - j5) - j6) - j7) + a2 j8) + a2 j9; which 1 (calc_L_1): the same
+// with + for the plain terms (vec_sum) and -a2 for the last two.
+__global__ void bc_lin_fg(int n, int S, int which, const int* __restrict__ inner,
+                          const double* __restrict__ JA, const double* __restrict__ JB,
+                          const double* __restrict__ FC, const double* __restrict__ GC,
+                          double a2, double* __restrict__ F, double* __restrict__ G,
+                          const int* __restrict__ mask)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * S) return;
+    const int i = (int)(e / S), s = (int)(e % S);
+    if (!mask[s]) {
+        F[e] = 0.0;
+        G[e] = 0.0;
+        return;
+    }
+    const double* ja = JA + (size_t)i * BC_PAIRS * S + (size_t)s * BC_PAIRS;
+    const double* jb = JB + (size_t)i * BC_PAIRS * S + (size_t)s * BC_PAIRS;
+    const size_t o = (size_t)inner[i] * S + s;
+    double f, g;
+    if (which == 0) {
+        f = -1.0 * ja[0] + -1.0 * ja[1];
+        f = 1.0 * f + -1.0 * ja[2];
+        f = 1.0 * f + -1.0 * ja[3];
+        g = -1.0 * ja[4] + -1.0 * jb[0];
+        g = 1.0 * g + -1.0 * jb[1];
+        g = 1.0 * g + -1.0 * jb[2];
+        g = 1.0 * g + a2 * jb[3];
+        g = 1.0 * g + a2 * jb[4];
+    } else {
+        f = ja[0] + ja[1];
+        f = f + ja[2];
+        f = f + ja[3];
+        g = ja[4] + jb[0];
+        g = g + jb[1];
+        g = g + jb[2];
+        g = 1.0 * g + -a2 * jb[3];
+        g = 1.0 * g + -a2 * jb[4];
+    }
+    F[e] = f + FC[o];
+    G[e] = g + GC[o];
+}
+
 // x[b * n + i][s] = u[inner[i]][s] for the four unknown blocks (the initial
 // guess of the 4n solve: the current w1, w2, u1, u2)
 __global__ void bc_pack4(int n, int S, const int* __restrict__ inner, const double* __restrict__ a,
@@ -211,6 +297,9 @@ struct phg_baroclin_s {
     double theta = 0.5;              // baroclin.cpp:80
     phg_matrix_s* blk[9] = {};       // the blocks of A_
     phg_matrix_t A4 = nullptr;       // A_ (4n x 4n)
+    phg_matrix_s* blkb[9] = {};      // the blocks of Ab_ (built on first calc_L_1)
+    phg_matrix_t Ab4 = nullptr;      // Ab_ (4n x 4n)
+    BcPrecond pcb;                   // the block preconditioner built on Ab_'s blocks
     phg_matrix_s* bnd3 = nullptr;    // SphereLaplace::bnd3_ (stiffness, boundary)
     phg_matrix_s* norm = nullptr;    // SphereNorm::NORM_
     phg_jacobian_t J = nullptr;
@@ -223,6 +312,7 @@ struct phg_baroclin_s {
     DevBuf<double> F, G, Y, Q, X;                           // n x S
     DevBuf<double> Ub;                                      // outer x S
     DevBuf<double> RP, ANS;                                 // 4n x S
+    DevBuf<double> Z1, Z2, DZ1, DZ2, JAC2;  // linearised steps: z, L(z) (V x S), n x 5S
     DevBuf<double> part, nrm;
     DevBuf<int> dmask;
     int Gn = 0;
@@ -230,6 +320,8 @@ struct phg_baroclin_s {
     {
         for (auto* b : blk) delete b;
         if (A4) phg_matrix_destroy(A4);
+        for (auto* b : blkb) delete b;
+        if (Ab4) phg_matrix_destroy(Ab4);
         delete bnd3;
         delete norm;
         if (J) phg_jacobian_destroy(J);
@@ -314,21 +406,21 @@ int dists(phg_baroclin_s* B, const double* a, const double* b, std::vector<doubl
 }
 
 // The 4n x 4n CSR of the nine blocks (all on the inner stencil).
-int stack_blocks(phg_baroclin_s* B)
+int stack_blocks(phg_baroclin_s* B, phg_matrix_s* const* blk, phg_matrix_t* out)
 {
     const int n = B->n;
     std::vector<int> rp(n + 1), ci;
     long nnz = 0;
     int nr = 0, nc = 0;
     PHG_HIP(hipDeviceSynchronize());
-    if (phg_matrix_info(B->blk[0], &nr, &nc, &nnz) != PHG_OK) return PHG_ERR_ARG;
+    if (phg_matrix_info(blk[0], &nr, &nc, &nnz) != PHG_OK) return PHG_ERR_ARG;
     ci.resize((size_t)nnz);
     std::vector<std::vector<double>> v(9, std::vector<double>((size_t)nnz));
     for (int k = 0; k < 9; ++k) {
-        int rc = phg_matrix_csr(B->blk[k], k == 0 ? rp.data() : nullptr,
+        int rc = phg_matrix_csr(blk[k], k == 0 ? rp.data() : nullptr,
                                 k == 0 ? ci.data() : nullptr, v[k].data());
         if (rc) return rc;
-        if (B->blk[k]->h_rp != B->blk[0]->h_rp || B->blk[k]->h_ci != B->blk[0]->h_ci) {
+        if (blk[k]->h_rp != blk[0]->h_rp || blk[k]->h_ci != blk[0]->h_ci) {
             set_error("phg_baroclin_create: blocks differ in structure");
             return PHG_ERR_ARG;
         }
@@ -348,7 +440,216 @@ int stack_blocks(phg_baroclin_s* B)
                         }
             brp[(size_t)r * n + i + 1] = (int)bci.size();
         }
-    return phg_matrix_from_csr(4 * n, 4 * n, brp.data(), bci.data(), bv.data(), &B->A4);
+    return phg_matrix_from_csr(4 * n, 4 * n, brp.data(), bci.data(), bv.data(), out);
+}
+
+// Ab_ (integrate_backward_cb, baroclin.cpp:184-244) as nine blocks, its 4n
+// CSR and the block preconditioner on its blocks, built on first use.
+int ensure_backward(phg_baroclin_s* B, hipStream_t s)
+{
+    if (B->Ab4) return PHG_OK;
+    const phg_bc_conf& c = B->conf;
+    const double tau = c.tau, sigma = c.sigma, mu = c.mu, sigma1 = c.sigma1, mu1 = c.mu1,
+                 alpha = c.alpha, theta = B->theta;
+    // the nine element formulas as (c_mass, c_lap), coefficients in the
+    // reference's order (baroclin.cpp:214-240; theta = 1/2 makes the scalings
+    // by 0.5 (1 - theta) exact, as for A_)
+    const double cm[9] = {1.0 / tau - 0.5 * (1.0 - theta) * sigma,
+                          (0.5 * sigma) * (1.0 - theta),
+                          -0.5 * (1.0 - theta) * sigma,
+                          1.0 / tau - 0.5 * (1.0 - theta) * sigma - alpha * alpha * (1.0 - theta) * mu1,
+                          -alpha * alpha / tau + alpha * alpha * (1.0 - theta) * sigma1,
+                          1.0, 0.0, 1.0, 0.0};
+    const double cl[9] = {(1.0 - theta) * mu, 0.0, 0.0, (1.0 - theta) * mu, 0.0, 0.0, -1.0, 0.0,
+                          -1.0};
+    for (int k = 0; k < 9; ++k) {
+        const int kind = (k == 5 || k == 7) ? PHG_MASS : PHG_HELMHOLTZ;
+        int rc = assemble_device(B->mesh, kind, cm[k], cl[k], s, &B->blkb[k]);
+        if (rc) return rc;
+    }
+    int rc = stack_blocks(B, B->blkb, &B->Ab4);
+    if (rc) return rc;
+    BcPrecond& P = B->pcb;
+    const int n = B->n;
+    P.n = n;
+    P.A21 = B->blkb[2];
+    P.M = B->blkb[5];
+    P.mg = B->mg;
+    for (int k : {0, 2, 3, 5}) {
+        rc = ensure_ell64(B->blkb[k], s);
+        if (rc) return rc;
+    }
+    PHG_HIP(P.d11.alloc(n));
+    PHG_HIP(P.d22.alloc(n));
+    if (n) {
+        hipLaunchKernelGGL(bc_diag_inv, g256(n), dim3(BV_BS), 0, s, n, B->blkb[0]->rp.p,
+                           B->blkb[0]->ci.p, B->blkb[0]->val.p, P.d11.p);
+        hipLaunchKernelGGL(bc_diag_inv, g256(n), dim3(BV_BS), 0, s, n, B->blkb[3]->rp.p,
+                           B->blkb[3]->ci.p, B->blkb[3]->val.p, P.d22.p);
+        PHG_HIP(hipGetLastError());
+    }
+    PHG_HIP(hipStreamSynchronize(s));
+    return PHG_OK;
+}
+
+// Baroclin::calc_L (which 0, baroclin.cpp:620-802: one pass with A_) and
+// calc_L_1 (which 1, :804-986: up to 10 passes with Ab_, stopping when the
+// distance is below 1e-8), with zero boundary values, for members 0..nm-1.
+int bc_linear(phg_baroclin_s* B, int which, int nm, const double* d_u1, const double* d_u2,
+              const double* d_z1, const double* d_z2, double* d_u11, double* d_u21,
+              const phg_bc_opts& o, phg_bc_info* infos, hipStream_t s)
+{
+    const int S = B->S, n = B->n;
+    const size_t V = B->V;
+    const phg_bc_conf& c = B->conf;
+    const double th = B->theta, tau = c.tau, sigma = c.sigma, mu = c.mu, sigma1 = c.sigma1,
+                 mu1 = c.mu1, alpha = c.alpha, a2 = alpha * alpha;
+    int rc;
+    if (which == 1 && (rc = ensure_backward(B, s))) return rc;
+    for (auto* b : {&B->Z1, &B->Z2, &B->DZ1, &B->DZ2})
+        if (!b->p) {
+            PHG_HIP(b->alloc(V * S));
+            PHG_HIP(hipMemsetAsync(b->p, 0, V * S * sizeof(double), s));
+        }
+    if (!B->JAC2.p) PHG_HIP(B->JAC2.alloc(std::max<size_t>((size_t)n * BC_PAIRS * S, 1)));
+    std::vector<int> mask(S, 0);
+    for (int k = 0; k < nm; ++k) mask[k] = 1;
+    if ((rc = upload_mask(B, mask, s))) return rc;
+    const dim3 gv = g256(V * S), bb(BV_BS);
+    for (int k = 0; k < nm; ++k) {
+        const double* src[4] = {d_u1, d_u2, d_z1, d_z2};
+        double* dst[4] = {B->U1.p, B->U2.p, B->Z1.p, B->Z2.p};
+        for (int q = 0; q < 4; ++q)
+            hipLaunchKernelGGL(bv_col_in, g256(V), bb, 0, s, (int)V, S, k, src[q] + k * V, dst[q]);
+    }
+    PHG_HIP(hipGetLastError());
+    std::vector<int> it_mass(S, 0), it_jac(S, 0), it_sys(S, 0), iters(S, 0), failed(S, 0);
+    std::vector<double> dist(S, 0.0);
+    if ((rc = calc1(B, B->U1.p, B->W1.p, o, it_mass, s))) return rc;    // w1
+    if ((rc = calc1(B, B->U2.p, B->W2.p, o, it_mass, s))) return rc;    // w2
+    if ((rc = calc1(B, B->W1.p, B->U1n1.p, o, it_mass, s))) return rc;  // dw1 (scratch)
+    if ((rc = calc1(B, B->W2.p, B->U2n1.p, o, it_mass, s))) return rc;  // dw2 (scratch)
+    if ((rc = calc1(B, B->Z1.p, B->DZ1.p, o, it_mass, s))) return rc;   // dz1
+    if ((rc = calc1(B, B->Z2.p, B->DZ2.p, o, it_mass, s))) return rc;   // dz2
+    auto lin = [&](double k1, const double* a, double k2, const double* b, double* r) {
+        hipLaunchKernelGGL(bv_axpby, gv, bb, 0, s, V, S, k1, a, k2, b, r, B->dmask.p, 1);
+    };
+    if (which == 0) {  // baroclin.cpp:684-700
+        lin(-0.5 * (1.0 - th) * sigma, B->W1.p, 0.5 * (1.0 - th) * sigma, B->W2.p, B->FC.p);
+        lin(1.0, B->FC.p, mu * (1.0 - th), B->U1n1.p, B->FC.p);
+        lin(1.0, B->FC.p, 1.0 / tau, B->W1.p, B->FC.p);
+        lin(-0.5 * (1.0 - th) * sigma, B->W1.p, -0.5 * (1.0 - th) * sigma, B->W2.p, B->GC.p);
+        lin(1.0, B->GC.p, mu * (1.0 - th), B->U2n1.p, B->GC.p);
+        lin(1.0, B->GC.p, 1.0 / tau, B->W2.p, B->GC.p);
+        lin(1.0, B->GC.p, -alpha * alpha / tau, B->U2.p, B->GC.p);
+        lin(1.0, B->GC.p, -alpha * alpha * mu1 * (1 - th), B->W2.p, B->GC.p);
+        lin(1.0, B->GC.p, alpha * alpha * sigma1 * (1 - th), B->U2.p, B->GC.p);
+    } else {           // baroclin.cpp:868-884
+        lin(0.5 * th * sigma, B->W1.p, -0.5 * th * sigma, B->W2.p, B->FC.p);
+        lin(1.0, B->FC.p, -mu * th, B->U1n1.p, B->FC.p);
+        lin(1.0, B->FC.p, 1.0 / tau, B->W1.p, B->FC.p);
+        lin(0.5 * th * sigma, B->W1.p, 0.5 * th * sigma, B->W2.p, B->GC.p);
+        lin(1.0, B->GC.p, -mu * th, B->U2n1.p, B->GC.p);
+        lin(1.0, B->GC.p, 1.0 / tau, B->W2.p, B->GC.p);
+        lin(1.0, B->GC.p, -alpha * alpha / tau, B->U2.p, B->GC.p);
+        lin(1.0, B->GC.p, alpha * alpha * mu1 * th, B->W2.p, B->GC.p);
+        lin(1.0, B->GC.p, -alpha * alpha * sigma1 * th, B->U2.p, B->GC.p);
+    }
+    hipLaunchKernelGGL(bv_copy_cols, gv, bb, 0, s, V, S, B->U1.p, B->U1n.p, B->dmask.p);
+    hipLaunchKernelGGL(bv_copy_cols, gv, bb, 0, s, V, S, B->U2.p, B->U2n.p, B->dmask.p);
+    hipLaunchKernelGGL(bv_copy_cols, gv, bb, 0, s, V, S, B->W1.p, B->W1n.p, B->dmask.p);
+    hipLaunchKernelGGL(bv_copy_cols, gv, bb, 0, s, V, S, B->W2.p, B->W2n.p, B->dmask.p);
+    hipLaunchKernelGGL(bc_pack4, g256((size_t)4 * n * S), bb, 0, s, n, S, B->mesh->d_inner.p,
+                       B->W1.p, B->W2.p, B->U1.p, B->U2.p, B->ANS.p, B->dmask.p);
+    PHG_HIP(hipGetLastError());
+    phg_gmres_opts jo = solve_opts(o.rtol_mass, PHG_CGS1, 0, o.max_solver_iters);
+    jo.restart = MASS_RESTART;
+    jo.diag_scale = 1;
+    phg_gmres_opts so = solve_opts(o.rtol_system, PHG_CGS2, 1, o.max_solver_iters);
+    const double ka = which == 0 ? 1.0 - th : th, kb = which == 0 ? th : 1.0 - th;
+    const int passes = which == 0 ? 1 : 10;
+    const int P = BC_PAIRS * S;
+    phg_matrix_t A4 = which == 0 ? B->A4 : B->Ab4;
+    BcPrecond* pc = o.precondition ? (which == 0 ? &B->pc : &B->pcb) : nullptr;
+    std::vector<phg_gmres_info> jinf((size_t)8 * BC_PAIRS), sinf(S);
+    std::vector<int> running = mask;
+    for (int pass = 1; pass <= passes; ++pass) {
+        for (int k = 0; k < S; ++k) {
+            if (!running[k]) continue;
+            for (int h = 0; h < 2; ++h) {
+                hipLaunchKernelGGL(bc_lin_pairs, g256(V * BC_PAIRS), bb, 0, s, (int)V, S, k, h, ka,
+                                   kb, B->U1.p, B->U1n.p, B->U2.p, B->U2n.p, B->W1.p, B->W1n.p,
+                                   B->W2.p, B->W2n.p, B->Z1.p, B->Z2.p, B->DZ1.p, B->DZ2.p,
+                                   B->lh.p, B->PU.p, B->PV.p);
+                PHG_HIP(hipGetLastError());
+                rc = jacobian_calc2_strided(B->J, BC_PAIRS, B->PU.p, B->PV.p, BC_PAIRS, 1,
+                                            (h == 0 ? B->JAC.p : B->JAC2.p) + BC_PAIRS * k, P, 1,
+                                            1.0, &jo, jinf.data(), s);
+                if (rc < 0) return rc;
+                for (int q = 0; q < 8 * BC_PAIRS; ++q) {
+                    it_jac[k] += jinf[q].iterations;
+                    if (!jinf[q].converged) failed[k] = 1;
+                }
+            }
+        }
+        hipLaunchKernelGGL(bc_lin_fg, g256((size_t)n * S), bb, 0, s, n, S, which,
+                           B->mesh->d_inner.p, B->JAC.p, B->JAC2.p, B->FC.p, B->GC.p, a2, B->F.p,
+                           B->G.p, B->dmask.p);
+        spmm(B, B->blk[5], B->F.p, B->RP.p, 0, s);
+        spmm(B, B->blk[5], B->G.p, B->RP.p + (size_t)n * S, 0, s);
+        hipLaunchKernelGGL(bv_zero_cols, g256((size_t)4 * n * S), bb, 0, s, (size_t)4 * n, S,
+                           B->RP.p, B->dmask.p);
+        hipLaunchKernelGGL(bv_zero_cols, g256((size_t)4 * n * S), bb, 0, s, (size_t)4 * n, S,
+                           B->ANS.p, B->dmask.p);
+        PHG_HIP(hipGetLastError());
+        rc = gmres_batch_solve_pc(A4, S, B->RP.p, B->ANS.p, &so, sinf.data(), s, pc);
+        if (rc < 0) return rc;
+        for (int k = 0; k < S; ++k)
+            if (running[k]) {
+                it_sys[k] += sinf[k].iterations;
+                if (!sinf[k].converged) failed[k] = 1;
+            }
+        const double* ans = B->ANS.p;
+        const size_t nb = (size_t)n * S;
+        double* outs[4] = {B->W1n.p, B->W2n.p, B->U1n1.p, B->U2n1.p};
+        for (int q = 0; q < 4; ++q)
+            hipLaunchKernelGGL(bv_p2u, gv, bb, 0, s, (int)V, S, B->mesh->d_isb.p,
+                               B->mesh->d_p2io.p, ans + q * nb, outs[q], B->dmask.p);
+        PHG_HIP(hipGetLastError());
+        std::vector<double> d1, d2;
+        if ((rc = dists(B, B->U1n1.p, B->U1n.p, d1, s))) return rc;
+        if ((rc = dists(B, B->U2n1.p, B->U2n.p, d2, s))) return rc;
+        hipLaunchKernelGGL(bv_copy_cols, gv, bb, 0, s, V, S, B->U1n1.p, B->U1n.p, B->dmask.p);
+        hipLaunchKernelGGL(bv_copy_cols, gv, bb, 0, s, V, S, B->U2n1.p, B->U2n.p, B->dmask.p);
+        PHG_HIP(hipGetLastError());
+        bool any = false;
+        for (int k = 0; k < S; ++k) {
+            if (!running[k]) continue;
+            iters[k] = pass;
+            dist[k] = std::max(d1[k], d2[k]);
+            if (dist[k] < 1e-8 || !std::isfinite(dist[k])) running[k] = 0;  // baroclin.cpp:978
+            any = any || running[k];
+        }
+        if (!any) break;
+        if ((rc = upload_mask(B, running, s))) return rc;
+    }
+    bool ok = true;
+    for (int k = 0; k < nm; ++k) {
+        hipLaunchKernelGGL(bv_col_out, g256(V), bb, 0, s, (int)V, S, k, B->U1n.p, d_u11 + k * V);
+        hipLaunchKernelGGL(bv_col_out, g256(V), bb, 0, s, (int)V, S, k, B->U2n.p, d_u21 + k * V);
+        ok = ok && !failed[k];
+        if (infos) {
+            infos[k].iterations = iters[k];
+            infos[k].distance = dist[k];
+            infos[k].converged = !failed[k] && (which == 0 || dist[k] < 1e-8);
+            infos[k].it_mass = it_mass[k];
+            infos[k].it_jacobian = it_jac[k];
+            infos[k].it_system = it_sys[k];
+        }
+    }
+    PHG_HIP(hipGetLastError());
+    PHG_HIP(hipStreamSynchronize(s));
+    return ok ? PHG_OK : PHG_ERR_NOCONV;
 }
 
 }  // namespace
@@ -402,7 +703,7 @@ int phg_baroclin_create(phg_mesh_t m, const phg_bc_conf* conf, const double* d_l
         int rc = assemble_device(m, kind, cm[k], cl[k], s, &B->blk[k]);
         if (rc) return rc;
     }
-    int rc = stack_blocks(B.get());
+    int rc = stack_blocks(B.get(), B->blk, &B->A4);
     if (rc) return rc;
     rc = assemble_boundary_device(m, PHG_LAPLACE, 1.0, 0.0, s, &B->bnd3);
     if (rc) return rc;
@@ -625,6 +926,39 @@ int phg_baroclin_calc(phg_baroclin_t B, int nmembers, const double* d_u1, const 
     PHG_HIP(hipGetLastError());
     PHG_HIP(hipStreamSynchronize(s));
     return all ? PHG_OK : PHG_ERR_NOCONV;
+}
+
+static int bc_linear_entry(phg_baroclin_t B, int which, int nmembers, const double* d_u1,
+                           const double* d_u2, const double* d_z1, const double* d_z2,
+                           double* d_u11, double* d_u21, const phg_bc_opts* opts,
+                           phg_bc_info* infos, phg_stream_t stream, const char* name)
+{
+    if (!B || !d_u1 || !d_u2 || !d_z1 || !d_z2 || !d_u11 || !d_u21 || nmembers < 1 ||
+        nmembers > B->S) {
+        set_error(std::string(name) + ": bad argument (1 <= nmembers <= slots)");
+        return PHG_ERR_ARG;
+    }
+    phg_bc_opts o;
+    if (opts) o = *opts;
+    else phg_bc_default_opts(&o);
+    return bc_linear(B, which, nmembers, d_u1, d_u2, d_z1, d_z2, d_u11, d_u21, o, infos,
+                     (hipStream_t)stream);
+}
+
+int phg_baroclin_calc_L(phg_baroclin_t B, int nmembers, const double* d_u1, const double* d_u2,
+                        const double* d_z1, const double* d_z2, double* d_u11, double* d_u21,
+                        const phg_bc_opts* opts, phg_bc_info* infos, phg_stream_t stream)
+{
+    return bc_linear_entry(B, 0, nmembers, d_u1, d_u2, d_z1, d_z2, d_u11, d_u21, opts, infos,
+                           stream, "phg_baroclin_calc_L");
+}
+
+int phg_baroclin_calc_L_1(phg_baroclin_t B, int nmembers, const double* d_u1, const double* d_u2,
+                          const double* d_z1, const double* d_z2, double* d_u11, double* d_u21,
+                          const phg_bc_opts* opts, phg_bc_info* infos, phg_stream_t stream)
+{
+    return bc_linear_entry(B, 1, nmembers, d_u1, d_u2, d_z1, d_z2, d_u11, d_u21, opts, infos,
+                           stream, "phg_baroclin_calc_L_1");
 }
 
 }  // extern "C"

@@ -112,6 +112,8 @@ SIGNATURES = {
     "phg_baroclin_slots": (_I, [_P]),
     "phg_baroclin_matrix": (_I, [_P, C.POINTER(_P)]),
     "phg_baroclin_calc": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "phg_baroclin_calc_L": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "phg_baroclin_calc_L_1": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "phg_serve_default_config": (None, [_P]),
     "phg_serve_create": (_I, [_P, C.POINTER(_P)]),
     "phg_serve_destroy": (None, [_P]),
@@ -727,6 +729,41 @@ class Baroclin:
         return [dict(iterations=i.iterations, distance=i.distance, converged=bool(i.converged),
                      it_mass=i.it_mass, it_jacobian=i.it_jacobian, it_system=i.it_system)
                 for i in infos]
+
+    def _lin(self, name, u11, u21, u1, u2, z1, z2, opts, stream):
+        k = 1 if u1.dim() == 1 else int(u1.shape[0])
+        for t in (u2, z1, z2, u11, u21):
+            if tuple(t.shape) != tuple(u1.shape):
+                raise PhelmError("Baroclin.%s: arguments must have one shape" % name)
+        infos = (BcInfo * k)()
+        rc = getattr(lib(), "phg_baroclin_" + name)(
+            self.h, k, _ptr(u1), _ptr(u2), _ptr(z1), _ptr(z2), _ptr(u11), _ptr(u21),
+            C.byref(opts) if opts is not None else None, infos, _stream(stream))
+        check(rc, "phg_baroclin_" + name)
+        return [dict(iterations=i.iterations, distance=i.distance, converged=bool(i.converged),
+                     it_mass=i.it_mass, it_jacobian=i.it_jacobian, it_system=i.it_system)
+                for i in infos]
+
+    def calc_L(self, u11, u21, u1, u2, z1, z2, opts=None, stream=None):
+        """Baroclin::calc_L (examples/baroclin.cpp:607-802): the step
+        linearised about (z1, z2), one pass with A_."""
+        return self._lin("calc_L", u11, u21, u1, u2, z1, z2, opts, stream)
+
+    def calc_L_1(self, u11, u21, u1, u2, z1, z2, opts=None, stream=None):
+        """Baroclin::calc_L_1 (examples/baroclin.cpp:804-986): up to 10 passes
+        with Ab_ until the distance is below 1e-8."""
+        return self._lin("calc_L_1", u11, u21, u1, u2, z1, z2, opts, stream)
+
+    def L_step(self, u11, u21, u1, u2, z1, z2, opts=None, stream=None):
+        return self.calc_L(u11, u21, u1, u2, z1, z2, opts, stream)
+
+    def L_1_step(self, u11, u21, u1, u2, z1, z2, opts=None, stream=None):
+        return self.calc_L_1(u11, u21, u1, u2, z1, z2, opts, stream)
+
+    def LT_step(self, u11, u21, u1, u2, z1, z2, opts=None, stream=None):
+        """The reference's calc_LT has an empty body (examples/baroclin.cpp:
+        988-993): the outputs stay as they are."""
+        return []
 
     def close(self):
         if self.h:

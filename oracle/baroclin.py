@@ -40,6 +40,7 @@ class Baroclin:
         brp, bci, bv = O.baroclin_matrix(rp, ci, v9)
         self.A = (brp, bci, bv)
         self.lu_A = spla.splu(sp.csr_matrix((bv, bci, brp), shape=(4 * n, 4 * n)).tocsc())
+        self._lu_Ab = None
         # SphereLaplace::calc1, SphereJacobian and SphereNorm of the same mesh
         self.bv = BarVortex(mesh, tau=1.0, sigma=0.0, mu=0.0, k1=0.0, k2=0.0, theta=0.5,
                             lh=np.zeros(len(mesh.p2io)))
@@ -111,6 +112,100 @@ class Baroclin:
             rp[:rs] = O.right_part_mass(self.m, F)
             rp[rs:2 * rs] = O.right_part_mass(self.m, G)
             ans = self.lu_A.solve(rp)
+            w1_n = self.bv.p2u(ans[:rs])
+            w2_n = self.bv.p2u(ans[rs:2 * rs])
+            u1_n1 = self.bv.p2u(ans[2 * rs:3 * rs])
+            u2_n1 = self.bv.p2u(ans[3 * rs:])
+            nr = max(self.dist(u1_n1, u1_n), self.dist(u2_n1, u2_n))
+            u1_n, u2_n = u1_n1, u2_n1
+            if nr < tol:
+                break
+        return u1_n, u2_n, it, nr
+
+
+    # ---- linearised steps (examples/baroclin.cpp:607-986), bnd = 0 ----------
+    @property
+    def lu_Ab(self):
+        """Ab_ (integrate_backward_cb, baroclin.cpp:184-244), factorised."""
+        if self._lu_Ab is None:
+            n = self.n
+            rp, ci, v9 = O.baroclin_blocks(self.m, self.tau, self.sigma, self.mu, self.sigma1,
+                                           self.mu1, self.alpha, self.theta, backward=True)
+            self.Ab_blocks = (rp, ci, v9)
+            brp, bci, bv = O.baroclin_matrix(rp, ci, v9)
+            self.Ab = (brp, bci, bv)
+            self._lu_Ab = spla.splu(sp.csr_matrix((bv, bci, brp), shape=(4 * n, 4 * n)).tocsc())
+        return self._lu_Ab
+
+    def calc_lin(self, which, u1, u2, z1, z2, tol=1e-8):
+        """Baroclin::calc_L (which 0, baroclin.cpp:620-802: one pass with A_)
+        or calc_L_1 (which 1, :804-986: at most 10 passes with Ab_ until the
+        distance is below tol).  The Jacobian arguments are the code's,
+        including J(L z1, avg w2) and J(L z2, avg u1) in G where the comments
+        write J(z1, .) and J(., L z2).  Returns (u11, u21, passes, distance)."""
+        th, tau = self.theta, self.tau
+        sigma, mu, sigma1, mu1, alpha = self.sigma, self.mu, self.sigma1, self.mu1, self.alpha
+        a2 = alpha * alpha
+        inner, rs = self.m.inner, self.n
+        f64 = lambda a: np.asarray(a, dtype=np.float64)  # noqa: E731
+        u1, u2, z1, z2 = f64(u1), f64(u2), f64(z1), f64(z2)
+        s1 = lambda a, b, k1, k2: k1 * a + k2 * b  # noqa: E731  vec_sum1
+        w1, w2 = self.calc1(u1), self.calc1(u2)
+        dw1, dw2 = self.calc1(w1), self.calc1(w2)
+        dz1, dz2 = self.calc1(z1), self.calc1(z2)
+        if which == 0:
+            FC = s1(w1, w2, -0.5 * (1.0 - th) * sigma, 0.5 * (1.0 - th) * sigma)
+            FC = s1(FC, dw1, 1.0, mu * (1.0 - th))
+            FC = s1(FC, w1, 1.0, 1.0 / tau)
+            GC = s1(w1, w2, -0.5 * (1.0 - th) * sigma, -0.5 * (1.0 - th) * sigma)
+            GC = s1(GC, dw2, 1.0, mu * (1.0 - th))
+            GC = s1(GC, w2, 1.0, 1.0 / tau)
+            GC = s1(GC, u2, 1.0, -alpha * alpha / tau)
+            GC = s1(GC, w2, 1.0, -alpha * alpha * mu1 * (1 - th))
+            GC = s1(GC, u2, 1.0, alpha * alpha * sigma1 * (1 - th))
+            ka, kb, lu, passes = 1.0 - th, th, self.lu_A, 1
+        else:
+            FC = s1(w1, w2, 0.5 * th * sigma, -0.5 * th * sigma)
+            FC = s1(FC, dw1, 1.0, -mu * th)
+            FC = s1(FC, w1, 1.0, 1.0 / tau)
+            GC = s1(w1, w2, 0.5 * th * sigma, 0.5 * th * sigma)
+            GC = s1(GC, dw2, 1.0, -mu * th)
+            GC = s1(GC, w2, 1.0, 1.0 / tau)
+            GC = s1(GC, u2, 1.0, -alpha * alpha / tau)
+            GC = s1(GC, w2, 1.0, alpha * alpha * mu1 * th)
+            GC = s1(GC, u2, 1.0, -alpha * alpha * sigma1 * th)
+            ka, kb, lu, passes = th, 1.0 - th, self.lu_Ab, 10
+        u1_n, u2_n, w1_n, w2_n = u1.copy(), u2.copy(), w1.copy(), w2.copy()
+        dz1lh = dz1 + self.lh                                     # vec_sum
+        J = self.jac
+        nr, it = float("nan"), 0
+        for it in range(1, passes + 1):
+            mu1_, mu2_ = s1(u1, u1_n, ka, kb), s1(u2, u2_n, ka, kb)
+            mw1_, mw2_ = s1(w1, w1_n, ka, kb), s1(w2, w2_n, ka, kb)
+            if which == 0:
+                F = s1(J(mu1_, dz1lh), J(z1, mw1_), -1.0, -1.0)
+                F = s1(F, J(z2, mw2_), 1.0, -1.0)
+                F = s1(F, J(mu2_, dz2), 1.0, -1.0)
+                G = s1(J(dz1, mw2_), J(dz2, mu1_), -1.0, -1.0)
+                G = s1(G, J(mu2_, dz1lh), 1.0, -1.0)
+                G = s1(G, J(z2, mw1_), 1.0, -1.0)
+                G = s1(G, J(z1, mu2_), 1.0, a2)
+                G = s1(G, J(mu1_, z2), 1.0, a2)
+            else:
+                F = J(mu1_, dz1lh) + J(z1, mw1_)                   # vec_sum
+                F = F + J(z2, mw2_)
+                F = F + J(mu2_, dz2)
+                G = J(dz1, mw2_) + J(dz2, mu1_)
+                G = G + J(mu2_, dz1lh)
+                G = G + J(z2, mw1_)
+                G = s1(G, J(z1, mu2_), 1.0, -a2)
+                G = s1(G, J(mu1_, z2), 1.0, -a2)
+            F = F + FC[inner]
+            G = G + GC[inner]
+            rp = np.zeros(4 * rs)
+            rp[:rs] = O.right_part_mass(self.m, F)
+            rp[rs:2 * rs] = O.right_part_mass(self.m, G)
+            ans = lu.solve(rp)
             w1_n = self.bv.p2u(ans[:rs])
             w2_n = self.bv.p2u(ans[rs:2 * rs])
             u1_n1 = self.bv.p2u(ans[2 * rs:3 * rs])

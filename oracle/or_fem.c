@@ -598,12 +598,14 @@ int or_assemble(const or_mesh *m, int kind, double c_mass, double c_lap,
  * nine n x n blocks with the inner stencil; vals9[k * nnz + e] is entry e of
  * block k in the order (1,1) (1,2) (2,1) (2,2) (2,4) (3,1) (3,3) (4,2) (4,4)
  * of the reference's push_back calls.  p = {tau, sigma, mu, sigma1, mu1,
- * alpha, theta}. */
+ * alpha, theta, backward}: backward != 0 gives Ab_ instead, from
+ * integrate_backward_cb (examples/baroclin.cpp:184-244), same block order. */
 int or_assemble_baroclin(const or_mesh *m, const double *p, const int *row_ptr,
                          const int *col_idx, double *vals9)
 {
     const double tau = p[0], sigma = p[1], mu = p[2], sigma1 = p[3], mu1 = p[4],
                  alpha = p[5], theta = p[6];
+    const int backward = p[7] != 0.0;
     const long nnz = row_ptr[m->n_inner];
     int rs = m->n_inner;
 #pragma omp parallel for schedule(static)
@@ -626,12 +628,22 @@ int or_assemble_baroclin(const or_mesh *m, const double *p, const int *row_ptr,
                 double a = or_element(0, phi_i, &phik[i0], xs, ys, 0.0, 0.0);
                 double b = or_element(1, phi_i, &phik[i0], xs, ys, 0.0, 0.0);
                 double v[9];
-                v[0] = a * (1.0 / tau + 0.5 * theta * sigma) - theta * mu * b;
-                v[1] = -0.5 * a * sigma * theta;
-                v[2] = 0.5 * theta * sigma * a;
-                v[3] = a * (1.0 / tau + 0.5 * theta * sigma + alpha * alpha * theta * mu1)
-                       - theta * mu * b;
-                v[4] = a * (-alpha * alpha / tau - alpha * alpha * theta * sigma1);
+                if (!backward) {
+                    v[0] = a * (1.0 / tau + 0.5 * theta * sigma) - theta * mu * b;
+                    v[1] = -0.5 * a * sigma * theta;
+                    v[2] = 0.5 * theta * sigma * a;
+                    v[3] = a * (1.0 / tau + 0.5 * theta * sigma + alpha * alpha * theta * mu1)
+                           - theta * mu * b;
+                    v[4] = a * (-alpha * alpha / tau - alpha * alpha * theta * sigma1);
+                } else {
+                    v[0] = a * (1.0 / tau - 0.5 * (1.0 - theta) * sigma) + (1.0 - theta) * mu * b;
+                    v[1] = 0.5 * a * sigma * (1.0 - theta);
+                    v[2] = -0.5 * (1.0 - theta) * sigma * a;
+                    v[3] = a * (1.0 / tau - 0.5 * (1.0 - theta) * sigma
+                                - alpha * alpha * (1.0 - theta) * mu1)
+                           + (1.0 - theta) * mu * b;
+                    v[4] = a * (-alpha * alpha / tau + alpha * alpha * (1.0 - theta) * sigma1);
+                }
                 v[5] = a;
                 v[6] = -b;
                 v[7] = a;

@@ -110,8 +110,9 @@ long or_csr_structure(const or_mesh *m, int *row_ptr, int *col_idx);
  * adjacency order (src/generators.cpp:57-114).  Uses OpenMP over rows. */
 int or_assemble(const or_mesh *m, int kind, double c_mass, double c_lap,
                 const int *row_ptr, const int *col_idx, double *vals);
-/* Baroclin's A_ as nine blocks (examples/baroclin.cpp:69-181); p = {tau,
- * sigma, mu, sigma1, mu1, alpha, theta}; vals9: 9 x nnz. */
+/* Baroclin's A_ (or Ab_ when backward != 0) as nine blocks
+ * (examples/baroclin.cpp:69-244); p = {tau, sigma, mu, sigma1, mu1, alpha,
+ * theta, backward}; vals9: 9 x nnz. */
 int or_assemble_baroclin(const or_mesh *m, const double *p, const int *row_ptr,
                          const int *col_idx, double *vals9);
 /* generate_right_part with the interior branch of Baroclin's right_part_cb

@@ -209,13 +209,15 @@ class Mesh:
         return rp, ci, vals[:len(ci)]
 
 
-def baroclin_blocks(mesh, tau, sigma, mu, sigma1, mu1, alpha, theta=0.5):
-    """The nine n x n blocks of Baroclin's A_ (examples/baroclin.cpp:115-181)
-    on the inner stencil: (rp, ci, vals9) with vals9[k] the values of block
-    k in the order (1,1) (1,2) (2,1) (2,2) (2,4) (3,1) (3,3) (4,2) (4,4)."""
+def baroclin_blocks(mesh, tau, sigma, mu, sigma1, mu1, alpha, theta=0.5, backward=False):
+    """The nine n x n blocks of Baroclin's A_ (examples/baroclin.cpp:115-181),
+    or of Ab_ with backward (integrate_backward_cb, :184-244), on the inner
+    stencil: (rp, ci, vals9) with vals9[k] the values of block k in the order
+    (1,1) (1,2) (2,1) (2,2) (2,4) (3,1) (3,3) (4,2) (4,4)."""
     rp, ci = mesh.csr_structure()
     vals = np.zeros(9 * len(ci))
-    prm = np.array([tau, sigma, mu, sigma1, mu1, alpha, theta], dtype=np.float64)
+    prm = np.array([tau, sigma, mu, sigma1, mu1, alpha, theta, 1.0 if backward else 0.0],
+                   dtype=np.float64)
     lib().or_assemble_baroclin(C.byref(mesh.c), _pd(prm), _pi(rp), _pi(ci), _pd(vals))
     return rp, ci, vals.reshape(9, len(ci))
 

@@ -13,7 +13,9 @@
 // t = 0.5, zero boundary arrays passed), bv_L.bin / bv_L1.bin / bv_LT.bin
 // (L_step, L_1_step of bv_L, LT_step about psi), bc_u11.bin / bc_u21.bin (one
 // Baroclin::calc step of (u, v) with the forcing written to bc_f.bin,
-// bc_g.bin and the Coriolis term bc_lh.bin), inputs u.bin, v.bin, x.bin,
+// bc_g.bin and the Coriolis term bc_lh.bin; bc_L1/2.bin = L_step of (u, v)
+// about (psi, u), bc_B1/2.bin = L_1_step of that, bc_LT1.bin = bc_L1 after
+// LT_step), inputs u.bin, v.bin, x.bin,
 // psi.bin and summary.txt (scalars).
 #include <cmath>
 #include <cstdio>
@@ -238,6 +240,17 @@ int main(int argc, char** argv)
         dump(out, "bc_u11.bin", from_dev(b1));
         dump(out, "bc_u21.bin", from_dev(b2));
         const phg_bc_info bci = bc.last_info();
+        // linearised steps about (psi, u): L_step of (u, v), L_1_step of that;
+        // LT_step leaves its outputs as they are (empty in the reference)
+        ArrayDevice<double> l1((size_t)V), l2((size_t)V), k1((size_t)V), k2((size_t)V);
+        bc.L_step(l1.data(), l2.data(), du.data(), dv.data(), dpsi.data(), du.data());
+        bc.L_1_step(k1.data(), k2.data(), l1.data(), l2.data(), dpsi.data(), du.data());
+        dump(out, "bc_L1.bin", from_dev(l1));
+        dump(out, "bc_L2.bin", from_dev(l2));
+        dump(out, "bc_B1.bin", from_dev(k1));
+        dump(out, "bc_B2.bin", from_dev(k2));
+        bc.LT_step(l1.data(), l2.data(), du.data(), dv.data(), dpsi.data(), du.data());
+        dump(out, "bc_LT1.bin", from_dev(l1));
 
         const std::string sp = out + "/summary.txt";
         FILE* f = std::fopen(sp.c_str(), "w");

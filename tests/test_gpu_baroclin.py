@@ -8,7 +8,10 @@ its forcing) within 1e-9 of the oracle step (SphereNorm distance relative to
 the state) with the block preconditioner, within 1e-6 without it (the plain
 solve's forward error at its 1e-12 residual); a member's result
 bit-identical alone and beside another member; the manufactured-solution
-tendency error falls with refinement as the oracle's does."""
+tendency error falls with refinement as the oracle's does; the linearised
+steps calc_L (within 1e-9 of the oracle's) and calc_L_1 (within 1e-8: up to
+10 fixed-point passes, stopping at a distance of 1e-8), member bits
+independent of the batch, and L_1(L(u)) = u without the Jacobian terms."""
 import numpy as np
 import pytest
 
@@ -114,3 +117,65 @@ def test_baroclin_manufactured_tendency_converges():
     # 0.037 (layer 2) at levels 3 / 4 / 5
     assert errs[1][0] < errs[0][0] / 2.5 and errs[2][0] < errs[1][0] / 2.5, errs
     assert errs[1][1] < errs[0][1] / 1.8 and errs[2][1] < errs[1][1] / 1.8, errs
+
+
+def _lin_fields(pm, om, seeds):
+    pr = pm.arrays()["pr"]
+    x, y, z = pr[:, 0], pr[:, 1], pr[:, 2]
+    out = []
+    for sd in seeds:
+        r = np.random.default_rng(sd)
+        f = (r.standard_normal() * x * y + r.standard_normal() * (x * x - y * y) * z
+             + r.standard_normal() * np.sin(2 * x + y) * z + 0.3 * r.standard_normal() * y)
+        out.append(f - f[om.outer[0]])
+    return np.stack(out), 2.0 * np.clip(z, -1, 1)
+
+
+def _rel2(ob, a1, a2, r1, r2):
+    return float(np.hypot(ob.dist(a1, r1), ob.dist(a2, r2))
+                 / np.hypot(ob.bv.norm(r1), ob.bv.norm(r2)))
+
+
+def test_baroclin_linear_steps_vs_oracle():
+    print("baroclin calc_L / calc_L_1 level=3", flush=True)
+    pm, om, xy = _setup(3)
+    U1, lh = _lin_fields(pm, om, (1, 2))
+    U2, _ = _lin_fields(pm, om, (3, 4))
+    Z1, _ = _lin_fields(pm, om, (5, 6))
+    Z2, _ = _lin_fields(pm, om, (7, 8))
+    tau = 2.5e-4
+    bc = P.Baroclin(pm, _dev(lh), tau=tau, slots=2, **PRM)
+    ob = OB(om, tau=tau, lh=lh, **PRM)
+    L1, L2 = torch.zeros_like(_dev(U1)), torch.zeros_like(_dev(U2))
+    for i in bc.calc_L(L1, L2, _dev(U1), _dev(U2), _dev(Z1), _dev(Z2)):
+        assert i["converged"] and i["iterations"] == 1 and i["it_jacobian"] > 0, i
+    B1, B2 = torch.zeros_like(L1), torch.zeros_like(L2)
+    inf_b = bc.calc_L_1(B1, B2, L1, L2, _dev(Z1), _dev(Z2))
+    g = [t.cpu().numpy() for t in (L1, L2, B1, B2)]
+    for q in range(2):
+        r1, r2, _, _ = ob.calc_lin(0, U1[q], U2[q], Z1[q], Z2[q])
+        assert _rel2(ob, g[0][q], g[1][q], r1, r2) <= 1e-9, q
+        r1, r2, it, nr = ob.calc_lin(1, g[0][q], g[1][q], Z1[q], Z2[q])
+        assert abs(inf_b[q]["iterations"] - it) <= 1, (inf_b[q], it)
+        assert _rel2(ob, g[2][q], g[3][q], r1, r2) <= 1e-8, q
+        assert _rel2(ob, g[2][q], g[3][q], U1[q], U2[q]) <= 3e-4   # inverts to O(tau^2)
+    # member 1 alone (slot 0 of the same engine) gives the same bits
+    a1, a2 = torch.zeros_like(L1[:1]), torch.zeros_like(L2[:1])
+    bc.calc_L(a1, a2, _dev(U1[1:]), _dev(U2[1:]), _dev(Z1[1:]), _dev(Z2[1:]))
+    assert torch.equal(a1[0], L1[1]) and torch.equal(a2[0], L2[1])
+
+
+def test_baroclin_inverse_step_without_jacobian():
+    print("baroclin L_1(L(u)) level=3", flush=True)
+    pm, om, xy = _setup(3)
+    U1, _ = _lin_fields(pm, om, (1,))
+    U2, _ = _lin_fields(pm, om, (2,))
+    zero = np.zeros((1, pm.n_points))
+    bc = P.Baroclin(pm, _dev(zero[0]), tau=1e-2, slots=1, **PRM)
+    ob = OB(om, tau=1e-2, lh=zero[0], **PRM)
+    L1, L2, B1, B2 = (torch.zeros_like(_dev(U1)) for _ in range(4))
+    bc.L_step(L1, L2, _dev(U1), _dev(U2), _dev(zero), _dev(zero))
+    bc.L_1_step(B1, B2, L1, L2, _dev(zero), _dev(zero))
+    g = [t.cpu().numpy()[0] for t in (L1, B1, B2)]
+    assert ob.dist(g[0], U1[0]) > 1e-3 * ob.bv.norm(U1[0])
+    assert _rel2(ob, g[1], g[2], U1[0], U2[0]) <= 1e-9

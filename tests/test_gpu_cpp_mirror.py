@@ -184,3 +184,11 @@ def test_baroclin_equals_python(run):
     assert info["iterations"] == summ["bc_iterations"]
     assert np.array_equal(o1.cpu().numpy()[0], load("bc_u11.bin"))
     assert np.array_equal(o2.cpu().numpy()[0], load("bc_u21.bin"))
+    # L_step of (u, v) about (psi, u), L_1_step of that; LT_step changes nothing
+    psi = torch.as_tensor(load("psi.bin"), device=DEV).reshape(1, -1)
+    l1, l2, k1, k2 = (torch.zeros_like(u1) for _ in range(4))
+    bc.L_step(l1, l2, u1, u2, psi, u1)
+    bc.L_1_step(k1, k2, l1, l2, psi, u1)
+    for t, name in ((l1, "bc_L1.bin"), (l2, "bc_L2.bin"), (k1, "bc_B1.bin"), (k2, "bc_B2.bin")):
+        assert np.array_equal(t.cpu().numpy()[0], load(name)), name
+    assert np.array_equal(load("bc_LT1.bin"), load("bc_L1.bin"))

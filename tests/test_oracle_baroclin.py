@@ -9,7 +9,16 @@ examples/baroclin.cpp:69-181, 445-606) on the CPU:
   tau approaches the exact one as the mesh is refined, for both layers;
 * a mistake planted in a copy of the oracle (the alpha^2 J(u1, u2) term of
   the second layer dropped) breaks that convergence, so the check tells such
-  mistakes apart.
+  mistakes apart;
+* the linearised steps calc_L / calc_L_1 (baroclin.cpp:607-986): Ab_'s blocks
+  are integrate_backward_cb's mass/stiffness combinations; without Coriolis
+  and about z = 0 (no Jacobian terms) calc_L is A_^-1 Ab_ and calc_L_1 is
+  Ab_^-1 A_ on consistent states, so L_1(L(u)) = u exactly; the Jacobian part
+  of calc_L equals a central difference of the nonlinear step about z to
+  O(tau) in the two subspaces where the code's J(L z1, avg w2) and
+  J(L z2, avg u1) (the comments write J(z1, .) and J(., L z2)) vanish:
+  (u1, z1) with u2 = z2 = 0, and (u2, z2) with u1 = z1 = 0 (linearising
+  about -z misses by 2x); with the Jacobian terms L_1(L(u)) = u + O(tau^2).
 """
 import numpy as np
 import pytest
@@ -97,3 +106,86 @@ class _NoG(Baroclin):
 def test_planted_mistake_breaks_convergence(wrong):
     errs = _tendency_errors(wrong, (3, 4))
     assert not _converges(errs), errs
+
+
+def test_backward_blocks_are_the_mass_stiffness_combinations():
+    M, _ = _mesh(3)
+    tau, th = TAU, 0.5
+    s, mu, s1, mu1, al = PRM["sigma"], PRM["mu"], PRM["sigma1"], PRM["mu1"], PRM["alpha"]
+    rp, ci, v9 = O.baroclin_blocks(M, tau, s, mu, s1, mu1, al, th, backward=True)
+    m = M.assemble(O.KIND_MASS)[2]
+    lap = M.assemble(O.KIND_LAPLACE)[2]
+    b = 1.0 - th
+    want = [(1 / tau - 0.5 * b * s) * m + b * mu * lap, 0.5 * s * b * m, -0.5 * b * s * m,
+            (1 / tau - 0.5 * b * s - al * al * b * mu1) * m + b * mu * lap,
+            (-al * al / tau + al * al * b * s1) * m, m, -lap, m, -lap]
+    for k in range(9):
+        assert np.abs(v9[k] - want[k]).max() <= 1e-15 * np.abs(want[k]).max(), k
+
+
+def _lin_fields(level):
+    a = mesh_ref.load_text(mesh_ref.sphere_mesh_text(level))
+    M = O.Mesh(a)
+    pr = np.asarray(a["pr"]).reshape(-1, 3)
+    x, y, z = pr[:, 0], pr[:, 1], pr[:, 2]
+
+    def field(seed):
+        r = np.random.default_rng(seed)
+        f = (r.standard_normal() * x * y + r.standard_normal() * (x * x - y * y) * z
+             + r.standard_normal() * np.sin(2 * x + y) * z + 0.3 * r.standard_normal() * y)
+        return f - f[M.outer[0]]
+
+    return M, field, 2.0 * np.clip(z, -1, 1)
+
+
+def _rel(bc, a, b, ref):
+    return float(np.hypot(bc.dist(a[0], b[0]), bc.dist(a[1], b[1]))
+                 / np.hypot(bc.bv.norm(ref[0]), bc.bv.norm(ref[1])))
+
+
+def test_linear_steps_without_jacobian_are_inverse():
+    M, field, lh = _lin_fields(3)
+    zero = np.zeros(len(lh))
+    bc = Baroclin(M, tau=1e-2, lh=zero, **PRM)
+    u = (field(1), field(2))
+    L = bc.calc_lin(0, u[0], u[1], zero, zero)[:2]
+    assert _rel(bc, L, u, u) > 1e-3                        # the step moves the state
+    back = bc.calc_lin(1, L[0], L[1], zero, zero)
+    assert back[2] == 2 and back[3] == 0.0                 # linear: the second pass repeats
+    assert _rel(bc, back[:2], u, u) <= 1e-12
+
+
+def test_linear_step_jacobian_part_is_the_derivative():
+    M, field, lh = _lin_fields(3)
+    zero = np.zeros(len(lh))
+    u1, u2, z1, z2 = field(1), field(2), field(3), field(4)
+    for du, zz in (((u1, zero), (z1, zero)), ((zero, u2), (zero, z2))):
+        rel = []
+        for tau in (4e-3, 1e-3):
+            bc = Baroclin(M, tau=tau, lh=lh, **PRM)
+            b0 = Baroclin(M, tau=tau, lh=zero, **PRM)
+            eps = 1e-5
+            sp = bc.calc(zz[0] + eps * du[0], zz[1] + eps * du[1], max_iters=100, tol=1e-15)
+            sm = bc.calc(zz[0] - eps * du[0], zz[1] - eps * du[1], max_iters=100, tol=1e-15)
+            D = [(sp[k] - sm[k]) / (2 * eps) for k in (0, 1)]
+            L = bc.calc_lin(0, du[0], du[1], zz[0], zz[1])[:2]
+            P = b0.calc_lin(0, du[0], du[1], zero, zero)[:2]
+            jl = [L[k] - P[k] for k in (0, 1)]
+            jd = [D[k] - P[k] for k in (0, 1)]
+            rel.append(_rel(bc, jl, jd, jl))
+            bad = bc.calc_lin(0, du[0], du[1], -zz[0], -zz[1])[:2]
+            assert _rel(bc, [bad[k] - P[k] for k in (0, 1)], jd, jl) > 1.5
+        assert rel[0] < 0.1 and rel[1] < 0.03 and rel[1] < rel[0] / 3.0, rel
+
+
+def test_inverse_step_inverts_to_second_order():
+    M, field, lh = _lin_fields(3)
+    u = (field(1), field(2))
+    z1, z2 = field(3), field(4)
+    errs = []
+    for tau in (1e-3, 2.5e-4):
+        bc = Baroclin(M, tau=tau, lh=lh, **PRM)
+        L = bc.calc_lin(0, u[0], u[1], z1, z2)[:2]
+        back = bc.calc_lin(1, L[0], L[1], z1, z2)
+        errs.append(_rel(bc, back[:2], u, u))
+    assert errs[1] < errs[0] / 10.0 and errs[1] < 1e-4, errs
