@@ -237,3 +237,25 @@ def test_gmres_graph_replay_matches_eager_and_repeats(ortho):
         assert info["converged"]
         xs.append(x.cpu().numpy())
     assert np.array_equal(xs[0], xs[1]) and np.array_equal(xs[1], xs[2])
+
+
+@pytest.mark.parametrize("op,kind,cm,cl", [("slaplace", O.KIND_LAPLACE, 1.0, 0.0),
+                                           ("helmholtz:0.01", O.KIND_HELMHOLTZ, CM, CL)])
+def test_dump_operator_matches_oracle(op, kind, cm, cl, tmp_path):
+    """python -m mke_amd.dump_operator (utils/dump_operator_matrix.cpp) on a
+    level-3 mesh file: the Matrix Market text holds the oracle's CSR, FP64
+    values exactly with --double."""
+    print("dump_operator %s" % op, flush=True)
+    sio = pytest.importorskip("scipy.io")
+    from mke_amd import dump_operator
+    text, _ = oracle_mesh(3)
+    mp = tmp_path / "ss3.txt"
+    mp.write_text(text)
+    out = tmp_path / "op.mtx"
+    assert dump_operator.main(["--double", "--mesh", str(mp), "--operator", op, "-o",
+                               str(out)]) == 0
+    got = sio.mmread(str(out)).tocsr()
+    got.sort_indices()
+    rp, ci, v = oracle_matrix(3, kind, cm, cl)
+    assert np.array_equal(got.indptr, rp) and np.array_equal(got.indices, ci)
+    assert np.array_equal(got.data, v)

@@ -93,3 +93,33 @@ def test_print_function_format(sphere_xyz):
                                    rtol=0, atol=2e-16)
     else:
         assert got == want
+
+
+def test_matrix_market_writer(tmp_path):
+    """dump_operator's writer (utils/dump_operator_matrix.cpp counterpart):
+    the Matrix Market text reads back as the same matrix, FP64 exactly with
+    --double and rounded to float without; ELL order lists slot k of every
+    row before slot k + 1."""
+    sio = pytest.importorskip("scipy.io")
+    from mke_amd.dump_operator import write_matrix_market
+    rp = np.array([0, 2, 5, 6], np.int32)
+    ci = np.array([0, 2, 0, 1, 2, 1], np.int32)
+    v = np.array([1.0 / 3.0, -2.5e-17, math.pi, 4.0, -1e30, 0.1])
+    dense = np.zeros((3, 3))
+    for i in range(3):
+        dense[i, ci[rp[i]:rp[i + 1]]] = v[rp[i]:rp[i + 1]]
+    for double, store in ((True, "CSR"), (False, "CSR"), (True, "ELL")):
+        p = tmp_path / ("m_%d_%s.mtx" % (double, store))
+        with open(p, "w") as f:
+            write_matrix_market(f, rp, ci, v, 3, double, store)
+        got = sio.mmread(str(p)).toarray()
+        if double:
+            assert np.array_equal(got, dense), store
+        else:   # 9 significant digits name the float exactly
+            assert np.array_equal(got.astype(np.float32), dense.astype(np.float32))
+            assert not np.array_equal(got, dense)
+        lines = open(p).read().split("\n")
+        assert lines[1] == "3 3 6"
+        if store == "ELL":   # slot 0 of rows 1..3, then slot 1, then slot 2
+            pos = [tuple(map(int, ln.split()[:2])) for ln in lines[2:8]]
+            assert pos == [(1, 1), (2, 1), (3, 2), (1, 3), (2, 2), (2, 3)], pos
