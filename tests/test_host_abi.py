@@ -84,3 +84,27 @@ def test_cpp_host_classes_compile():
     r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "cpp"), "syntax"],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_argument_errors_without_a_gpu():
+    """Entry points check their arguments before any device work and report
+    PHG_ERR_ARG with a message, as the C++ classes then throw."""
+    import ctypes as C
+    L = P.lib()
+    ERR_ARG = -1
+    for name, args in (
+        ("phg_barvortex_calc_L", (None, 1, None, None, None, None, None, None)),
+        ("phg_barvortex_calc_L_1", (None, 1, None, None, None, None, None, None)),
+        ("phg_barvortex_calc_LT", (None, 1, None, None, None, None, None, None)),
+        ("phg_barvortex_calc_rp", (None, 1, None, None, None, None, None, None)),
+        ("phg_baroclin_calc_L", (None, 1) + (None,) * 9),
+        ("phg_baroclin_calc_L_1", (None, 1) + (None,) * 9),
+    ):
+        assert getattr(L, name)(*args) == ERR_ARG, name
+        assert L.phg_last_error().decode(), name
+    m = P.Mesh.sphere(2)
+    conf = phelm.BvConf(1.0, 0.0, 0.0, 1.0, 1.0, 0.5)
+    h = C.c_void_p()
+    lh = C.c_void_p(1)  # never dereferenced: the slot count is refused first
+    assert L.phg_barvortex_create(m.h, C.byref(conf), lh, 3, None, C.byref(h)) == ERR_ARG
+    assert "slots" in L.phg_last_error().decode()
