@@ -580,7 +580,7 @@ def main():
             Xk = torch.zeros_like(Bk)
             mg.solve_batch(Xk, Bk, rtol=1e-10, precision=prec, stream=stream)  # warmup
             torch.cuda.synchronize()
-            reps = max(3, args.steps // (1 if k == 1 else 5))
+            reps = max(10, args.steps // (1 if k == 1 else 2))
             lat_k = []
             t0 = time.perf_counter()
             for _ in range(reps):

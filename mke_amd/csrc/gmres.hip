@@ -17,7 +17,8 @@
 // pass (CGS2: the j+1 dots of a pass are computed together).
 //
 // Kernels per Arnoldi step j: MGS j+2 (SpMV, j MGS passes, final pass with
-// the norm); CGS2 5.  The Givens rotation of step j-1 and the decision to
+// the norm); CGS2 3 (SpMV with the first-pass dots, and two update kernels,
+// each of which first adds the previous kernel's dot partials itself).  The Givens rotation of step j-1 and the decision to
 // continue are computed redundantly by every block at the start of the SpMV
 // kernel of step j (all blocks read the same inputs and agree); block 0
 // records them.  A whole restart cycle is captured once in a hipGraph and
@@ -56,7 +57,7 @@ struct GmresWork {
     int n = 0, m = 0, G = 0, rpt = 1, prec = 0;
     size_t ldv = 0;
     DevBuf<char> V, W;     // (m+1) x ldv and 2 x ldv vectors of the Krylov type
-    DevBuf<double> part;   // (m+2) x G partial sums
+    DevBuf<double> part;   // 2 x (m+2) x G partial sums (two regions for CGS2)
     DevBuf<GmresState> st;
     GmresState* host = nullptr;  // pinned mirror
     // captured restart cycle
@@ -431,41 +432,68 @@ k_mgs(int n, int rpt, VT* __restrict__ w, const VT* __restrict__ V, size_t ldv, 
     if (threadIdx.x == 0) part[(size_t)i * G + lblock()] = acc;
 }
 
-// CGS2: adds the partials of the j+1 dots of a pass, one wavefront per dot;
-// pass 0 stores them in H, pass 1 adds the correction.
-__global__ void __launch_bounds__(GM_BS)
-k_cgs_reduce(int j, int pass, const double* __restrict__ part, int G, GmresState* __restrict__ st)
-{
-    if (!st->active) return;
-    const int wid = threadIdx.x >> 6;
-    for (int i = wid; i <= j; i += GM_BS / 64) {
-        const double s = sum_partials(part + (size_t)i * G, G);
-        if ((threadIdx.x & 63) == 0) {
-            st->hc[i] = s;
-            if (pass == 0) st->H[i * st->m + j] = s;
-            else st->H[i * st->m + j] += s;
-        }
-    }
-}
 
-// CGS2 update: w -= sum_i hc_i v_i (i = 0..j); then partial dots with all
-// v_i (second pass) or the partial of ||w||^2 (final).
+// CGS2 update: hc_i = the sum of the G partials of dot i (i = 0..j, read
+// from pin; every block sums them itself, in the same fixed order, so the
+// pass needs no separate reduction launch; block 0 records H(i, j)), then
+// w -= sum_i hc_i v_i; then partial dots with all v_i (first pass, to pout)
+// or the partial of ||w||^2 (final pass, to pout slot j + 1).  pin and pout
+// are different regions: blocks still read pin while others write pout.
 template <typename VT>
 __global__ void __launch_bounds__(GM_BS)
 k_cgs_update(int n, int rpt, VT* __restrict__ w, const VT* __restrict__ V, size_t ldv, int j,
-             int final_pass, double* __restrict__ part, int G, const GmresState* __restrict__ st)
+             int final_pass, const double* __restrict__ pin, double* __restrict__ pout, int G,
+             GmresState* __restrict__ st)
 {
     __shared__ double sh[GM_BS / 64];
     __shared__ double hs[GM_MAXM + 1];
+    __shared__ double hw[GM_BS / 64][GM_MAXM + 1];
     if (!st->active) return;
-    if ((int)threadIdx.x <= j) hs[threadIdx.x] = st->hc[threadIdx.x];
-    __syncthreads();
     VT wv[GM_RPT];
     const int base = lblock() * GM_BS * rpt + threadIdx.x;
 #pragma unroll
     for (int r = 0; r < GM_RPT; ++r) {
         const int row = base + r * GM_BS;
         wv[r] = (r < rpt && row < n) ? w[row] : VT(0);
+    }
+    {
+        // thread t adds partials t, t + 256, ... of 16 dots at a time (the
+        // 16 loads issued first), then per dot a wavefront sum; below, the
+        // four wavefronts in order: the same order in every block
+        const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        for (int i0 = 0; i0 <= j; i0 += 16) {
+            double acc[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+            for (int g0 = 0; g0 < G; g0 += GM_BS) {
+                const int g = g0 + (int)threadIdx.x;
+                double v[16];
+#pragma unroll
+                for (int q = 0; q < 16; ++q)
+                    v[q] = (i0 + q <= j && g < G) ? pin[(size_t)(i0 + q) * G + g] : 0.0;
+#pragma unroll
+                for (int q = 0; q < 16; ++q) acc[q] += v[q];
+            }
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                if (i0 + q > j) break;
+                const double t = wave_sum(acc[q]);
+                if (lane == 0) hw[wid][i0 + q] = t;
+            }
+        }
+        __syncthreads();
+        if ((int)threadIdx.x <= j) {
+            const int i = threadIdx.x;
+            double t = 0.0;
+            for (int ww = 0; ww < GM_BS / 64; ++ww) t += hw[ww][i];
+            hs[i] = t;
+            if (lblock() == 0) {
+                st->hc[i] = t;
+                if (!final_pass) st->H[i * st->m + j] = t;
+                else st->H[i * st->m + j] += t;
+            }
+        }
+        __syncthreads();
     }
     // w -= sum_i hc_i v_i in order i = 0..j, basis loads batched
     for (int i0 = 0; i0 <= j; i0 += GM_LB) {
@@ -491,8 +519,8 @@ k_cgs_update(int n, int rpt, VT* __restrict__ w, const VT* __restrict__ V, size_
         const int row = base + r * GM_BS;
         if (r < rpt && row < n) w[row] = wv[r];
     }
-    if (final_pass) block_dots<VT>(0, -1, wv, V, ldv, base, rpt, n, wv, part, G, j + 1);
-    else block_dots<VT>(j + 1, -1, wv, V, ldv, base, rpt, n, wv, part, G, 0);
+    if (final_pass) block_dots<VT>(0, -1, wv, V, ldv, base, rpt, n, wv, pout, G, j + 1);
+    else block_dots<VT>(j + 1, -1, wv, V, ldv, base, rpt, n, wv, pout, G, 0);
 }
 
 // End of a cycle: the pending Givens rotation (if the cycle ran to its last
@@ -580,7 +608,7 @@ int prepare_work(phg_matrix_s* A, int prec, int m, GmresWork** out)
         const size_t es = prec == PHG_F64 ? 8 : 4;
         PHG_HIP(w->V.alloc((size_t)(m + 1) * w->ldv * es));
         PHG_HIP(w->W.alloc(2 * w->ldv * es));
-        PHG_HIP(w->part.alloc((size_t)(m + 2) * G));
+        PHG_HIP(w->part.alloc((size_t)2 * (m + 2) * G));  // two regions (CGS2 passes)
         PHG_HIP(w->st.alloc(1));
         PHG_HIP(hipHostMalloc((void**)&w->host, sizeof(GmresState), hipHostMallocDefault));
         slot.reset(w.release());
@@ -612,12 +640,13 @@ void enqueue_cycle(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mgs<VT>), grid, block, 0, s, n, rpt, dst, V,
                                    ldv, i, j, part, G, st);
         } else {
-            hipLaunchKernelGGL(k_cgs_reduce, dim3(1), block, 0, s, j, 0, part, G, st);
+            // first pass: dots from the step kernel (region A) -> second-pass
+            // dots (region B); second pass: region B -> ||w||^2 (region A)
+            double* partB = part + (size_t)(w->m + 2) * G;
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cgs_update<VT>), grid, block, 0, s, n, rpt, dst,
-                               V, ldv, j, 0, part, G, st);
-            hipLaunchKernelGGL(k_cgs_reduce, dim3(1), block, 0, s, j, 1, part, G, st);
+                               V, ldv, j, 0, part, partB, G, st);
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cgs_update<VT>), grid, block, 0, s, n, rpt, dst,
-                               V, ldv, j, 1, part, G, st);
+                               V, ldv, j, 1, partB, part, G, st);
         }
     }
     hipLaunchKernelGGL(k_solve_y, dim3(1), dim3(GM_BS), 0, s, part, G, st);
