@@ -259,6 +259,18 @@ def l8_leg(steps, oracle_check, seed=20251203):
     return out
 
 
+def calib_copy():
+    """Five device copies of 256 MiB: what rocprof's byte counters read for
+    them scales the counters of the other kernels."""
+    import torch
+    src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 256 MiB
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log("calibration copy: 5 x 256 MiB")
+
+
 def l8_spmv_roofline(A, stream, copies=8):
     """FP64 SpMV of the level-8 matrix: the same matrix repeated (its 65 MB
     stay in the 256 MB Infinity Cache) and round-robin over `copies` copies
@@ -414,6 +426,9 @@ def main():
                     help="run only the ensemble serving leg and print its results (profiling)")
     ap.add_argument("--barvortex-only", type=int, default=0, metavar="LEVEL",
                     help="run only a standalone 8-member BarVortex step at LEVEL (profiling)")
+    ap.add_argument("--spmv-only", type=int, default=0, metavar="LEVEL",
+                    help="run only the FP64 SpMV timing of the Helmholtz matrix at LEVEL, "
+                         "repeated and over 8 copies (profiling)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 256 MiB device copy (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -434,6 +449,15 @@ def main():
         return
     if args.barvortex_only:
         print(json.dumps({"barvortex_only": barvortex_leg(args.barvortex_only)}))
+        return
+    if args.spmv_only:
+        mesh = P.Mesh.sphere(args.spmv_only)
+        cm, cl = P.helmholtz_coeffs(tau=1.0, sigma=0.0, theta=1.0, mu=TAU)
+        A = P.Matrix.assemble(mesh, P.HELMHOLTZ, cm, cl)
+        res = l8_spmv_roofline(A, torch.cuda.current_stream())
+        if args.calib_copy:
+            calib_copy()
+        print(json.dumps({"spmv_only": dict(res, level=args.spmv_only)}))
         return
     stream = torch.cuda.current_stream()
     dev = "cuda:0"
@@ -634,12 +658,7 @@ def main():
         cold_gbs = bspmv / (cold_ms * 1e-3) / 1e9
         log("spmv fp64 cold (40 copies): %.2f us (%.0f GB/s)" % (cold_ms * 1e3, cold_gbs))
     if args.calib_copy:
-        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)  # 256 MiB
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log("calibration copy: 5 x 256 MiB")
+        calib_copy()
 
     # ---- forward error vs the FP64 oracle + CPU baseline ----
     cpu = None
