@@ -44,6 +44,12 @@ class BarVortex:
         # Ab_: integrate_backward_cb (barvortex_impl.h:83-103)
         self.Ab = mesh.assemble(O.KIND_HELMHOLTZ, 1.0 / tau - sigma * (1.0 - theta),
                                 (1.0 - theta) * mu)
+        self.bndA = mesh.assemble_boundary(O.KIND_HELMHOLTZ, cm, cl)    # bnd_ (integrate_cb)
+        self.bnd2 = mesh.assemble_boundary(O.KIND_HELMHOLTZ, 0.0, -1.0)  # laplace_bnd2_cb
+        # barvortex_impl.h:287-293 ADDS bnd_ bnd_w to the right part, where the
+        # Dirichlet elimination of the same operator subtracts it; +1 follows
+        # the reference, -1 is the consistent elimination (used to pin the rest).
+        self.bnd_sign = 1.0
         self.norm_m = mesh.assemble_full(O.KIND_MASS)
         self.J = SphereJacobian(mesh)
         self.lu_idt = spla.splu(_csr(self.idt, n).tocsc())
@@ -52,36 +58,49 @@ class BarVortex:
         self.lu_lap = spla.splu(_csr(self.lap, n).tocsc())
         self.V = len(mesh.p2io)
 
-    def p2u(self, p):
+    def p2u(self, p, bnd=None):
+        """p2u (src/proj.cpp:49-75): inner values, boundary values from bnd
+        (indexed by outer number) or 0."""
         u = np.zeros(self.V)
         u[self.m.inner] = p
+        if bnd is not None:
+            u[self.m.outer] = np.asarray(bnd, dtype=np.float64)
         return u
 
-    def calc1(self, u):
-        """SphereLaplace::calc1 with bnd = 0: p2u(idt^-1 (laplace u_in + bnd3 u_bnd))."""
+    def _bmul(self, t, bnd):
+        brp, bci, bv = t
+        return O.spmv(brp, bci, bv, np.ascontiguousarray(bnd, dtype=np.float64))
+
+    def calc1(self, u, bnd=None):
+        """SphereLaplace::calc1 (src/impl/slaplace_impl.h:153-192):
+        p2u(idt^-1 (laplace u_in + bnd3 u_bnd), bnd)."""
         u_in, u_bnd = u[self.m.inner], u[self.m.outer]
         out = O.spmv(*self.lap, u_in)
-        brp, bci, bv = self.bnd3
-        tmp = O.spmv(brp, bci, bv, np.ascontiguousarray(u_bnd)) if len(bci) else 0.0 * out
+        tmp = self._bmul(self.bnd3, u_bnd) if len(self.bnd3[1]) else 0.0 * out
         out = out + tmp
-        return self.p2u(self.lu_idt.solve(out))
+        return self.p2u(self.lu_idt.solve(out), bnd)
 
-    def poisson(self, w):
-        """SphereLaplace::solve with bnd = 0: p2u(laplace^-1 (idt u2p(w)))."""
+    def poisson(self, w, bnd=None):
+        """SphereLaplace::solve (src/impl/slaplace_impl.h:84-130):
+        p2u(laplace^-1 (idt u2p(w) + bnd2 bnd), bnd), bnd2 = -(the stiffness
+        boundary block) (laplace_bnd2_cb, src/slaplace.cpp:159-169)."""
         b = O.spmv(*self.idt, w[self.m.inner])
-        return self.p2u(self.lu_lap.solve(b))
+        if bnd is not None:
+            b = b + self._bmul(self.bnd2, bnd)
+        return self.p2u(self.lu_lap.solve(b), bnd)
 
     def norm(self, u):
         return math.sqrt(float(np.dot(u, O.spmv(*self.norm_m, u))))
 
-    def calc(self, u, max_iters=1000, tol=1e-14, forcing=None):
+    def calc(self, u, max_iters=1000, tol=1e-14, forcing=None, bnd_u=None, bnd_w=None):
         """One step; forcing: the right-part callback conf.rp(x, y, t, mu, sigma)
         evaluated at every mesh point for this step's t (or None), added to
-        FC as barvortex_impl.h:233-244 does."""
+        FC as barvortex_impl.h:233-244 does; bnd_u, bnd_w: the boundary (pole)
+        values of psi and of its Laplacian at the new time (or None: 0)."""
         u = np.asarray(u, dtype=np.float64)
         th, k1, k2 = self.theta, self.k1, self.k2
         nr0 = self.norm(u)
-        w = self.calc1(u)
+        w = self.calc1(u, bnd_w)
         dw = self.calc1(w)
         FC = (1.0 / self.tau) * w + (self.mu * (1.0 - th)) * dw
         FC = 1.0 * FC + (-self.sigma * (1.0 - th)) * w
@@ -98,8 +117,10 @@ class BarVortex:
             jac = self.J.calc2(tmp2, tmp1)
             F = FC[self.m.inner] - jac
             rp = O.spmv(*self.idt, F)
-            w_n = self.p2u(self.lu_A.solve(rp))
-            u_n1 = self.poisson(w_n)
+            if bnd_w is not None:
+                rp = rp + self.bnd_sign * self._bmul(self.bndA, bnd_w)
+            w_n = self.p2u(self.lu_A.solve(rp), bnd_w)
+            u_n1 = self.poisson(w_n, bnd_u)
             nr = self.norm(u_n1 - u_n)
             u_n = u_n1
             rel = nr / nr0 if nr0 > 0 else nr

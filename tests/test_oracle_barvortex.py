@@ -24,6 +24,10 @@ examples/barvortex_impl.h:171-310) by what the mathematics fixes, on the CPU:
   nonlinear step about z to O(tau) (a sign or argument swap misses by 2x);
   the Jacobian part of calc_LT approaches the adjoint of calc_L's with
   refinement.
+* Boundary (pole) values: a constant carried only by bnd_u, and the steady
+  state psi = z with bnd_u, bnd_w at the pole, both at O(h^2) (the second
+  with the bnd_ term's sign flipped to the consistent elimination; the
+  reference's own sign leaves a fixed offset, checked as such).
 """
 import math
 
@@ -181,3 +185,53 @@ def test_adjoint_step_jacobian_part_approaches_the_adjoint():
         jt = float(u @ O.spmv(*N, bv.calc_LT(v, z) - b0.calc_LT(v, z)))
         gaps.append(abs(jl - jt) / abs(jl))
     assert gaps[0] > gaps[1] > gaps[2] and gaps[2] < 0.3, gaps
+
+
+def test_boundary_values_of_psi_carry_a_constant():
+    """psi = 1 + (x^2 - y^2) z with pole value 1 (lap psi = 0 there): the
+    constant is in the kernel of lap, so the step keeps it through the
+    boundary values alone (slaplace_impl.h:84-130 bnd2 term and p2u) and
+    damps the harmonic by the theta factor, at O(h^2).  With zero boundary
+    values the constant is lost entirely."""
+    tau, sigma, mu, theta = 1.0, 0.5, 0.05, 0.75
+    f3 = _factor(sigma + 12.0 * mu, tau, theta)
+    errs = []
+    for level in (3, 4, 5):
+        bv, pr = _setup(level, tau=tau, sigma=sigma, mu=mu, k1=0.0, k2=0.0, theta=theta)
+        assert len(bv.m.outer) == 1
+        h3 = (pr[:, 0] ** 2 - pr[:, 1] ** 2) * pr[:, 2]
+        want = 1.0 + f3 * h3
+        u1, _, _ = bv.calc(1.0 + h3, max_iters=50, bnd_u=[1.0], bnd_w=[0.0])
+        errs.append(bv.norm(u1 - want) / bv.norm(want))
+        u0, _, _ = bv.calc(1.0 + h3, max_iters=50)
+        assert bv.norm(u0 - want) / bv.norm(want) > 0.5
+    assert errs[-1] < 1e-4, errs
+    assert errs[0] / errs[1] > 3.0 and errs[1] / errs[2] > 3.0, errs
+
+
+def test_boundary_values_of_omega_steady_state():
+    """psi = z with the forcing f = -(2 sigma + 4 mu) z is steady (lap z = -2z,
+    lap^2 z = 4z); the pole is (0, 0, -1), so bnd_u = -1, bnd_w = 2.  With the
+    bnd_ term eliminated consistently (subtracted) the step reproduces z at
+    O(h^2): this pins calc1's boundary values, the bnd_ block and its index.
+    The reference adds the term (barvortex_impl.h:287-293), which leaves an
+    O(1) offset that refinement does not remove; the oracle follows it."""
+    tau, sigma, mu, theta = 1.0, 0.5, 0.05, 0.75
+    cons, lit = [], []
+    for level in (3, 4, 5):
+        bv, pr = _setup(level, tau=tau, sigma=sigma, mu=mu, k1=0.0, k2=0.0, theta=theta)
+        assert np.allclose(pr[bv.m.outer], [[0.0, 0.0, -1.0]])
+        z = pr[:, 2]
+        frc = -(2.0 * sigma + 4.0 * mu) * z
+        u, _, _ = bv.calc(z, max_iters=50, forcing=frc, bnd_u=[-1.0], bnd_w=[2.0])
+        lit.append(bv.norm(u - z) / bv.norm(z))
+        bv.bnd_sign = -1.0
+        u, _, _ = bv.calc(z, max_iters=50, forcing=frc, bnd_u=[-1.0], bnd_w=[2.0])
+        cons.append(bv.norm(u - z) / bv.norm(z))
+        # the bnd_ term is not dropped: without it the error is O(1) too
+        bv.bnd_sign = 0.0
+        u, _, _ = bv.calc(z, max_iters=50, forcing=frc, bnd_u=[-1.0], bnd_w=[2.0])
+        assert bv.norm(u - z) / bv.norm(z) > 0.05
+    assert cons[-1] < 2e-3, cons
+    assert cons[0] / cons[1] > 3.0 and cons[1] / cons[2] > 3.0, cons
+    assert all(0.1 < e < 0.3 for e in lit), lit     # the reference's offset
