@@ -377,6 +377,17 @@ int phg_barvortex_calc(phg_barvortex_t B, int nmembers, const double *d_u, doubl
 int phg_barvortex_calc_rp(phg_barvortex_t B, int nmembers, const double *d_u,
                           const double *d_rp, double *d_u1, const phg_bv_opts *opts,
                           phg_bv_info *infos, phg_stream_t s);
+/* BarVortex::calc(u1, u, bnd_u, bnd_w, t) (barvortex_impl.h:171-310) with
+ * the boundary values: d_bnd_u and d_bnd_w hold, member after member, the
+ * new-time values of psi and of its Laplacian at the mesh's boundary points
+ * (outer order, n_outer each); either may be NULL (zeros).  w = L(u) takes
+ * bnd_w as its boundary values, bnd_ bnd_w is ADDED to the Helmholtz right
+ * part as the reference does (:287-293), and the Poisson solve takes bnd_u
+ * (SphereLaplace::solve, src/impl/slaplace_impl.h:84-130). */
+int phg_barvortex_calc_bnd(phg_barvortex_t B, int nmembers, const double *d_u,
+                           const double *d_rp, const double *d_bnd_u, const double *d_bnd_w,
+                           double *d_u1, const phg_bv_opts *opts, phg_bv_info *infos,
+                           phg_stream_t s);
 /* The same step, slot by slot, for a caller that admits and retires members
  * while others run (continuous batching): load a free slot with u, then call
  * iterate (starts loaded slots, runs one fixed-point pass of the running
@@ -387,6 +398,11 @@ int phg_barvortex_load(phg_barvortex_t B, int slot, const double *d_u, phg_strea
  * d_rp = phg_barvortex_load. */
 int phg_barvortex_load_rp(phg_barvortex_t B, int slot, const double *d_u,
                           const double *d_rp, phg_stream_t s);
+/* After load or load_rp: the slot's boundary values (as
+ * phg_barvortex_calc_bnd, n_outer each, NULL = zeros).  A later load of the
+ * slot clears them. */
+int phg_barvortex_load_bnd(phg_barvortex_t B, int slot, const double *d_bnd_u,
+                           const double *d_bnd_w, phg_stream_t s);
 int phg_barvortex_iterate(phg_barvortex_t B, const phg_bv_opts *opts, int *state,
                           phg_bv_info *infos, phg_stream_t s);
 int phg_barvortex_store(phg_barvortex_t B, int slot, double *d_u1, phg_stream_t s);

@@ -459,9 +459,10 @@ struct BarVortexConf {
 // (examples/barvortex_impl.h:171-310) on the device, calc_L / calc_L_1 /
 // calc_LT the linearised, inverse-linearised and adjoint steps
 // (:326-535) and S_step / L_step / L_1_step / LT_step their wrappers
-// (:537-571).  Boundary (pole) values are zero, as the reference drivers
-// pass them: bnd_u / bnd_w may be null or point to zeros (device pointers);
-// non-zero values are rejected.
+// (:537-571).  calc takes the boundary (pole) values bnd_u / bnd_w (device
+// pointers, outer-size each, null = zeros); the linearised steps take zero
+// boundary values, as the reference drivers pass them (null or zeros),
+// and reject non-zero ones.
 class BarVortex {
 public:
     // the Coriolis term as a device mesh function (the callbacks of conf are
@@ -489,7 +490,6 @@ public:
     }
     void calc(double* Ans, const double* F, const double* bnd_u, const double* bnd_w, double t)
     {
-        check_zero_bnd(bnd_u, bnd_w, "BarVortex::calc");
         const double* rp = nullptr;
         if (conf_.rp) {
             const size_t V = x_.size();
@@ -498,7 +498,7 @@ public:
             rp_.upload(h.data());
             rp = rp_.data();
         }
-        check(phg_barvortex_calc_rp(h_, 1, F, rp, Ans, &opts_, &info_, nullptr),
+        check(phg_barvortex_calc_bnd(h_, 1, F, rp, bnd_u, bnd_w, Ans, &opts_, &info_, nullptr),
               "BarVortex::calc");
     }
     void calc_L(double* u1, const double* u, const double* z, const double* bnd_u,

@@ -50,6 +50,8 @@ struct phg_barvortex_s {
     phg_matrix_s* bnd3 = nullptr;   // SphereLaplace::bnd3_ (slaplace_integrate_cb, boundary)
     phg_matrix_s* A = nullptr;      // BarVortex::A_ (Helmholtz)
     phg_matrix_s* Ab = nullptr;     // BarVortex::Ab_ (backward; built on first calc_L_1)
+    phg_matrix_s* bndA = nullptr;   // BarVortex::bnd_ (integrate_cb, boundary; first load_bnd)
+    phg_matrix_s* bnd2 = nullptr;   // SphereLaplace::bnd2_ (laplace_bnd2_cb; first load_bnd)
     phg_matrix_s* norm = nullptr;   // SphereNorm::NORM_ (full mass, V x V)
     phg_jacobian_t J = nullptr;
     phg_mg_t mg = nullptr;          // Poisson: laplace_ and its V-cycle
@@ -58,6 +60,7 @@ struct phg_barvortex_s {
     DevBuf<double> JAC, F, RP, X, Y, Z, Q;          // n x S
     DevBuf<double> Ub;                              // outer x S
     DevBuf<double> RPF;                             // V x S right parts (conf.rp values)
+    DevBuf<double> BU, BW;                          // outer x S boundary values of psi, lap psi
     DevBuf<double> part, nrm;       // reductions
     DevBuf<int> dmask;
     int G = 0;
@@ -66,6 +69,7 @@ struct phg_barvortex_s {
     std::vector<double> nr0, rel, rel_prev, rel_prev2, rel_min;
     std::vector<int> stalled, diverged;
     std::vector<int> has_rp;                        // slot loaded with a right part
+    std::vector<int> has_bnd;                       // slot loaded with boundary values
     std::vector<int> solve_fail;                    // a solve of the slot did not converge
     ~phg_barvortex_s()
     {
@@ -73,6 +77,8 @@ struct phg_barvortex_s {
         delete bnd3;
         delete A;
         delete Ab;
+        delete bndA;
+        delete bnd2;
         delete norm;
         if (J) phg_jacobian_destroy(J);
         if (mg) phg_mg_destroy(mg);
@@ -80,6 +86,13 @@ struct phg_barvortex_s {
 };
 
 namespace {
+
+bool any_bnd(const phg_barvortex_s* B, const std::vector<int>& mask)
+{
+    for (int k = 0; k < B->S; ++k)
+        if (mask[k] && B->has_bnd[k]) return true;
+    return false;
+}
 
 int upload_mask(phg_barvortex_s* B, const std::vector<int>& m, hipStream_t s)
 {
@@ -157,9 +170,10 @@ int batched(phg_barvortex_s* B, phg_matrix_s* A, const double* rhs, double* x, d
 }
 
 // SphereLaplace::calc1 (src/impl/slaplace_impl.h:153-192) for the masked
-// slots: out = p2u(idt^-1 (laplace u_in + bnd3 u_bnd), 0).
+// slots: out = p2u(idt^-1 (laplace u_in + bnd3 u_bnd), bnd) (bnd: outer x S,
+// or null for 0).
 int calc1(phg_barvortex_s* B, const double* u, double* out, const phg_bv_opts& o,
-          std::vector<int>& acc, hipStream_t s)
+          std::vector<int>& acc, hipStream_t s, const double* bnd = nullptr)
 {
     const int S = B->S, n = B->n;
     phg_matrix_t lap = nullptr;
@@ -176,8 +190,12 @@ int calc1(phg_barvortex_s* B, const double* u, double* out, const phg_bv_opts& o
     int rc = batched(B, B->idt, B->Q.p, B->X.p, o.rtol_mass, o.precision, PHG_CGS1, 0, nullptr,
                      o.max_solver_iters, acc, s, 1, MASS_RESTART);
     if (rc) return rc;
-    hipLaunchKernelGGL(bv_p2u, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
-                       B->mesh->d_isb.p, B->mesh->d_p2io.p, B->X.p, out, B->dmask.p);
+    if (bnd)
+        hipLaunchKernelGGL(bv_p2u_bnd, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
+                           B->mesh->d_isb.p, B->mesh->d_p2io.p, B->X.p, bnd, out, B->dmask.p);
+    else
+        hipLaunchKernelGGL(bv_p2u, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
+                           B->mesh->d_isb.p, B->mesh->d_p2io.p, B->X.p, out, B->dmask.p);
     PHG_HIP(hipGetLastError());
     return PHG_OK;
 }
@@ -196,7 +214,8 @@ int init_slots(phg_barvortex_s* B, const std::vector<int>& mask, const phg_bv_op
         if (mask[k]) fresh |= 1u << k;
     rc = jacobian_clear_pairs(B->J, S, fresh, s);               // Jacobian solves from zero
     if (rc) return rc;
-    rc = calc1(B, B->U.p, B->W.p, o, B->it_mass, s);            // w = L(u)
+    const double* bw = any_bnd(B, mask) ? B->BW.p : nullptr;   // 0 for slots without
+    rc = calc1(B, B->U.p, B->W.p, o, B->it_mass, s, bw);        // w = L(u), bnd_w
     if (rc) return rc;
     rc = calc1(B, B->W.p, B->T1.p, o, B->it_mass, s);           // dw = L(w)
     if (rc) return rc;
@@ -282,20 +301,31 @@ int iterate_slots(phg_barvortex_s* B, const std::vector<int>& mask, const phg_bv
     hipLaunchKernelGGL(bv_rhs, g256((size_t)n * S), dim3(BV_BS), 0, s, n, S, B->mesh->d_inner.p,
                        B->FC.p, B->JAC.p, B->F.p, B->dmask.p);
     spmm(B, B->idt, B->F.p, B->RP.p, 0, s);
+    // Boundary values (barvortex_impl.h:287-293): rp += bnd_ bnd_w.  The
+    // reference adds the block that a Dirichlet elimination would subtract;
+    // this follows it.  Slots without boundary values hold zeros.
+    const bool bnd = any_bnd(B, mask);
+    if (bnd) spmm(B, B->bndA, B->BW.p, B->RP.p, 1, s);
     zero_cols(B, n, B->RP.p, s);
-    // w_n = p2u(A^-1 rp), starting from the current w_n
+    // w_n = p2u(A^-1 rp, bnd_w), starting from the current w_n
     hipLaunchKernelGGL(bv_u2p, g256((size_t)n * S), dim3(BV_BS), 0, s, n, S, B->mesh->d_inner.p,
                        B->Wn.p, B->X.p, B->dmask.p);
     PHG_HIP(hipGetLastError());
     rc = batched(B, B->A, B->RP.p, B->X.p, o.rtol_helmholtz, o.precision, PHG_CGS2, 1, nullptr,
                  o.max_solver_iters, B->it_helm, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(bv_p2u, g256(V * S), dim3(BV_BS), 0, s, B->V, S, B->mesh->d_isb.p,
-                       B->mesh->d_p2io.p, B->X.p, B->Wn.p, B->dmask.p);
-    // u_n1 = p2u(laplace^-1 (idt u2p(w_n))), starting from the current u_n
+    if (bnd)
+        hipLaunchKernelGGL(bv_p2u_bnd, g256(V * S), dim3(BV_BS), 0, s, B->V, S, B->mesh->d_isb.p,
+                           B->mesh->d_p2io.p, B->X.p, B->BW.p, B->Wn.p, B->dmask.p);
+    else
+        hipLaunchKernelGGL(bv_p2u, g256(V * S), dim3(BV_BS), 0, s, B->V, S, B->mesh->d_isb.p,
+                           B->mesh->d_p2io.p, B->X.p, B->Wn.p, B->dmask.p);
+    // u_n1 = p2u(laplace^-1 (idt u2p(w_n) + bnd2 bnd_u), bnd_u)
+    // (SphereLaplace::solve, slaplace_impl.h:84-130), starting from the current u_n
     hipLaunchKernelGGL(bv_u2p, g256((size_t)n * S), dim3(BV_BS), 0, s, n, S, B->mesh->d_inner.p,
                        B->Wn.p, B->Y.p, B->dmask.p);
     spmm(B, B->idt, B->Y.p, B->Q.p, 0, s);
+    if (bnd) spmm(B, B->bnd2, B->BU.p, B->Q.p, 1, s);
     zero_cols(B, n, B->Q.p, s);
     hipLaunchKernelGGL(bv_u2p, g256((size_t)n * S), dim3(BV_BS), 0, s, n, S, B->mesh->d_inner.p,
                        B->Un.p, B->Z.p, B->dmask.p);
@@ -305,8 +335,12 @@ int iterate_slots(phg_barvortex_s* B, const std::vector<int>& mask, const phg_bv
     rc = batched(B, lap, B->Q.p, B->Z.p, o.rtol_poisson, PHG_F64, PHG_CGS2, 1, B->mg,
                  o.max_solver_iters, B->it_pois, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(bv_p2u, g256(V * S), dim3(BV_BS), 0, s, B->V, S, B->mesh->d_isb.p,
-                       B->mesh->d_p2io.p, B->Z.p, B->Un1.p, B->dmask.p);
+    if (bnd)
+        hipLaunchKernelGGL(bv_p2u_bnd, g256(V * S), dim3(BV_BS), 0, s, B->V, S, B->mesh->d_isb.p,
+                           B->mesh->d_p2io.p, B->Z.p, B->BU.p, B->Un1.p, B->dmask.p);
+    else
+        hipLaunchKernelGGL(bv_p2u, g256(V * S), dim3(BV_BS), 0, s, B->V, S, B->mesh->d_isb.p,
+                           B->mesh->d_p2io.p, B->Z.p, B->Un1.p, B->dmask.p);
     PHG_HIP(hipGetLastError());
     // nr = ||u_n1 - u_n||, then u_n = u_n1
     rc = norms(B, B->Un1.p, B->Un.p, nr, s);
@@ -627,6 +661,7 @@ int phg_barvortex_create(phg_mesh_t m, const phg_bv_conf* conf, const double* d_
     B->stalled.assign(S, 0);
     B->diverged.assign(S, 0);
     B->has_rp.assign(S, 0);
+    B->has_bnd.assign(S, 0);
     B->solve_fail.assign(S, 0);
     PHG_HIP(hipStreamSynchronize(s));
     *out = B.release();
@@ -648,6 +683,13 @@ int phg_barvortex_load(phg_barvortex_t B, int slot, const double* d_u, phg_strea
     PHG_HIP(hipGetLastError());
     B->status[slot] = SLOT_NEW;
     B->has_rp[slot] = 0;
+    if (B->has_bnd[slot]) {
+        for (DevBuf<double>* b : {&B->BU, &B->BW})
+            hipLaunchKernelGGL(bv_col_set, g256(B->os), dim3(BV_BS), 0, s, B->os, B->S, slot,
+                               nullptr, b->p);
+        PHG_HIP(hipGetLastError());
+        B->has_bnd[slot] = 0;
+    }
     for (auto* v : {&B->it_mass, &B->it_jac, &B->it_helm, &B->it_pois}) (*v)[slot] = 0;
     return PHG_OK;
 }
@@ -666,6 +708,41 @@ int phg_barvortex_load_rp(phg_barvortex_t B, int slot, const double* d_u, const 
                        B->RPF.p);
     PHG_HIP(hipGetLastError());
     B->has_rp[slot] = 1;
+    return PHG_OK;
+}
+
+int phg_barvortex_load_bnd(phg_barvortex_t B, int slot, const double* d_bnd_u,
+                           const double* d_bnd_w, phg_stream_t stream)
+{
+    if (!B || slot < 0 || slot >= B->S || B->status[slot] != SLOT_NEW) {
+        set_error("phg_barvortex_load_bnd: bad argument or slot not loaded");
+        return PHG_ERR_ARG;
+    }
+    if (!d_bnd_u && !d_bnd_w) return PHG_OK;
+    if (B->os == 0) {
+        set_error("phg_barvortex_load_bnd: the mesh has no boundary points");
+        return PHG_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    if (!B->BU.p) {
+        const phg_bv_conf& c = B->conf;
+        // bnd_ with integrate_cb's coefficients, bnd2_ = laplace_bnd2_cb = -(stiffness block)
+        int rc = assemble_boundary_device(B->mesh, PHG_HELMHOLTZ, 1.0 / c.tau + c.sigma * c.theta,
+                                          -c.theta * c.mu, s, &B->bndA);
+        if (rc) return rc;
+        rc = assemble_boundary_device(B->mesh, PHG_HELMHOLTZ, 0.0, -1.0, s, &B->bnd2);
+        if (rc) return rc;
+        for (DevBuf<double>* b : {&B->BU, &B->BW}) {
+            PHG_HIP(b->alloc((size_t)B->os * B->S));
+            PHG_HIP(hipMemsetAsync(b->p, 0, (size_t)B->os * B->S * sizeof(double), s));
+        }
+    }
+    hipLaunchKernelGGL(bv_col_set, g256(B->os), dim3(BV_BS), 0, s, B->os, B->S, slot, d_bnd_u,
+                       B->BU.p);
+    hipLaunchKernelGGL(bv_col_set, g256(B->os), dim3(BV_BS), 0, s, B->os, B->S, slot, d_bnd_w,
+                       B->BW.p);
+    PHG_HIP(hipGetLastError());
+    B->has_bnd[slot] = 1;
     return PHG_OK;
 }
 
@@ -766,6 +843,15 @@ int phg_barvortex_calc_rp(phg_barvortex_t B, int nmembers, const double* d_u, co
                           double* d_u1, const phg_bv_opts* opts, phg_bv_info* infos,
                           phg_stream_t stream)
 {
+    return phg_barvortex_calc_bnd(B, nmembers, d_u, d_rp, nullptr, nullptr, d_u1, opts, infos,
+                                  stream);
+}
+
+int phg_barvortex_calc_bnd(phg_barvortex_t B, int nmembers, const double* d_u,
+                           const double* d_rp, const double* d_bnd_u, const double* d_bnd_w,
+                           double* d_u1, const phg_bv_opts* opts, phg_bv_info* infos,
+                           phg_stream_t stream)
+{
     if (!B || !d_u || !d_u1 || nmembers < 1 || nmembers > B->S) {
         set_error("phg_barvortex_calc: bad argument (1 <= nmembers <= slots)");
         return PHG_ERR_ARG;
@@ -779,6 +865,10 @@ int phg_barvortex_calc_rp(phg_barvortex_t B, int nmembers, const double* d_u, co
     for (int k = 0; k < nmembers; ++k) {
         rc = phg_barvortex_load_rp(B, k, d_u + (size_t)k * B->V,
                                    d_rp ? d_rp + (size_t)k * B->V : nullptr, stream);
+        if (rc) return rc;
+        const size_t ob = (size_t)k * B->os;
+        rc = phg_barvortex_load_bnd(B, k, d_bnd_u ? d_bnd_u + ob : nullptr,
+                                    d_bnd_w ? d_bnd_w + ob : nullptr, stream);
         if (rc) return rc;
     }
     std::vector<int> state(B->S);

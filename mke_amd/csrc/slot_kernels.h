@@ -52,6 +52,27 @@ __global__ void bv_p2u(int V, int S, const int* __restrict__ isb, const int* __r
     u[e] = isb[k] ? 0.0 : p[(size_t)p2io[k] * S + s];
 }
 
+// u[k][s] = boundary ? bnd[p2io[k]][s] : p[p2io[k]][s]  (p2u, src/proj.cpp:49-75;
+// bnd is outer x S and p2io numbers the boundary points in outer order)
+__global__ void bv_p2u_bnd(int V, int S, const int* __restrict__ isb, const int* __restrict__ p2io,
+                           const double* __restrict__ p, const double* __restrict__ bnd,
+                           double* __restrict__ u, const int* __restrict__ mask)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)V * S) return;
+    const int k = (int)(e / S), s = (int)(e % S);
+    if (!mask[s]) return;
+    u[e] = (isb[k] ? bnd : p)[(size_t)p2io[k] * S + s];
+}
+
+// column s of a rows x S array = v (or 0 when v is null)
+__global__ void bv_col_set(int rows, int S, int s, const double* __restrict__ v,
+                           double* __restrict__ M)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < rows) M[(size_t)k * S + s] = v ? v[k] : 0.0;
+}
+
 // r = k1 a + k2 b (vec_sum1), masked columns; zero elsewhere when zero_rest.
 __global__ void bv_axpby(size_t rows, int S, double k1, const double* __restrict__ a, double k2,
                          const double* __restrict__ b, double* __restrict__ r,

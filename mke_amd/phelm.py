@@ -103,6 +103,8 @@ SIGNATURES = {
     "phg_barvortex_store": (_I, [_P, _I, _P, _P]),
     "phg_barvortex_calc_rp": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     "phg_barvortex_load_rp": (_I, [_P, _I, _P, _P, _P]),
+    "phg_barvortex_calc_bnd": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "phg_barvortex_load_bnd": (_I, [_P, _I, _P, _P, _P]),
     "phg_barvortex_calc_L": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     "phg_barvortex_calc_L_1": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     "phg_barvortex_calc_LT": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
@@ -577,17 +579,25 @@ class BarVortex:
                                          _stream(stream), C.byref(self.h)), "phg_barvortex_create")
         self.slots = slots
 
-    def calc(self, u1, u, opts=None, stream=None, rp=None):
+    def calc(self, u1, u, opts=None, stream=None, rp=None, bnd_u=None, bnd_w=None):
         """u, u1: (nmembers, n_points) FP64 device tensors (or (n_points,));
         rp: the right part conf.rp(x, y, t, mu, sigma) at every point for the
-        step's t, same shape as u (or None)."""
+        step's t, same shape as u (or None); bnd_u, bnd_w: the new-time
+        boundary values of psi and of lap psi, (nmembers, outer_size) (or
+        (outer_size,)) FP64 device tensors, or None for zeros."""
         k = 1 if u.dim() == 1 else int(u.shape[0])
         if rp is not None and tuple(rp.shape) != tuple(u.shape):
             raise PhelmError("BarVortex.calc: rp must have the shape of u")
+        for b in (bnd_u, bnd_w):
+            if b is not None and (b.numel() != k * self.m.outer_size or
+                                  str(b.dtype) != "torch.float64" or not b.is_contiguous()):
+                raise PhelmError("BarVortex.calc: bnd_u/bnd_w must be contiguous FP64 with "
+                                 "outer_size values per member")
         infos = (BvInfo * k)()
-        rc = lib().phg_barvortex_calc_rp(self.h, k, _ptr(u), _ptr(rp) if rp is not None else None,
-                                         _ptr(u1), C.byref(opts) if opts is not None else None,
-                                         infos, _stream(stream))
+        opt = lambda t: _ptr(t) if t is not None else None
+        rc = lib().phg_barvortex_calc_bnd(self.h, k, _ptr(u), opt(rp), opt(bnd_u), opt(bnd_w),
+                                          _ptr(u1), C.byref(opts) if opts is not None else None,
+                                          infos, _stream(stream))
         check(rc, "phg_barvortex_calc")
         return [_bv_info(i) for i in infos]
 
@@ -644,9 +654,13 @@ class BarVortex:
         """BarVortex::LT_step (examples/barvortex_impl.h:564-570)."""
         return self.calc_LT(ans, F, z, opts, stream)
 
-    def load(self, slot, u, stream=None, rp=None):
+    def load(self, slot, u, stream=None, rp=None, bnd_u=None, bnd_w=None):
         check(lib().phg_barvortex_load_rp(self.h, slot, _ptr(u), _ptr(rp) if rp is not None else None,
                                           _stream(stream)), "phg_barvortex_load")
+        if bnd_u is not None or bnd_w is not None:
+            opt = lambda t: _ptr(t) if t is not None else None
+            check(lib().phg_barvortex_load_bnd(self.h, slot, opt(bnd_u), opt(bnd_w),
+                                               _stream(stream)), "phg_barvortex_load_bnd")
 
     def iterate(self, opts=None, stream=None):
         state = (C.c_int * self.slots)()

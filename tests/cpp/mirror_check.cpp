@@ -172,8 +172,14 @@ int main(int argc, char** argv)
             const double one = 1.0;
             nzb.upload(&one);
         }
+        // a pole value of psi of 1 at the new time (bnd_w null = 0)
+        ArrayDevice<double> u1b((size_t)V);
+        bv.calc(u1b.data(), dpsi.data(), nzb.data(), nullptr, 0.0);
+        dump(out, "bv_u1_bnd.bin", from_dev(u1b));
+        // the linearised step takes zero boundary values only
         try {
-            bv.calc(u1.data(), dpsi.data(), nzb.data(), nullptr, 0.0);
+            ArrayDevice<double> l((size_t)V);
+            bv.calc_L(l.data(), dpsi.data(), dpsi.data(), nzb.data(), nullptr, 0.0);
         } catch (const Error& e) {
             threw = e.code() == PHG_ERR_ARG;
         }

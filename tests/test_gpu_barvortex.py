@@ -189,3 +189,49 @@ def test_barvortex_linear_steps_vs_oracle():
     b0.L_step(a, _dev(U[:1]), _dev(Z[:1]), opts=opts)
     b0.L_1_step(back, a, _dev(Z[:1]), opts=opts)
     assert ob.norm(back.cpu().numpy()[0] - U[0]) <= 1e-9 * ob.norm(U[0])
+
+
+def test_barvortex_boundary_values_vs_oracle():
+    """BarVortex::calc with non-zero boundary (pole) values bnd_u, bnd_w
+    (barvortex_impl.h:171-310) at level 4, two members with different values
+    against the oracle; a member without boundary values keeps the bits of
+    the plain step, and a member with them the bits it has alone."""
+    print("barvortex boundary values level=4", flush=True)
+    from oracle.barvortex import BarVortex as OB
+    om, pr, U, lh = _states(4, 2)
+    pm = P.Mesh.load(oracle_mesh(4)[0])
+    assert pm.outer_size == len(om.outer) == 1
+    ob = OB(om, lh=lh, **CONF)
+    U = U + np.array([[0.3], [-0.2]])                      # pole values 0.3, -0.2
+    bu = np.array([[0.31], [-0.19]])                       # new-time psi at the pole
+    bw = np.stack([ob.calc1(U[q])[om.outer] * (1.0 + 0.1 * q) for q in range(2)])
+    bv = P.BarVortex(pm, _dev(lh), slots=2, **CONF)
+    U1 = torch.zeros_like(_dev(U))
+    infos = bv.calc(U1, _dev(U), bnd_u=_dev(bu), bnd_w=_dev(bw))
+    got = U1.cpu().numpy()
+    for q in range(2):
+        ref, it, rel = ob.calc(U[q], max_iters=100, bnd_u=bu[q], bnd_w=bw[q])
+        assert infos[q]["converged"], infos[q]
+        assert infos[q]["iterations"] <= it + 2, (infos[q], it)
+        err = ob.norm(got[q] - ref) / ob.norm(ref)
+        assert err <= 1e-9, (q, err)
+        assert got[q][om.outer[0]] == bu[q][0]
+        plain, _, _ = ob.calc(U[q], max_iters=100)
+        assert ob.norm(plain - ref) > 1e-3 * ob.norm(ref)   # the values matter
+    # member 0 alone with its values; member 1 without values = the plain step
+    one = torch.zeros(1, pm.n_points, dtype=torch.float64, device=DEV)
+    bv.calc(one, _dev(U[:1]), bnd_u=_dev(bu[:1]), bnd_w=_dev(bw[:1]))
+    assert torch.equal(one[0], U1[0])
+    bv.load(0, _dev(U[1]))
+    bv.load(1, _dev(U[0]), bnd_u=_dev(bu[0]), bnd_w=_dev(bw[0]))
+    while True:
+        st = bv.iterate()[0]
+        if all(x != 1 for x in st):
+            break
+    mix = torch.zeros(2, pm.n_points, dtype=torch.float64, device=DEV)
+    bv.store(0, mix[0])
+    bv.store(1, mix[1])
+    plain = torch.zeros(1, pm.n_points, dtype=torch.float64, device=DEV)
+    bv.calc(plain, _dev(U[1:]))
+    assert torch.equal(mix[0], plain[0])
+    assert torch.equal(mix[1], U1[0])
