@@ -7,9 +7,12 @@ never imports it.
 
 The nine blocks of A_ are the oracle's literal restatement of integrate_cb
 (oracle/or_fem.c: or_assemble_baroclin); the right part is generate_right_part
-with right_part_cb's interior branch (or_right_part_mass; the boundary branch
-adds -bnd * element, zero here: the drivers pass zero boundary values,
-examples/test_baroclin.cpp:215).  A_.solve of the reference goes through its
+with right_part_cb's interior branch (or_right_part_mass) and its boundary
+branch (baroclin.cpp:259-323): for a boundary column j0 every element a of
+integrate_cb's four block columns adds -bnd[j0] a to its block row, one bnd
+for w1, w2, u1 and u2 (BW1 = BW2 = BU1 = BU2 = bnd, :541-544); here per block
+with the block's boundary matrix (block_kinds).  The drivers pass zero
+boundary values (examples/test_baroclin.cpp:215).  A_.solve of the reference goes through its
 external solver library (GMRES, parity unpinned); here it is a sparse LU
 factorisation of the 4n x 4n matrix.  SphereLaplace::calc1, SphereJacobian and
 SphereNorm are the oracle's (oracle/barvortex.py, oracle/sjacobian.py).
@@ -24,6 +27,23 @@ import scipy.sparse.linalg as spla
 
 from . import oracle as O
 from .barvortex import BarVortex
+
+
+def block_kinds(tau, sigma, mu, sigma1, mu1, alpha, theta):
+    """integrate_cb's nine element formulas (baroclin.cpp:142-178) as
+    (kind, c_mass, c_lap), in the order of oracle.BAROCLIN_BLOCKS; each
+    block is c_mass (mass element) + c_lap (laplace element)."""
+    H, M = O.KIND_HELMHOLTZ, O.KIND_MASS
+    a2 = alpha * alpha
+    return [(H, 1.0 / tau + 0.5 * theta * sigma, -(theta * mu)),
+            (H, (-0.5 * sigma) * theta, 0.0),
+            (H, 0.5 * theta * sigma, 0.0),
+            (H, 1.0 / tau + 0.5 * theta * sigma + a2 * theta * mu1, -(theta * mu)),
+            (H, -a2 / tau - a2 * theta * sigma1, 0.0),
+            (M, 1.0, 0.0),
+            (H, 0.0, -1.0),
+            (M, 1.0, 0.0),
+            (H, 0.0, -1.0)]
 
 
 class Baroclin:
@@ -41,13 +61,26 @@ class Baroclin:
         self.A = (brp, bci, bv)
         self.lu_A = spla.splu(sp.csr_matrix((bv, bci, brp), shape=(4 * n, 4 * n)).tocsc())
         self._lu_Ab = None
+        self.kinds = block_kinds(tau, sigma, mu, sigma1, mu1, alpha, self.theta)
+        self.bnd_blocks = [mesh.assemble_boundary(kd, cm, cl) for kd, cm, cl in self.kinds]
         # SphereLaplace::calc1, SphereJacobian and SphereNorm of the same mesh
         self.bv = BarVortex(mesh, tau=1.0, sigma=0.0, mu=0.0, k1=0.0, k2=0.0, theta=0.5,
                             lh=np.zeros(len(mesh.p2io)))
         self.V = len(mesh.p2io)
 
-    def calc1(self, u):
-        return self.bv.calc1(u)
+    def calc1(self, u, bnd=None):
+        return self.bv.calc1(u, bnd)
+
+    def bnd_rhs(self, bnd):
+        """right_part_cb's boundary branch: the 4n vector of -sum_c
+        (block (r, c) boundary columns) bnd per block row r."""
+        rs = self.n
+        out = np.zeros(4 * rs)
+        for (r, c), (brp, bci, bv) in zip(O.BAROCLIN_BLOCKS, self.bnd_blocks):
+            if len(bci):
+                out[r * rs:(r + 1) * rs] += O.spmv(brp, bci, -bv,
+                                                   np.ascontiguousarray(bnd, dtype=np.float64))
+        return out
 
     def jac(self, u, v):
         return self.bv.J.calc2(u, v)
@@ -55,10 +88,11 @@ class Baroclin:
     def dist(self, u, v):
         return self.bv.norm(u - v)
 
-    def calc(self, u1, u2, f=None, g=None, max_iters=20, tol=1e-8):
+    def calc(self, u1, u2, f=None, g=None, max_iters=20, tol=1e-8, bnd=None):
         """One step (u1, u2) -> (u11, u21); f, g: the forcing callbacks'
-        values at every mesh point for this step's t (or None).  Returns
-        (u11, u21, iterations, last max distance)."""
+        values at every mesh point for this step's t (or None); bnd: the
+        boundary values of w1, w2, u1, u2 (one array, outer order) or None
+        for 0.  Returns (u11, u21, iterations, last max distance)."""
         th, tau = self.theta, self.tau
         sigma, mu, sigma1, mu1, alpha = self.sigma, self.mu, self.sigma1, self.mu1, self.alpha
         a2 = alpha * alpha
@@ -66,10 +100,11 @@ class Baroclin:
         rs = self.n
         u1 = np.asarray(u1, dtype=np.float64)
         u2 = np.asarray(u2, dtype=np.float64)
-        w1 = self.calc1(u1)
-        w2 = self.calc1(u2)
-        dw1 = self.calc1(w1)
-        dw2 = self.calc1(w2)
+        w1 = self.calc1(u1, bnd)
+        w2 = self.calc1(u2, bnd)
+        dw1 = self.calc1(w1, bnd)
+        dw2 = self.calc1(w2, bnd)
+        brhs = self.bnd_rhs(bnd) if bnd is not None else None
         s1 = lambda a, b, k1, k2: k1 * a + k2 * b  # noqa: E731  vec_sum1
         FC = s1(w1, w2, -0.5 * (1.0 - th) * sigma, 0.5 * (1.0 - th) * sigma)
         FC = s1(FC, dw1, 1.0, mu * (1.0 - th))
@@ -111,11 +146,13 @@ class Baroclin:
             rp = np.zeros(4 * rs)
             rp[:rs] = O.right_part_mass(self.m, F)
             rp[rs:2 * rs] = O.right_part_mass(self.m, G)
+            if brhs is not None:
+                rp = rp + brhs
             ans = self.lu_A.solve(rp)
-            w1_n = self.bv.p2u(ans[:rs])
-            w2_n = self.bv.p2u(ans[rs:2 * rs])
-            u1_n1 = self.bv.p2u(ans[2 * rs:3 * rs])
-            u2_n1 = self.bv.p2u(ans[3 * rs:])
+            w1_n = self.bv.p2u(ans[:rs], bnd)
+            w2_n = self.bv.p2u(ans[rs:2 * rs], bnd)
+            u1_n1 = self.bv.p2u(ans[2 * rs:3 * rs], bnd)
+            u2_n1 = self.bv.p2u(ans[3 * rs:], bnd)
             nr = max(self.dist(u1_n1, u1_n), self.dist(u2_n1, u2_n))
             u1_n, u2_n = u1_n1, u2_n1
             if nr < tol:

@@ -189,3 +189,56 @@ def test_inverse_step_inverts_to_second_order():
         back = bc.calc_lin(1, L[0], L[1], z1, z2)
         errs.append(_rel(bc, back[:2], u, u))
     assert errs[1] < errs[0] / 10.0 and errs[1] < 1e-4, errs
+
+
+def test_block_kinds_reproduce_the_blocks():
+    """The (kind, c_mass, c_lap) table behind the boundary blocks assembles
+    the nine inner blocks of or_assemble_baroclin's literal formulas."""
+    from oracle.baroclin import block_kinds
+    M, _ = _mesh(3)
+    rp, ci, v9 = O.baroclin_blocks(M, TAU, PRM["sigma"], PRM["mu"], PRM["sigma1"], PRM["mu1"],
+                                   PRM["alpha"], 0.5)
+    kinds = block_kinds(TAU, PRM["sigma"], PRM["mu"], PRM["sigma1"], PRM["mu1"], PRM["alpha"], 0.5)
+    for k, (kd, cm, cl) in enumerate(kinds):
+        got = M.assemble(kd, cm, cl)[2]
+        assert np.abs(got - v9[k]).max() <= 1e-14 * np.abs(v9[k]).max(), k
+
+
+def test_boundary_values_are_a_dirichlet_elimination():
+    """right_part_cb's boundary branch (baroclin.cpp:259-323) is the
+    elimination of known boundary values: for any four full fields whose
+    boundary values all equal bnd, A_ times their inner values equals the
+    inner rows of the FULL block products (generate_full_matrix, another
+    assembly route) plus bnd_rhs(bnd).  A dropped block, a wrong sign or a
+    boundary index mix-up fails it."""
+    M, _ = _mesh(3)
+    bc = Baroclin(M, TAU, lh=np.zeros(len(M.p2io)), **PRM)
+    rng = np.random.default_rng(5)
+    V, inner, outer, n = len(M.p2io), M.inner, M.outer, M.n_inner
+    bnd = rng.standard_normal(len(outer))
+    X = rng.standard_normal((4, V))
+    X[:, outer] = bnd
+    rhs = np.zeros(4 * n)
+    for (r, c), (kd, cm, cl) in zip(O.BAROCLIN_BLOCKS, bc.kinds):
+        frp, fci, fv = M.assemble_full(kd, cm, cl)
+        rhs[r * n:(r + 1) * n] += O.spmv(frp, fci, fv, X[c])[inner]
+    brp, bci, bv = bc.A
+    lhs = O.spmv(brp, bci, bv, np.concatenate([X[c][inner] for c in range(4)]))
+    err = np.abs(lhs - (rhs + bc.bnd_rhs(bnd))).max()
+    assert err <= 1e-10 * np.abs(rhs).max(), err
+    assert np.abs(bc.bnd_rhs(bnd)).max() > 1e-3 * np.abs(rhs).max()   # the term matters
+
+
+def test_step_takes_the_boundary_values():
+    """The step writes bnd into the four results' boundary points, zero
+    boundary values give the step without them bit for bit, and non-zero
+    ones change it."""
+    M, xy = _mesh(3)
+    bc = Baroclin(M, TAU, lh=np.zeros(len(M.p2io)), **PRM)
+    u1, u2 = manufactured(xy, 0.0, **PRM)[:2]
+    a = bc.calc(u1, u2, max_iters=5)
+    b = bc.calc(u1, u2, max_iters=5, bnd=np.zeros(len(M.outer)))
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    c = bc.calc(u1, u2, max_iters=5, bnd=np.full(len(M.outer), 0.5))
+    assert np.all(c[0][M.outer] == 0.5) and np.all(c[1][M.outer] == 0.5)
+    assert bc.dist(c[0], a[0]) > 1e-6 * bc.dist(a[0], 0 * a[0])
