@@ -481,6 +481,15 @@ int phg_baroclin_calc(phg_baroclin_t B, int nmembers, const double *d_u1,
                       const double *d_u2, const double *d_f, const double *d_g,
                       double *d_u11, double *d_u21, const phg_bc_opts *opts,
                       phg_bc_info *infos, phg_stream_t s);
+/* The same step with the boundary (pole) values: d_bnd holds, member after
+ * member, n_outer values (outer order) that are the boundary values of w1,
+ * w2, u1 and u2 alike (baroclin.cpp:494-498, 541-544, 587-590); right_part_cb
+ * subtracts the boundary columns of the blocks times them (:259-323).  NULL
+ * = zeros. */
+int phg_baroclin_calc_bnd(phg_baroclin_t B, int nmembers, const double *d_u1,
+                          const double *d_u2, const double *d_f, const double *d_g,
+                          const double *d_bnd, double *d_u11, double *d_u21,
+                          const phg_bc_opts *opts, phg_bc_info *infos, phg_stream_t s);
 /* The linearised steps about (z1, z2), zero boundary values, for nmembers
  * (<= slots) members; all arrays hold nmembers mesh functions one after the
  * other.  The Jacobian arguments are the reference code's (two of them,

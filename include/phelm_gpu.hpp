@@ -600,8 +600,10 @@ private:
 // Baroclin (examples/baroclin.h:60-292): the constructor takes the
 // reference's callbacks; calc() evaluates f and g at the mesh points' zone-0
 // chart coordinates for its t (as baroclin.cpp:520-535 does) and runs the
-// step on the device.  Boundary (pole) values are zero, as the reference
-// driver passes them; bnd must be null or all zero (it is not read).
+// step on the device.  calc takes the boundary (pole) values bnd (one
+// device array of outer-size for w1, w2, u1, u2, as the reference; null =
+// zeros); the linearised steps take zero boundary values, as the reference
+// driver passes them (bnd null or all zero; it is not read there).
 class Baroclin {
 public:
     typedef double (*rp_t)(double x, double y, double t, double sigma, double mu, double sigma1,
@@ -644,7 +646,6 @@ public:
     void calc(double* u11, double* u21, const double* u1, const double* u2, const double* bnd,
               double t)
     {
-        (void)bnd;
         const size_t V = x_.size();
         std::vector<double> hf(V), hg(V);
         for (size_t i = 0; i < V; ++i) {
@@ -654,8 +655,9 @@ public:
         }
         fv_.upload(hf.data());
         gv_.upload(hg.data());
-        check(phg_baroclin_calc(h_, 1, u1, u2, f_ ? fv_.data() : nullptr,
-                                g_ ? gv_.data() : nullptr, u11, u21, &opts_, &info_, nullptr),
+        check(phg_baroclin_calc_bnd(h_, 1, u1, u2, f_ ? fv_.data() : nullptr,
+                                    g_ ? gv_.data() : nullptr, bnd, u11, u21, &opts_, &info_,
+                                    nullptr),
               "Baroclin::calc");
     }
     // the linearised steps about (z1, z2) (baroclin.cpp:607-986); bnd as calc

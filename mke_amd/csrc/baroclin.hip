@@ -311,6 +311,8 @@ struct phg_baroclin_s {
     DevBuf<double> JAC;                                     // n x 5S
     DevBuf<double> F, G, Y, Q, X;                           // n x S
     DevBuf<double> Ub;                                      // outer x S
+    DevBuf<double> BND;                                     // outer x S boundary values
+    phg_matrix_s* bndr[4] = {};     // -(sum of block row r's boundary blocks), n x outer
     DevBuf<double> RP, ANS;                                 // 4n x S
     DevBuf<double> Z1, Z2, DZ1, DZ2, JAC2;  // linearised steps: z, L(z) (V x S), n x 5S
     DevBuf<double> part, nrm;
@@ -323,6 +325,7 @@ struct phg_baroclin_s {
         for (auto* b : blkb) delete b;
         if (Ab4) phg_matrix_destroy(Ab4);
         delete bnd3;
+        for (auto* b : bndr) delete b;
         delete norm;
         if (J) phg_jacobian_destroy(J);
         if (mg) phg_mg_destroy(mg);
@@ -358,10 +361,11 @@ phg_gmres_opts solve_opts(double rtol, int ortho, int use_x0, int max_iters)
     return o;
 }
 
-// SphereLaplace::calc1 (src/impl/slaplace_impl.h:153-192) with zero boundary
-// values for the masked slots: out = p2u(idt^-1 (laplace u_in + bnd3 u_bnd)).
+// SphereLaplace::calc1 (src/impl/slaplace_impl.h:153-192) for the masked
+// slots: out = p2u(idt^-1 (laplace u_in + bnd3 u_bnd), bnd)
+// (bnd: outer x S, or null for 0).
 int calc1(phg_baroclin_s* B, const double* u, double* out, const phg_bc_opts& o,
-          std::vector<int>& acc, hipStream_t s)
+          std::vector<int>& acc, hipStream_t s, const double* bnd = nullptr)
 {
     const int S = B->S, n = B->n;
     phg_matrix_t lap = nullptr;
@@ -383,8 +387,12 @@ int calc1(phg_baroclin_s* B, const double* u, double* out, const phg_bc_opts& o,
     int rc = gmres_batch_solve(B->blk[5], S, B->Q.p, B->X.p, &go, inf.data(), s);
     if (rc < 0) return rc;
     for (int k = 0; k < S; ++k) acc[k] += inf[k].iterations;
-    hipLaunchKernelGGL(bv_p2u, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
-                       B->mesh->d_isb.p, B->mesh->d_p2io.p, B->X.p, out, B->dmask.p);
+    if (bnd)
+        hipLaunchKernelGGL(bv_p2u_bnd, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
+                           B->mesh->d_isb.p, B->mesh->d_p2io.p, B->X.p, bnd, out, B->dmask.p);
+    else
+        hipLaunchKernelGGL(bv_p2u, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
+                           B->mesh->d_isb.p, B->mesh->d_p2io.p, B->X.p, out, B->dmask.p);
     PHG_HIP(hipGetLastError());
     return PHG_OK;
 }
@@ -703,6 +711,22 @@ int phg_baroclin_create(phg_mesh_t m, const phg_bc_conf* conf, const double* d_l
         int rc = assemble_device(m, kind, cm[k], cl[k], s, &B->blk[k]);
         if (rc) return rc;
     }
+    // right_part_cb's boundary branch (baroclin.cpp:259-323) subtracts every
+    // block's boundary element times the one bnd array (BW1 = BW2 = BU1 =
+    // BU2 = bnd, :541-544): per block row, the boundary block of the summed
+    // coefficients, negated.
+    if (B->os) {
+        const int row[9] = {0, 0, 1, 1, 1, 2, 2, 3, 3};
+        double sm[4] = {}, sl[4] = {};
+        for (int k = 0; k < 9; ++k) {
+            sm[row[k]] += cm[k];
+            sl[row[k]] += cl[k];
+        }
+        for (int r = 0; r < 4; ++r) {
+            int rc = assemble_boundary_device(m, PHG_HELMHOLTZ, -sm[r], -sl[r], s, &B->bndr[r]);
+            if (rc) return rc;
+        }
+    }
     int rc = stack_blocks(B.get(), B->blk, &B->A4);
     if (rc) return rc;
     rc = assemble_boundary_device(m, PHG_LAPLACE, 1.0, 0.0, s, &B->bnd3);
@@ -781,8 +805,21 @@ int phg_baroclin_calc(phg_baroclin_t B, int nmembers, const double* d_u1, const 
                       const double* d_f, const double* d_g, double* d_u11, double* d_u21,
                       const phg_bc_opts* opts, phg_bc_info* infos, phg_stream_t stream)
 {
+    return phg_baroclin_calc_bnd(B, nmembers, d_u1, d_u2, d_f, d_g, nullptr, d_u11, d_u21, opts,
+                                 infos, stream);
+}
+
+int phg_baroclin_calc_bnd(phg_baroclin_t B, int nmembers, const double* d_u1,
+                          const double* d_u2, const double* d_f, const double* d_g,
+                          const double* d_bnd, double* d_u11, double* d_u21,
+                          const phg_bc_opts* opts, phg_bc_info* infos, phg_stream_t stream)
+{
     if (!B || !d_u1 || !d_u2 || !d_u11 || !d_u21 || nmembers < 1 || nmembers > B->S) {
         set_error("phg_baroclin_calc: bad argument (1 <= nmembers <= slots)");
+        return PHG_ERR_ARG;
+    }
+    if (d_bnd && B->os == 0) {
+        set_error("phg_baroclin_calc_bnd: the mesh has no boundary points");
         return PHG_ERR_ARG;
     }
     phg_bc_opts o;
@@ -805,13 +842,28 @@ int phg_baroclin_calc(phg_baroclin_t B, int nmembers, const double* d_u1, const 
                            B->U2.p);
     }
     PHG_HIP(hipGetLastError());
+    // Boundary values: once an object has taken them, every call runs the
+    // boundary path, members without values holding zeros (the right part's
+    // u rows are then written, not left at zero).
+    if (d_bnd && !B->BND.p) {
+        PHG_HIP(B->BND.alloc((size_t)B->os * S));
+        PHG_HIP(hipMemsetAsync(B->BND.p, 0, (size_t)B->os * S * sizeof(double), s));
+    }
+    if (B->BND.p) {
+        for (int k = 0; k < nmembers; ++k)
+            hipLaunchKernelGGL(bv_col_set, g256(B->os), dim3(BV_BS), 0, s, B->os, S, k,
+                               d_bnd ? d_bnd + (size_t)k * B->os : nullptr, B->BND.p);
+        PHG_HIP(hipGetLastError());
+    }
+    const double* bnd = B->BND.p;
     std::vector<int> it_mass(S, 0), it_jac(S, 0), it_sys(S, 0), iters(S, 0);
     std::vector<double> dist(S, 0.0);
-    // w = L(u), dw = L(w)  (dw1, dw2 in U1n1, U2n1 as scratch)
-    if ((rc = calc1(B, B->U1.p, B->W1.p, o, it_mass, s))) return rc;
-    if ((rc = calc1(B, B->U2.p, B->W2.p, o, it_mass, s))) return rc;
-    if ((rc = calc1(B, B->W1.p, B->U1n1.p, o, it_mass, s))) return rc;
-    if ((rc = calc1(B, B->W2.p, B->U2n1.p, o, it_mass, s))) return rc;
+    // w = L(u), dw = L(w) with the boundary values (baroclin.cpp:494-498;
+    // dw1, dw2 in U1n1, U2n1 as scratch)
+    if ((rc = calc1(B, B->U1.p, B->W1.p, o, it_mass, s, bnd))) return rc;
+    if ((rc = calc1(B, B->U2.p, B->W2.p, o, it_mass, s, bnd))) return rc;
+    if ((rc = calc1(B, B->W1.p, B->U1n1.p, o, it_mass, s, bnd))) return rc;
+    if ((rc = calc1(B, B->W2.p, B->U2n1.p, o, it_mass, s, bnd))) return rc;
     const dim3 gv = g256(V * S), bb(BV_BS);
     // FC and GC, vec_sum1 by vec_sum1 as baroclin.cpp:496-513
     auto lin = [&](double k1, const double* a, double k2, const double* b, double* r) {
@@ -868,6 +920,9 @@ int phg_baroclin_calc(phg_baroclin_t B, int nmembers, const double* d_u1, const 
         // right part [M F; M G; 0; 0] and the 4n solve from the previous answer
         spmm(B, B->blk[5], B->F.p, B->RP.p, 0, s);
         spmm(B, B->blk[5], B->G.p, B->RP.p + (size_t)n * S, 0, s);
+        if (bnd)
+            for (int r = 0; r < 4; ++r)
+                spmm(B, B->bndr[r], bnd, B->RP.p + (size_t)r * n * S, r < 2, s);
         hipLaunchKernelGGL(bv_zero_cols, g256((size_t)4 * n * S), bb, 0, s, (size_t)4 * n, S,
                            B->RP.p, B->dmask.p);
         hipLaunchKernelGGL(bv_zero_cols, g256((size_t)4 * n * S), bb, 0, s, (size_t)4 * n, S,
@@ -880,14 +935,15 @@ int phg_baroclin_calc(phg_baroclin_t B, int nmembers, const double* d_u1, const 
         // w1_n, w2_n, u1_n1, u2_n1 = p2u of the four parts
         const double* ans = B->ANS.p;
         const size_t nb = (size_t)n * S;
-        hipLaunchKernelGGL(bv_p2u, gv, bb, 0, s, (int)V, S, B->mesh->d_isb.p, B->mesh->d_p2io.p,
-                           ans, B->W1n.p, B->dmask.p);
-        hipLaunchKernelGGL(bv_p2u, gv, bb, 0, s, (int)V, S, B->mesh->d_isb.p, B->mesh->d_p2io.p,
-                           ans + nb, B->W2n.p, B->dmask.p);
-        hipLaunchKernelGGL(bv_p2u, gv, bb, 0, s, (int)V, S, B->mesh->d_isb.p, B->mesh->d_p2io.p,
-                           ans + 2 * nb, B->U1n1.p, B->dmask.p);
-        hipLaunchKernelGGL(bv_p2u, gv, bb, 0, s, (int)V, S, B->mesh->d_isb.p, B->mesh->d_p2io.p,
-                           ans + 3 * nb, B->U2n1.p, B->dmask.p);
+        double* outs[4] = {B->W1n.p, B->W2n.p, B->U1n1.p, B->U2n1.p};
+        for (int r = 0; r < 4; ++r) {
+            if (bnd)
+                hipLaunchKernelGGL(bv_p2u_bnd, gv, bb, 0, s, (int)V, S, B->mesh->d_isb.p,
+                                   B->mesh->d_p2io.p, ans + r * nb, bnd, outs[r], B->dmask.p);
+            else
+                hipLaunchKernelGGL(bv_p2u, gv, bb, 0, s, (int)V, S, B->mesh->d_isb.p,
+                                   B->mesh->d_p2io.p, ans + r * nb, outs[r], B->dmask.p);
+        }
         PHG_HIP(hipGetLastError());
         std::vector<double> d1, d2;
         if ((rc = dists(B, B->U1n1.p, B->U1n.p, d1, s))) return rc;

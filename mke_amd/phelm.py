@@ -104,6 +104,7 @@ SIGNATURES = {
     "phg_barvortex_calc_rp": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     "phg_barvortex_load_rp": (_I, [_P, _I, _P, _P, _P]),
     "phg_barvortex_calc_bnd": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "phg_baroclin_calc_bnd": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "phg_barvortex_load_bnd": (_I, [_P, _I, _P, _P, _P]),
     "phg_barvortex_calc_L": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     "phg_barvortex_calc_L_1": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
@@ -727,18 +728,24 @@ class Baroclin:
         check(lib().phg_baroclin_matrix(self.h, C.byref(h)), "phg_baroclin_matrix")
         return Matrix(h, owned=False)
 
-    def calc(self, u11, u21, u1, u2, f=None, g=None, opts=None, stream=None):
+    def calc(self, u11, u21, u1, u2, f=None, g=None, opts=None, stream=None, bnd=None):
         """u1, u2, u11, u21: (nmembers, n_points) FP64 device tensors (or
         (n_points,)); f, g: the forcing at the mesh points for this step's t
-        (device tensors of n_points values) or None.  Returns one info dict
-        per member."""
+        (device tensors of n_points values) or None; bnd: the boundary values
+        of w1, w2, u1, u2, (nmembers, outer_size) (or (outer_size,)) FP64
+        device tensor, or None for zeros.  Returns one info dict per member."""
         k = 1 if u1.dim() == 1 else int(u1.shape[0])
+        if bnd is not None and (bnd.numel() != k * self.m.outer_size or
+                                str(bnd.dtype) != "torch.float64" or not bnd.is_contiguous()):
+            raise PhelmError("Baroclin.calc: bnd must be contiguous FP64 with outer_size "
+                             "values per member")
         infos = (BcInfo * k)()
-        rc = lib().phg_baroclin_calc(self.h, k, _ptr(u1), _ptr(u2),
-                                     _ptr(f) if f is not None else None,
-                                     _ptr(g) if g is not None else None, _ptr(u11), _ptr(u21),
-                                     C.byref(opts) if opts is not None else None, infos,
-                                     _stream(stream))
+        rc = lib().phg_baroclin_calc_bnd(self.h, k, _ptr(u1), _ptr(u2),
+                                         _ptr(f) if f is not None else None,
+                                         _ptr(g) if g is not None else None,
+                                         _ptr(bnd) if bnd is not None else None, _ptr(u11),
+                                         _ptr(u21), C.byref(opts) if opts is not None else None,
+                                         infos, _stream(stream))
         check(rc, "phg_baroclin_calc")
         return [dict(iterations=i.iterations, distance=i.distance, converged=bool(i.converged),
                      it_mass=i.it_mass, it_jacobian=i.it_jacobian, it_system=i.it_system)

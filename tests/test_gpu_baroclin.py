@@ -179,3 +179,34 @@ def test_baroclin_inverse_step_without_jacobian():
     g = [t.cpu().numpy()[0] for t in (L1, B1, B2)]
     assert ob.dist(g[0], U1[0]) > 1e-3 * ob.bv.norm(U1[0])
     assert _rel2(ob, g[1], g[2], U1[0], U2[0]) <= 1e-9
+
+
+def test_baroclin_boundary_values_vs_oracle():
+    """Baroclin::calc with non-zero boundary (pole) values, one array for w1,
+    w2, u1, u2 (baroclin.cpp:445-606), two members with different values,
+    against the oracle; afterwards a call without values on the same object
+    gives the plain step's bits."""
+    print("baroclin boundary values", flush=True)
+    pm, om, xy = _setup(3)
+    U1, U2, f, g = _states(xy, om)
+    assert pm.outer_size == len(om.outer) >= 1
+    bnd = np.array([[0.02], [-0.03]]) * np.ones((2, pm.outer_size))
+    opts = P.bc_opts(precondition=1, max_solver_iters=20000)
+    bc = P.Baroclin(pm, _dev(np.zeros(len(xy))), tau=TAU, slots=2, **PRM)
+    plain1, plain2 = torch.zeros_like(_dev(U1)), torch.zeros_like(_dev(U2))
+    bc.calc(plain1, plain2, _dev(U1), _dev(U2), _dev(f), _dev(g), opts=opts)
+    D1, D2 = torch.zeros_like(_dev(U1)), torch.zeros_like(_dev(U2))
+    infos = bc.calc(D1, D2, _dev(U1), _dev(U2), _dev(f), _dev(g), opts=opts, bnd=_dev(bnd))
+    ob = OB(om, tau=TAU, lh=np.zeros(len(xy)), **PRM)
+    got1, got2 = D1.cpu().numpy(), D2.cpu().numpy()
+    for q in range(2):
+        r1, r2, it, nr = ob.calc(U1[q], U2[q], f, g, bnd=bnd[q])
+        assert infos[q]["converged"] and abs(infos[q]["iterations"] - it) <= 1, (infos[q], it)
+        assert ob.dist(got1[q], r1) <= 1e-9 * ob.bv.norm(r1), q
+        assert ob.dist(got2[q], r2) <= 1e-9 * ob.bv.norm(r2), q
+        assert np.all(got1[q][om.outer] == bnd[q]) and np.all(got2[q][om.outer] == bnd[q])
+        p1 = plain1.cpu().numpy()[q]
+        assert ob.dist(got1[q], p1) > 1e-6 * ob.bv.norm(p1)    # the values matter
+    again1, again2 = torch.zeros_like(D1), torch.zeros_like(D2)
+    bc.calc(again1, again2, _dev(U1), _dev(U2), _dev(f), _dev(g), opts=opts)
+    assert torch.equal(again1, plain1) and torch.equal(again2, plain2)
