@@ -97,6 +97,9 @@ def test_argument_errors_without_a_gpu():
         ("phg_barvortex_calc_L_1", (None, 1, None, None, None, None, None, None)),
         ("phg_barvortex_calc_LT", (None, 1, None, None, None, None, None, None)),
         ("phg_barvortex_calc_rp", (None, 1, None, None, None, None, None, None)),
+        ("phg_barvortex_calc_bnd", (None, 1) + (None,) * 8),
+        ("phg_barvortex_load_bnd", (None, 0, None, None, None)),
+        ("phg_baroclin_calc_bnd", (None, 1) + (None,) * 10),
         ("phg_baroclin_calc_L", (None, 1) + (None,) * 9),
         ("phg_baroclin_calc_L_1", (None, 1) + (None,) * 9),
     ):
