@@ -191,15 +191,16 @@ def test_barvortex_linear_steps_vs_oracle():
     assert ob.norm(back.cpu().numpy()[0] - U[0]) <= 1e-9 * ob.norm(U[0])
 
 
-def test_barvortex_boundary_values_vs_oracle():
+@pytest.mark.parametrize("level", [4, 5])
+def test_barvortex_boundary_values_vs_oracle(level):
     """BarVortex::calc with non-zero boundary (pole) values bnd_u, bnd_w
-    (barvortex_impl.h:171-310) at level 4, two members with different values
+    (barvortex_impl.h:171-310) at levels 4 and 5, two members with different values
     against the oracle; a member without boundary values keeps the bits of
     the plain step, and a member with them the bits it has alone."""
-    print("barvortex boundary values level=4", flush=True)
+    print("barvortex boundary values level=%d" % level, flush=True)
     from oracle.barvortex import BarVortex as OB
-    om, pr, U, lh = _states(4, 2)
-    pm = P.Mesh.load(oracle_mesh(4)[0])
+    om, pr, U, lh = _states(level, 2)
+    pm = P.Mesh.load(oracle_mesh(level)[0])
     assert pm.outer_size == len(om.outer) == 1
     ob = OB(om, lh=lh, **CONF)
     U = U + np.array([[0.3], [-0.2]])                      # pole values 0.3, -0.2
