@@ -4,12 +4,10 @@
 // are compiled with -ffp-contract=off so that results are bit-identical on
 // host and device.
 //
-//   cos             : Cody-Waite reduction by pi/2 (three-part constant) and
-//                     Taylor polynomials of degree 16 (cos) / 17 (sin) on
-//                     |r| <= pi/4, Horner with fused multiply-add.  Stays
-//                     within 2 ulp of libm; a fixed routine is used because the
-//                     mass entries cancel ~1e7-fold and a 1-ulp change of cos
-//                     moves them ~1e-9 relative.
+//   cos             : correctly rounded (cr_trig.h), since the mass entries
+//                     cancel ~1e7-fold and a 1-ulp change of cos moves them
+//                     ~1e-10 relative; the reference calls the C library's
+//                     cos (src/util.cpp:194-215).
 //   ipow            : repeated multiplication (the reference's ipow comes from
 //                     its solver library, absent from its tree).
 //   GK15            : src/quadrature.c:49-128 (15-digit node/weight tables)
@@ -22,75 +20,16 @@
 
 #include <cmath>
 
+#include "cr_trig.h"
+
 #define PHG_HD __host__ __device__ inline
 
 namespace phg {
 
-// ---------------------------------------------------------------- cos/sin --
-struct TrigConst {
-    static constexpr double pio2_a = 0x1.921fb54400000p+0;
-    static constexpr double pio2_b = 0x1.0b4611a600000p-34;
-    static constexpr double pio2_c = 0x1.3198a2e037073p-69;
-    static constexpr double two_over_pi = 0x1.45f306dc9c883p-1;
-};
-
-PHG_HD double taylor_cos(double r)
-{
-    // (-1)^k / (2k)!, k = 1..8
-    const double z = r * r;
-    double p = 0x1.ae7f3e733b81fp-45;
-    p = fma(p, z, -0x1.93974a8c07c9dp-37);
-    p = fma(p, z, 0x1.1eed8eff8d898p-29);
-    p = fma(p, z, -0x1.27e4fb7789f5cp-22);
-    p = fma(p, z, 0x1.a01a01a01a01ap-16);
-    p = fma(p, z, -0x1.6c16c16c16c17p-10);
-    p = fma(p, z, 0x1.5555555555555p-5);
-    p = fma(p, z, -0x1.0000000000000p-1);
-    return fma(z, p, 1.0);
-}
-
-PHG_HD double taylor_sin(double r)
-{
-    // (-1)^k / (2k+1)!, k = 1..8
-    const double z = r * r;
-    double p = 0x1.952c77030ad4ap-49;
-    p = fma(p, z, -0x1.ae7f3e733b81fp-41);
-    p = fma(p, z, 0x1.6124613a86d09p-33);
-    p = fma(p, z, -0x1.ae64567f544e4p-26);
-    p = fma(p, z, 0x1.71de3a556c734p-19);
-    p = fma(p, z, -0x1.a01a01a01a01ap-13);
-    p = fma(p, z, 0x1.1111111111111p-7);
-    p = fma(p, z, -0x1.5555555555555p-3);
-    return fma(r * z, p, r);
-}
-
-PHG_HD double fem_cos(double x)
-{
-    const double k = rint(x * TrigConst::two_over_pi);
-    double r = fma(-k, TrigConst::pio2_a, x);
-    r = fma(-k, TrigConst::pio2_b, r);
-    r = fma(-k, TrigConst::pio2_c, r);
-    switch (((int)(long long)k) & 3) {
-    case 0: return taylor_cos(r);
-    case 1: return -taylor_sin(r);
-    case 2: return -taylor_cos(r);
-    default: return taylor_sin(r);
-    }
-}
-
-PHG_HD double fem_sin(double x)
-{
-    const double k = rint(x * TrigConst::two_over_pi);
-    double r = fma(-k, TrigConst::pio2_a, x);
-    r = fma(-k, TrigConst::pio2_b, r);
-    r = fma(-k, TrigConst::pio2_c, r);
-    switch (((int)(long long)k) & 3) {
-    case 0: return taylor_sin(r);
-    case 1: return taylor_cos(r);
-    case 2: return -taylor_sin(r);
-    default: return -taylor_cos(r);
-    }
-}
+// ---------------------------------------------------------------- cos ------
+// The correctly rounded cos of cr_trig.h (Ziv's method): the value every
+// correctly rounding C library returns for the reference's cos call.
+PHG_HD double fem_cos(double x) { return cr_cos(x); }
 
 // ----------------------------------------------------------------- GK15 ----
 // Kronrod nodes/weights of src/quadrature.c:64-96, in the reference's order.

@@ -21,11 +21,13 @@
  * Two functions the reference takes from outside its tree are fixed here:
  *   ipow(x, p)  - linal's pow.h is absent: repeated multiplication r *= x,
  *                 p times, from r = 1 ("parity unpinned" vs linal).
- *   cos(x)      - libm's cos is not bit-reproducible across libm builds; the
- *                 oracle uses a named method (Cody-Waite reduction by pi/2 in
- *                 three parts + Taylor polynomials to degree 16/17 on
- *                 |r| <= pi/4, Horner with fused multiply-add).  or_use_libm_cos(1)
- *                 switches to libm's cos to measure the spread this causes.
+ *   cos(x)      - the reference calls the C library's cos (src/util.cpp:
+ *                 194-215), which is not bit-reproducible across library
+ *                 builds; the oracle and the device kernels share one
+ *                 correctly rounded cos/sin (mke_amd/csrc/cr_trig.h, Ziv's
+ *                 method), pinned against 200-bit mpmath values.
+ *                 or_use_libm_cos(1) switches to the C library's cos to
+ *                 measure the spread between the two.
  */
 #ifndef PHELM_ORACLE_H
 #define PHELM_ORACLE_H
@@ -42,6 +44,15 @@ double or_cos(double x);
 double or_sin(double x);
 double or_ipow(double x, int p);
 void   or_use_libm_cos(int on);
+/* test hooks of the correctly rounded routine: the set of distinct cos
+ * arguments evaluated while buf (cap entries, a power of two) is installed
+ * (NULL removes it); the count, or minus the arguments lost to a full set */
+void   or_record_cos_args(double *buf, long cap);
+long   or_recorded_cos_args(void);
+double or_cr_cos_bound(double x, double err);
+double or_cr_sin_bound(double x, double err);
+double or_cr_slow(double x, int odd);
+void   or_cr_fast_dd(double x, int odd, double *hi, double *lo);
 
 /* --- quadrature (src/quadrature.c) --------------------------------------- */
 double or_gauss_kronrod15(double a, double b, or_fx_t f, void *d);

@@ -54,6 +54,14 @@ def lib():
         L.or_ipow.restype = _D
         L.or_ipow.argtypes = [_D, _I]
         L.or_use_libm_cos.argtypes = [_I]
+        L.or_record_cos_args.argtypes = [_PD, C.c_long]
+        L.or_recorded_cos_args.restype = C.c_long
+        for name in ("or_cr_cos_bound", "or_cr_sin_bound"):
+            getattr(L, name).restype = _D
+            getattr(L, name).argtypes = [_D, _D]
+        L.or_cr_slow.restype = _D
+        L.or_cr_slow.argtypes = [_D, _I]
+        L.or_cr_fast_dd.argtypes = [_D, _I, _PD, _PD]
         L.or_gauss_kronrod15.restype = _D
         L.or_gauss_kronrod15.argtypes = [_D, _D, FX, C.c_void_p]
         L.or_cubature7.restype = _D
