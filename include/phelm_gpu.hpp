@@ -10,6 +10,7 @@
 //   SphereJacobian   src/sjacobian.h:65-149
 //   SphereNorm       src/norm.h:58-120 with fast_scalar/norm/dist
 //                    (src/mesh.h:550-591)
+//   SphereChafe      examples/schafe.h:52-97, examples/schafe_impl.h:102-206
 //   BarVortex        examples/barvortex.h:62-155 (calc only)
 //   Baroclin         examples/baroclin.h:60-292 (calc only)
 //
@@ -443,6 +444,96 @@ private:
     ArrayDevice<T> tmp_, diff_;
 };
 
+// SphereChafe_Private::f (examples/schafe.cpp:55-63): the forcing for which
+// x sin(y + t) cos^4 x solves du/dt = mu Delta u - sigma u + f on the sphere
+// (examples/test_schafe.cpp:30-34).  ipow(c, 2) is (1 c) c, as the
+// reference's repeated multiplication gives.
+inline double schafe_f(double x, double y, double t, double mu, double sigma)
+{
+    const double c2 = (1.0 * std::cos(x)) * std::cos(x);
+    return c2 * (x * std::cos(y + t) * c2 + sigma * std::sin(y + t) * c2 * x +
+                 9 * mu * std::sin(y + t) * std::sin(x) * std::cos(x) -
+                 15 * mu * std::sin(y + t) * x + 20 * mu * std::sin(y + t) * x * c2);
+}
+
+// SphereChafe<T> (examples/schafe.h:52-97): du/dt = mu Delta u - sigma u + f,
+// one Crank-Nicolson step per solve() (examples/schafe_impl.h:126-206).
+// A_ = (1/tau + sigma/2) M - (mu/2) L and its boundary matrix bnd_
+// (schafe_integrate_cb, examples/schafe.cpp:67-89) are GPU-assembled
+// Helmholtz kinds; the right part u/tau + (mu/2) calc2(u) - (sigma/2) u + f
+// (f at the inner points' zone-0 chart coordinates, evaluated on the host as
+// the reference does), times idt_, plus bnd_ bnd; then A_ x = right part by
+// GMRES and Ans = p2u(x, bnd).  Ans may alias X0.
+template <typename T = double>
+class SphereChafe {
+public:
+    typedef double (*rp_t)(double x, double y, double t, double mu, double sigma);
+    double tau_, sigma_, mu_;
+    const Mesh& m_;
+    SphereLaplace<T> laplace_;
+    Solver<T> A_;
+    Solver<T> bnd_;
+
+    SphereChafe(const Mesh& m, double tau, double sigma, double mu, rp_t f = schafe_f)
+        : tau_(tau), sigma_(sigma), mu_(mu), m_(m), laplace_(m),
+          A_(Solver<T>::assemble(m, PHG_HELMHOLTZ, 1.0 / tau + sigma * 0.5, -0.5 * mu)),
+          bnd_(Solver<T>::assemble_boundary(m, PHG_HELMHOLTZ, 1.0 / tau + sigma * 0.5,
+                                            -0.5 * mu)),
+          f_(f), u_((size_t)m.inner_size()), du_((size_t)m.inner_size()),
+          crp_((size_t)m.inner_size()), tmp_((size_t)m.inner_size()),
+          x_((size_t)m.inner_size()), rp_((size_t)m.inner_size())
+    {
+        int V = 0, Tn = 0, ni = 0, no = 0, nz = 0;
+        check(phg_mesh_info(m.handle(), &V, &Tn, &ni, &no, &nz), "phg_mesh_info");
+        std::vector<double> coords((size_t)V * nz * 2);
+        std::vector<int> inner((size_t)std::max(ni, 1));
+        check(phg_mesh_arrays(m.handle(), nullptr, coords.data(), nullptr, nullptr, nullptr,
+                              nullptr, nullptr, inner.data(), nullptr, nullptr),
+              "phg_mesh_arrays");
+        x_in_.resize((size_t)ni);
+        y_in_.resize((size_t)ni);
+        for (int i = 0; i < ni; ++i) {
+            x_in_[i] = coords[(size_t)inner[i] * nz * 2];
+            y_in_[i] = coords[(size_t)inner[i] * nz * 2 + 1];
+        }
+    }
+    SphereChafe(const SphereChafe&) = delete;
+    SphereChafe& operator=(const SphereChafe&) = delete;
+
+    void solve(T* Ans, const T* X0, const T* bnd, double t)
+    {
+        const int rs = m_.inner_size();
+        check(phg_u2p(m_.handle(), X0, u_.data(), nullptr), "u2p");
+        laplace_.calc2(du_.data(), X0);
+        // u/dt + mu Delta u / 2, then - sigma u / 2 (vec_sum1)
+        check(phg_axpby(rs, (T)(1.0 / tau_), u_.data(), (T)(mu_ * 0.5), du_.data(), du_.data(),
+                        nullptr),
+              "vec_sum1");
+        check(phg_axpby(rs, (T)1.0, du_.data(), (T)(-sigma_ * 0.5), u_.data(), du_.data(),
+                        nullptr),
+              "vec_sum1");
+        // + f (vec_sum)
+        std::vector<double> h((size_t)rs);
+        for (int i = 0; i < rs; ++i) h[i] = f_ ? f_(x_in_[i], y_in_[i], t, mu_, sigma_) : 0.0;
+        rp_.upload(h.data());
+        check(phg_axpby(rs, 1.0, du_.data(), 1.0, rp_.data(), u_.data(), nullptr), "vec_sum");
+        // idt_ u (+ bnd_ bnd), then the inner solve and p2u
+        laplace_.idt_.mult_vector(crp_.data(), u_.data());
+        if (bnd) {
+            bnd_.mult_vector(tmp_.data(), bnd);
+            check(phg_axpby(rs, 1.0, crp_.data(), 1.0, tmp_.data(), crp_.data(), nullptr),
+                  "vec_sum");
+        }
+        A_.solve(x_.data(), crp_.data());
+        check(phg_p2u(m_.handle(), x_.data(), bnd, Ans, nullptr), "p2u");
+    }
+
+private:
+    rp_t f_;
+    ArrayDevice<T> u_, du_, crp_, tmp_, x_, rp_;
+    std::vector<double> x_in_, y_in_;
+};
+
 // BarVortexConf (examples/barvortex.h:62-96) with the reference's callbacks:
 // rp(phi, lambda, t, mu, sigma) is the right part, coriolis(phi, lambda) the
 // Coriolis + orography term l + h; either may be null.  Both are evaluated at
@@ -622,6 +713,7 @@ public:
               "phg_mesh_arrays");
         x_.resize((size_t)V);
         y_.resize((size_t)V);
+        os_ = no;
         std::vector<double> lh((size_t)V);
         for (int i = 0; i < V; ++i) {
             x_[i] = coords[(size_t)i * nz * 2];
@@ -664,14 +756,14 @@ public:
     void calc_L(double* u11, double* u21, const double* u1, const double* u2, const double* z1,
                 const double* z2, const double* bnd, double)
     {
-        (void)bnd;
+        check_zero_bnd(bnd, "Baroclin::calc_L");
         check(phg_baroclin_calc_L(h_, 1, u1, u2, z1, z2, u11, u21, &opts_, &info_, nullptr),
               "Baroclin::calc_L");
     }
     void calc_L_1(double* u11, double* u21, const double* u1, const double* u2,
                   const double* z1, const double* z2, const double* bnd, double)
     {
-        (void)bnd;
+        check_zero_bnd(bnd, "Baroclin::calc_L_1");
         check(phg_baroclin_calc_L_1(h_, 1, u1, u2, z1, z2, u11, u21, &opts_, &info_, nullptr),
               "Baroclin::calc_L_1");
     }
@@ -703,6 +795,19 @@ private:
         double sigma, mu, sigma1, mu1, alpha;
     };
     Config conf_c() const { return {conf_.sigma, conf_.mu, conf_.sigma1, conf_.mu1, conf_.alpha}; }
+    // the linearised steps run with zero boundary values: a bnd array that
+    // holds anything else is refused rather than ignored
+    void check_zero_bnd(const double* bnd, const char* what) const
+    {
+        if (!bnd || os_ == 0) return;
+        std::vector<double> h((size_t)os_);
+        check_hip(hipMemcpy(h.data(), bnd, h.size() * sizeof(double), hipMemcpyDeviceToHost),
+                  what);
+        for (double v : h)
+            if (v != 0.0)
+                throw Error(std::string(what) + ": non-zero boundary values are not supported",
+                            PHG_ERR_ARG);
+    }
     const Mesh& m_;
     rp_t f_, g_;
     phg_bc_conf conf_;
@@ -711,6 +816,7 @@ private:
     phg_baroclin_t h_ = nullptr;
     phg_bc_opts opts_{};
     phg_bc_info info_{};
+    int os_ = 0;
 };
 
 }  // namespace phelm_gpu

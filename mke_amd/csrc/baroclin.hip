@@ -605,6 +605,11 @@ int bc_linear(phg_baroclin_s* B, int which, int nm, const double* d_u1, const do
                            B->G.p, B->dmask.p);
         spmm(B, B->blk[5], B->F.p, B->RP.p, 0, s);
         spmm(B, B->blk[5], B->G.p, B->RP.p + (size_t)n * S, 0, s);
+        // the u rows of the right part are zero here (zero boundary values);
+        // a step with boundary values on this object wrote them, so clear
+        // them for every column
+        hipLaunchKernelGGL(bv_fill0, g256((size_t)2 * n * S), bb, 0, s, (size_t)2 * n * S,
+                           B->RP.p + (size_t)2 * n * S);
         hipLaunchKernelGGL(bv_zero_cols, g256((size_t)4 * n * S), bb, 0, s, (size_t)4 * n, S,
                            B->RP.p, B->dmask.p);
         hipLaunchKernelGGL(bv_zero_cols, g256((size_t)4 * n * S), bb, 0, s, (size_t)4 * n, S,

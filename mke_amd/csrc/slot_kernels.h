@@ -189,6 +189,14 @@ __global__ void bv_outer(int os, int S, const int* __restrict__ outer, const dou
     Ub[e] = mask[s] ? u[(size_t)outer[o] * S + s] : 0.0;
 }
 
+// x[e] = 0 for every e < count, every column (a fill kernel rather than a
+// memset, so that it may sit in captured work)
+__global__ void bv_fill0(size_t count, double* __restrict__ x)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < count) x[e] = 0.0;
+}
+
 __global__ void bv_set_cols(size_t rows, int S, double v, double* __restrict__ x,
                             const int* __restrict__ mask)
 {

@@ -297,6 +297,16 @@ class Mesh:
     def p2u(self, p, bnd, u, stream=None):
         check(lib().phg_p2u(self.h, _ptr(p), _ptr(bnd), _ptr(u), _stream(stream)), "phg_p2u")
 
+    def proj_bnd(self, u, b, stream=None):
+        """b[k] = u[outer[k]] (proj_bnd, src/proj.cpp:140-149)."""
+        check(lib().phg_proj_bnd(self.h, _ptr(u), _ptr(b), _stream(stream)), "phg_proj_bnd")
+
+    def set_bnd(self, u, bnd, stream=None):
+        """u's boundary points from bnd (None: zero), inner points untouched
+        (set_bnd, src/proj.cpp:109-133)."""
+        check(lib().phg_set_bnd(self.h, _ptr(bnd) if bnd is not None else None, _ptr(u),
+                                _stream(stream)), "phg_set_bnd")
+
     def close(self):
         if self.h:
             lib().phg_mesh_destroy(self.h)
@@ -589,11 +599,7 @@ class BarVortex:
         k = 1 if u.dim() == 1 else int(u.shape[0])
         if rp is not None and tuple(rp.shape) != tuple(u.shape):
             raise PhelmError("BarVortex.calc: rp must have the shape of u")
-        for b in (bnd_u, bnd_w):
-            if b is not None and (b.numel() != k * self.m.outer_size or
-                                  str(b.dtype) != "torch.float64" or not b.is_contiguous()):
-                raise PhelmError("BarVortex.calc: bnd_u/bnd_w must be contiguous FP64 with "
-                                 "outer_size values per member")
+        self._check_bnd("BarVortex.calc", k, bnd_u, bnd_w)
         infos = (BvInfo * k)()
         opt = lambda t: _ptr(t) if t is not None else None
         rc = lib().phg_barvortex_calc_bnd(self.h, k, _ptr(u), opt(rp), opt(bnd_u), opt(bnd_w),
@@ -601,6 +607,13 @@ class BarVortex:
                                           infos, _stream(stream))
         check(rc, "phg_barvortex_calc")
         return [_bv_info(i) for i in infos]
+
+    def _check_bnd(self, what, k, *bnds):
+        for b in bnds:
+            if b is not None and (b.numel() != k * self.m.outer_size or
+                                  str(b.dtype) != "torch.float64" or not b.is_contiguous()):
+                raise PhelmError("%s: bnd_u/bnd_w must be contiguous FP64 with "
+                                 "outer_size values per member" % what)
 
     def rp_values(self, fn, t):
         """conf.rp evaluated as BarVortex::calc does (barvortex_impl.h:233-244):
@@ -656,6 +669,13 @@ class BarVortex:
         return self.calc_LT(ans, F, z, opts, stream)
 
     def load(self, slot, u, stream=None, rp=None, bnd_u=None, bnd_w=None):
+        n_points = self.m.n_points
+        for t in (u, rp):
+            if t is not None and (t.numel() != n_points or str(t.dtype) != "torch.float64"
+                                  or not t.is_contiguous()):
+                raise PhelmError("BarVortex.load: u/rp must be contiguous FP64 with n_points "
+                                 "values")
+        self._check_bnd("BarVortex.load", 1, bnd_u, bnd_w)
         check(lib().phg_barvortex_load_rp(self.h, slot, _ptr(u), _ptr(rp) if rp is not None else None,
                                           _stream(stream)), "phg_barvortex_load")
         if bnd_u is not None or bnd_w is not None:
@@ -1026,6 +1046,71 @@ class SphereNorm:
         check(lib().phg_axpby(self.m.n_points, 1.0, _ptr(u), -1.0, _ptr(v), _ptr(d),
                               _stream(self.stream)), "vec_diff")
         return self.norm(d)
+
+
+def schafe_f(x, y, t, mu, sigma):
+    """SphereChafe_Private::f (examples/schafe.cpp:55-63): the forcing for
+    which x sin(y + t) cos^4 x solves du/dt = mu Delta u - sigma u + f
+    (examples/test_schafe.cpp:25-34); ipow(c, 2) = (1 c) c."""
+    x = np.asarray(x, dtype=np.float64)
+    c = np.cos(x)
+    c2 = (1.0 * c) * c
+    s = np.sin(y + t)
+    return c2 * (x * np.cos(y + t) * c2 + sigma * s * c2 * x + 9 * mu * s * np.sin(x) * c
+                 - 15 * mu * s * x + 20 * mu * s * x * c2)
+
+
+class SphereChafe:
+    """SphereChafe<T> (examples/schafe.h:52-97): du/dt = mu Delta u - sigma u
+    + f, one step per solve() (examples/schafe_impl.h:126-206).  A_ =
+    (1/tau + sigma/2) M - (mu/2) L and its boundary matrix bnd_
+    (schafe_integrate_cb, examples/schafe.cpp:67-89) are GPU-assembled; the
+    right part u/tau + (mu/2) calc2(u) - (sigma/2) u + f(x, y, t) (f at the
+    inner points' zone-0 chart coordinates, on the host as the reference
+    evaluates it), times idt_, plus bnd_ bnd; A_ x = right part by GMRES;
+    ans = p2u(x, bnd).  Vectors are FP64 device tensors."""
+
+    def __init__(self, mesh, tau, sigma, mu, f=schafe_f, stream=None):
+        self.m = mesh
+        self.tau_, self.sigma_, self.mu_ = tau, sigma, mu
+        self.f = f
+        self.laplace_ = SphereLaplace(mesh, stream=stream)
+        cm, cl = 1.0 / tau + sigma * 0.5, -0.5 * mu
+        self.A_ = Matrix.assemble(mesh, HELMHOLTZ, cm, cl, stream=stream)
+        self.bnd_ = Matrix.assemble_boundary(mesh, HELMHOLTZ, cm, cl)
+        inner = mesh.arrays()["inner"]
+        xy = mesh.zone0_xy()
+        self._x, self._y = xy[inner, 0], xy[inner, 1]
+        self.stream = stream
+
+    def solve(self, ans, X0, bnd=None, t=0.0, restart=30, rtol=1e-10, max_iters=20000,
+              ortho=MGS):
+        import torch
+        s = _stream(self.stream)
+        rs = self.m.inner_size
+        vec = lambda: torch.zeros(rs, dtype=torch.float64, device="cuda")
+        u, du, crp, x = vec(), vec(), vec(), vec()
+        L = lib()
+        check(L.phg_u2p(self.m.h, _ptr(X0), _ptr(u), s), "u2p")
+        self.laplace_.calc2(du, X0)
+        check(L.phg_axpby(rs, 1.0 / self.tau_, _ptr(u), self.mu_ * 0.5, _ptr(du), _ptr(du), s),
+              "vec_sum1")
+        check(L.phg_axpby(rs, 1.0, _ptr(du), -self.sigma_ * 0.5, _ptr(u), _ptr(du), s),
+              "vec_sum1")
+        rp = torch.as_tensor(np.ascontiguousarray(
+            self.f(self._x, self._y, t, self.mu_, self.sigma_) if self.f is not None
+            else np.zeros(rs)), dtype=torch.float64, device="cuda")
+        check(L.phg_axpby(rs, 1.0, _ptr(du), 1.0, _ptr(rp), _ptr(u), s), "vec_sum")
+        self.laplace_.idt_.mult_vector(crp, u, self.stream)
+        if bnd is not None:
+            tmp = vec()
+            self.bnd_.mult_vector(tmp, bnd, self.stream)
+            check(L.phg_axpby(rs, 1.0, _ptr(crp), 1.0, _ptr(tmp), _ptr(crp), s), "vec_sum")
+        info = self.A_.solve(x, crp, restart=restart, rtol=rtol, max_iters=max_iters,
+                             ortho=ortho, stream=self.stream)
+        check(L.phg_p2u(self.m.h, _ptr(x), _ptr(bnd) if bnd is not None else None, _ptr(ans),
+                        s), "p2u")
+        return info
 
 
 class Helmholtz:

@@ -210,3 +210,33 @@ def test_baroclin_boundary_values_vs_oracle():
     again1, again2 = torch.zeros_like(D1), torch.zeros_like(D2)
     bc.calc(again1, again2, _dev(U1), _dev(U2), _dev(f), _dev(g), opts=opts)
     assert torch.equal(again1, plain1) and torch.equal(again2, plain2)
+
+
+def test_baroclin_linear_steps_after_boundary_values():
+    """calc_L and calc_L_1 after a calc with non-zero boundary values on the
+    same object give the bits of a fresh object: the linearised steps run with
+    zero boundary values, so the right part's u rows that the boundary step
+    wrote must not leak into their solves."""
+    print("baroclin calc_L after calc with boundary values", flush=True)
+    pm, om, xy = _setup(3)
+    U1, U2, f, g = _states(xy, om)
+    L1f, lh = _lin_fields(pm, om, (1, 2))
+    L2f, _ = _lin_fields(pm, om, (3, 4))
+    Z1, _ = _lin_fields(pm, om, (5, 6))
+    Z2, _ = _lin_fields(pm, om, (7, 8))
+    tau = 2.5e-4
+    bnd = np.array([[0.02], [-0.03]]) * np.ones((2, pm.outer_size))
+    opts = P.bc_opts(precondition=1, max_solver_iters=20000)
+    used = P.Baroclin(pm, _dev(lh), tau=tau, slots=2, **PRM)
+    D1, D2 = torch.zeros_like(_dev(U1)), torch.zeros_like(_dev(U2))
+    used.calc(D1, D2, _dev(U1), _dev(U2), _dev(f), _dev(g), opts=opts, bnd=_dev(bnd))
+    fresh = P.Baroclin(pm, _dev(lh), tau=tau, slots=2, **PRM)
+    outs = []
+    for bc in (used, fresh):
+        L1, L2 = torch.zeros_like(_dev(L1f)), torch.zeros_like(_dev(L2f))
+        bc.calc_L(L1, L2, _dev(L1f), _dev(L2f), _dev(Z1), _dev(Z2))
+        B1, B2 = torch.zeros_like(L1), torch.zeros_like(L2)
+        bc.calc_L_1(B1, B2, L1, L2, _dev(Z1), _dev(Z2))
+        outs.append((L1, L2, B1, B2))
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
