@@ -6,5 +6,5 @@ mirror."""
 from .phelm import (BF16, CGS1, CGS2, DIFF1, DIFF1_COS, DIFF2, DIFF2_COS,  # noqa: F401
                     F32, F64, HELMHOLTZ, JAC_MASS, LAPLACE, MASS, MGS, Helmholtz, Matrix,
                     BarVortex, Baroclin, Mesh, Multigrid, PhelmError, Server, Solver, SphereJacobian,
-                    SphereLaplace, SphereNorm, batch_shape, bc_opts, bv_opts, set_batch_shape,
-                    helmholtz_coeffs, lib, sphere_mesh_text)
+                    SphereChafe, SphereLaplace, SphereNorm, batch_shape, bc_opts, bv_opts, set_batch_shape,
+                    helmholtz_coeffs, lib, schafe_f, sphere_mesh_text)

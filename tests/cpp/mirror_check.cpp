@@ -15,8 +15,12 @@
 // Baroclin::calc step of (u, v) with the forcing written to bc_f.bin,
 // bc_g.bin and the Coriolis term bc_lh.bin; bc_L1/2.bin = L_step of (u, v)
 // about (psi, u), bc_B1/2.bin = L_1_step of that, bc_LT1.bin = bc_L1 after
-// LT_step), inputs u.bin, v.bin, x.bin,
-// psi.bin and summary.txt (scalars).
+// LT_step), sc_u1.bin / sc_u2.bin (two SphereChafe::solve steps of the
+// reference driver's analytic case from sc_u0.bin with the boundary values
+// sc_b1.bin, sc_b2.bin), inputs u.bin, v.bin, x.bin, psi.bin and summary.txt
+// (scalars; bc_bnd_throws: Baroclin::calc_L refused non-zero boundary
+// values).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -257,6 +261,55 @@ int main(int argc, char** argv)
         dump(out, "bc_B2.bin", from_dev(k2));
         bc.LT_step(l1.data(), l2.data(), du.data(), dv.data(), dpsi.data(), du.data());
         dump(out, "bc_LT1.bin", from_dev(l1));
+        // the linearised steps refuse non-zero boundary values
+        bool bc_threw = false;
+        {
+            std::vector<double> hb((size_t)std::max(1, m.outer_size()), 0.5);
+            ArrayDevice<double> db(hb.size());
+            db.upload(hb.data());
+            try {
+                bc.calc_L(l1.data(), l2.data(), du.data(), dv.data(), dpsi.data(), du.data(),
+                          db.data(), 0.0);
+            } catch (const Error& e) {
+                bc_threw = e.code() == PHG_ERR_ARG;
+            }
+        }
+
+        // SphereChafe: two steps of the reference driver's analytic case
+        // (examples/test_schafe.cpp:61-125: tau 0.01, mu 1, sigma 70, boundary
+        // values of x sin(y + t) cos^4 x at the new time)
+        phg_gmres_info sci{};
+        {
+            SphereChafe<double> sc(m, 0.01, 70.0, 1.0);
+            sc.A_.opts().rtol = 1e-14;
+            int nz = 0, np_ = 0, nt = 0, ni = 0, no = 0;
+            check(phg_mesh_info(m.handle(), &np_, &nt, &ni, &no, &nz), "phg_mesh_info");
+            std::vector<double> coords((size_t)np_ * nz * 2);
+            std::vector<int> outer((size_t)std::max(no, 1));
+            check(phg_mesh_arrays(m.handle(), nullptr, coords.data(), nullptr, nullptr, nullptr,
+                                  nullptr, nullptr, nullptr, outer.data(), nullptr),
+                  "phg_mesh_arrays");
+            auto ans = [&](int p, double t) {
+                const double x = coords[(size_t)p * nz * 2], y = coords[(size_t)p * nz * 2 + 1];
+                const double c = std::cos(x);
+                return x * std::sin(y + t) * ((((1.0 * c) * c) * c) * c);
+            };
+            std::vector<double> u0((size_t)V), b1((size_t)no), b2((size_t)no);
+            for (int p = 0; p < V; ++p) u0[p] = ans(p, 0.0);
+            for (int k = 0; k < no; ++k) {
+                b1[k] = ans(outer[k], 0.01);
+                b2[k] = ans(outer[k], 0.02);
+            }
+            ArrayDevice<double> dU = to_dev(u0), dB1 = to_dev(b1), dB2 = to_dev(b2);
+            sc.solve(dU.data(), dU.data(), dB1.data(), 0.0);
+            dump(out, "sc_u1.bin", from_dev(dU));
+            sc.solve(dU.data(), dU.data(), dB2.data(), 0.01);
+            dump(out, "sc_u2.bin", from_dev(dU));
+            sci = sc.A_.last_info();
+            dump(out, "sc_u0.bin", u0);
+            dump(out, "sc_b1.bin", b1);
+            dump(out, "sc_b2.bin", b2);
+        }
 
         const std::string sp = out + "/summary.txt";
         FILE* f = std::fopen(sp.c_str(), "w");
@@ -272,6 +325,8 @@ int main(int argc, char** argv)
         std::fprintf(f, "bv_rp_converged %d\nbv_rp_iterations %d\n", bri.converged,
                      bri.iterations);
         std::fprintf(f, "bc_converged %d\nbc_iterations %d\n", bci.converged, bci.iterations);
+        std::fprintf(f, "bc_bnd_throws %d\n", bc_threw ? 1 : 0);
+        std::fprintf(f, "sc_rel_residual %.17g\n", sci.rel_residual);
         std::fclose(f);
         std::printf("mirror_check level %d ok\n", level);
     } catch (const std::exception& e) {

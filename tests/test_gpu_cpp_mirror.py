@@ -198,3 +198,27 @@ def test_baroclin_equals_python(run):
     for t, name in ((l1, "bc_L1.bin"), (l2, "bc_L2.bin"), (k1, "bc_B1.bin"), (k2, "bc_B2.bin")):
         assert np.array_equal(t.cpu().numpy()[0], load(name)), name
     assert np.array_equal(load("bc_LT1.bin"), load("bc_L1.bin"))
+
+
+def test_baroclin_linear_steps_refuse_boundary_values(run):
+    summ, _ = run
+    assert summ["bc_bnd_throws"] == 1
+
+
+def test_sphere_chafe_cpp_vs_oracle(run):
+    """Two SphereChafe::solve steps of the C++ class (A_ solved to 1e-14)
+    against the oracle restatement (sparse LU) from the same inputs."""
+    import oracle.schafe as S
+    from oracle import mesh_ref
+    summ, load = run
+    a = mesh_ref.load_text(mesh_ref.sphere_mesh_text(LEVEL))
+    om = O.Mesh(a)
+    xy = np.asarray(a["coords"]).reshape(len(om.p2io), -1, 2)[:, 0, :]
+    osc = S.SphereChafe(om, xy, 0.01, 70.0, 1.0)
+    U = osc.solve(load("sc_u0.bin"), load("sc_b1.bin"), 0.0)
+    g1 = load("sc_u1.bin")
+    assert S.norm_dist(om, g1, U) <= 1e-11 * S.norm_dist(om, U, 0 * U)
+    U = osc.solve(U, load("sc_b2.bin"), 0.01)
+    g2 = load("sc_u2.bin")
+    assert S.norm_dist(om, g2, U) <= 1e-11 * S.norm_dist(om, U, 0 * U)
+    assert summ["sc_rel_residual"] <= 1e-13

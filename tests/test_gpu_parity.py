@@ -235,6 +235,29 @@ def test_gather_scatter():
     assert np.array_equal(out.cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("level", [3, 5])
+def test_set_bnd_and_proj_bnd(level):
+    """set_bnd (src/proj.cpp:109-133): the boundary points of u take bnd (by
+    their outer number; NULL: zero) and the inner points keep their values;
+    proj_bnd (:140-149) reads the boundary points back."""
+    pm, om = _mesh(level)
+    a = pm.arrays()
+    rng = np.random.default_rng(level)
+    u = rng.standard_normal(pm.n_points)
+    bnd = rng.standard_normal(pm.outer_size)
+    du = _dev(u)
+    pm.set_bnd(du, _dev(bnd))
+    ref = u.copy()
+    ref[a["outer"]] = bnd
+    assert np.array_equal(du.cpu().numpy(), ref)
+    b = torch.zeros(pm.outer_size, dtype=torch.float64, device=DEV)
+    pm.proj_bnd(du, b)
+    assert np.array_equal(b.cpu().numpy(), bnd)
+    pm.set_bnd(du, None)
+    ref[a["outer"]] = 0.0
+    assert np.array_equal(du.cpu().numpy(), ref)
+
+
 def test_solver_add_path_matches_assembly():
     """Solver::add staging (generators.cpp:49-55): accumulating the oracle's
     per-element adds in row order gives the same CSR as phg_assemble."""
