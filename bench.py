@@ -4,9 +4,12 @@ Workload (one "step" = one solve): the level-7 icosahedral mesh (163,842
 vertices, 163,841 inner unknowns, 1,146,869 CSR nonzeros), system
 (M - tau L) u = M f with tau = 1e-2 assembled on the GPU, f a seeded real
 spherical-harmonic field of degree <= 32, x0 = 0, GMRES(30) to a true FP64
-relative residual of 1e-10.  `value` is plain-FP64 solves/s (the drop-in
-form); the mixed-precision and other forms are reported beside it with their
-error against the FP64 oracle solution.
+relative residual of 1e-10.  `value` is plain-FP64 MGS solves/s (the drop-in
+form, each restart cycle one cooperative kernel); the mixed-precision and
+other forms are reported beside it with their error against the FP64 oracle
+solution.  `roofline` describes that cycle kernel (GMRES(30) byte model over
+its HIP-event duration), the whole solve, and the SpMV in cache (level 7)
+and streamed from HBM (level 8).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
@@ -39,6 +42,23 @@ def sph_field(xyz, lmax, seed):
 def spmv_bytes(n, nnz, sv, sx):
     """Algorithmic bytes of one CSR SpMV with int32 indices."""
     return nnz * (sv + 4) + 4 * (n + 1) + 2 * n * sx
+
+
+def gmres_solve_bytes(n, nnz, iterations, restart=30, sv=8, sx=8):
+    """Algorithmic bytes of a GMRES(restart) solve of `iterations` Arnoldi
+    steps: step j reads the matrix and its source vector and writes w (one
+    SpMV), then touches w twice and each of v_0..v_j once (j + 5 vectors of
+    n); each cycle adds the residual (an SpMV and 2 vectors), v_0's
+    normalisation (2) and the x update (k + 2 vectors for k steps).  A full
+    30-step cycle is 31 SpMV + 621 vectors."""
+    B = spmv_bytes(n, nnz, sv, sx)
+    total, left = 0, iterations
+    while left > 0:
+        k = min(restart, left)
+        left -= k
+        total += k * B + sum(j + 5 for j in range(k)) * n * sx
+        total += B + 2 * n * sx + 2 * n * sx + (k + 2) * n * sx
+    return total
 
 
 def time_solves(A, b, x, steps, warmup, stream, **kw):
@@ -390,11 +410,73 @@ def baroclin_leg(level=5, members=8, steps=3):
     return out
 
 
+def l7_system(stream):
+    """The metric's system: level-7 Helmholtz matrix and b = M f (f of degree
+    <= 32, seed 20251202), assembled on the GPU."""
+    import torch
+
+    import mke_amd as P
+    mesh = P.Mesh.sphere(LEVEL)
+    cm, cl = P.helmholtz_coeffs(tau=1.0, sigma=0.0, theta=1.0, mu=TAU)
+    A = P.Matrix.assemble(mesh, P.HELMHOLTZ, cm, cl, stream=stream)
+    Mm = P.Matrix.assemble(mesh, P.MASS, stream=stream)
+    arrays = mesh.arrays()
+    f = sph_field(arrays["pr"], 32, 20251202)[arrays["inner"]]
+    b = torch.zeros(A.n_rows, dtype=torch.float64, device="cuda")
+    Mm.mult_vector(b, torch.tensor(f, dtype=torch.float64, device="cuda"), stream=stream)
+    torch.cuda.synchronize()
+    return mesh, A, Mm, b
+
+
+def solve_only_leg(steps, warmup, stream, only=None):
+    """The level-7 single right-hand-side forms alone: FP64 MGS as a
+    persistent cooperative cycle and as a kernel per MGS pass, FP64 CGS2,
+    mixed FP32 MGS (persistent) and CGS2."""
+    import torch
+
+    import mke_amd as P
+    mesh, A, Mm, b = l7_system(stream)
+    out = {}
+    for name, kw in (("fp64_mgs_persistent", dict(ortho=P.MGS, persistent=True)),
+                     ("fp64_mgs_per_pass", dict(ortho=P.MGS, persistent=False)),
+                     ("fp64_cgs2", dict(ortho=P.CGS2)),
+                     ("mixed_f32_mgs_persistent", dict(ortho=P.MGS, precision=P.F32)),
+                     ("mixed_f32_cgs2", dict(ortho=P.CGS2, precision=P.F32))):
+        if only and name != only:
+            continue
+        x = torch.zeros_like(b)
+        w, lat, q50, q99, inf = time_solves(A, b, x, steps, warmup, stream, restart=30,
+                                            rtol=1e-10, **kw)
+        out[name] = dict(solves_per_s=round(steps / w, 3), p50_ms=round(q50, 3),
+                         p99_ms=round(q99, 3), iterations=inf[-1]["iterations"],
+                         rel_residual=inf[-1]["rel_residual"], converged=inf[-1]["converged"])
+        launches = sum(i["cycle_launches"] for i in inf)
+        if launches:
+            out[name]["cycle_avg_launch_us"] = round(sum(i["cycle_ms"] for i in inf)
+                                                     / launches * 1e3, 3)
+            out[name]["cycle_launches_per_solve"] = launches / len(inf)
+        log("%s: %.2f solves/s, p50 %.2f ms, p99 %.2f ms, %d iters" %
+            (name, out[name]["solves_per_s"], q50, q99, inf[-1]["iterations"]))
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(rp, ci, v, b, budget_s):
-    """Oracle CSR GMRES(30)-MGS (OpenMP) on the host cores, same A and b."""
+    """Oracle CSR GMRES(30)-MGS (OpenMP) on the host cores, same A and b, for
+    about budget_s seconds on all threads and a bounded sample on one."""
     from oracle import oracle as O
-    O.lib()
-    threads = O.lib().or_num_threads()
+    L = O.lib()
+    threads = L.or_num_threads()
     t0 = time.perf_counter()
     n_solves, info = 0, None
     while True:
@@ -403,7 +485,57 @@ def cpu_baseline(rp, ci, v, b, budget_s):
         if time.perf_counter() - t0 >= budget_s:
             break
     dt = time.perf_counter() - t0
-    return n_solves / dt, threads, n_solves, dt, info
+    L.or_set_num_threads(1)
+    try:
+        t1 = time.perf_counter()
+        n1 = 0
+        while True:
+            O.gmres(rp, ci, v, b, restart=30, rtol=1e-10)
+            n1 += 1
+            if time.perf_counter() - t1 >= budget_s / 2:
+                break
+        dt1 = time.perf_counter() - t1
+    finally:
+        L.or_set_num_threads(threads)
+    return n_solves / dt, threads, n_solves, dt, info, n1 / dt1, n1, dt1
+
+
+def cpu_assembly_and_config1(threads):
+    """The oracle's row-parallel assembly (generate_matrix's OpenMP row loop)
+    of the level-7 Helmholtz matrix, with the C library's cos (the
+    reference's arithmetic) and with the correctly rounded one; and the
+    config-1 solve (level 4, f of degree <= 8, GMRES(30)-MGS to 1e-10)."""
+    from oracle import mesh_ref
+    from oracle import oracle as O
+    from tests._common import real_sph_harm_field
+    L = O.lib()
+    out = {}
+    om = O.Mesh(mesh_ref.load_text(mesh_ref.sphere_mesh_text(LEVEL)))
+    cm, cl = 1.0, -TAU
+    for key, libm in (("libm_cos", 1), ("cr_cos", 0)):
+        L.or_use_libm_cos(libm)
+        try:
+            t = time.perf_counter()
+            om.assemble(O.KIND_HELMHOLTZ, cm, cl)
+            out["assembly_L7_helmholtz_ms_" + key] = round((time.perf_counter() - t) * 1e3, 1)
+        finally:
+            L.or_use_libm_cos(0)
+    a4 = mesh_ref.load_text(mesh_ref.sphere_mesh_text(4))
+    m4 = O.Mesh(a4)
+    rp4, ci4, v4 = m4.assemble(O.KIND_HELMHOLTZ, cm, cl)
+    _, _, vm4 = m4.assemble(O.KIND_MASS)
+    import numpy as np
+    pr4 = np.asarray(a4["pr"]).reshape(-1, 3)
+    b4 = O.spmv(rp4, ci4, vm4, real_sph_harm_field(pr4, 8, 20251202)[m4.inner])
+    t = time.perf_counter()
+    k = 0
+    while time.perf_counter() - t < 1.0:
+        _, inf4 = O.gmres(rp4, ci4, v4, b4, restart=30, rtol=1e-10)
+        k += 1
+    out["config1_L4_solves_per_s"] = round(k / (time.perf_counter() - t), 2)
+    out["config1_L4_iterations"] = inf4["iterations"]
+    out["threads"] = threads
+    return out
 
 
 def main():
@@ -431,6 +563,10 @@ def main():
                          "repeated and over 8 copies (profiling)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 256 MiB device copy (calibrates rocprof byte counters)")
+    ap.add_argument("--solve-only", action="store_true",
+                    help="run only the level-7 single-RHS solve forms (timing, profiling)")
+    ap.add_argument("--form", default=None,
+                    help="with --solve-only: run this form only (e.g. fp64_mgs_persistent)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (single-GPU boxes); "
@@ -461,6 +597,12 @@ def main():
         return
     stream = torch.cuda.current_stream()
     dev = "cuda:0"
+    if args.solve_only:
+        res = solve_only_leg(args.steps, args.warmup, stream, args.form)
+        if args.calib_copy:
+            calib_copy()
+        print(json.dumps({"solve_only": res}))
+        return
 
     t = time.perf_counter()
     mesh = P.Mesh.sphere(LEVEL)
@@ -482,14 +624,26 @@ def main():
     log("assembly (Helmholtz, GPU): %.1f ms incl. host structure; n=%d nnz=%d" %
         (t_asm * 1e3, n, nnz))
 
-    # ---- timed region: plain FP64 GMRES(30), MGS (the reference form) ----
+    # ---- timed region: plain FP64 GMRES(30), MGS (the reference form), each
+    #      restart cycle one cooperative kernel (HIP events around each) ----
     wall, lat, p50, p99, infos = time_solves(A, b, x, args.steps, args.warmup, stream,
                                              restart=30, rtol=1e-10, ortho=P.MGS)
     ok = all(i["converged"] and i["rel_residual"] <= 1e-10 for i in infos)
     iters = infos[-1]["iterations"]
     solves_s = args.steps / wall
-    log("fp64-mgs: %.2f solves/s, p50 %.2f ms, p99 %.2f ms, %d iters, rel %.2e" %
-        (solves_s, p50, p99, iters, infos[-1]["rel_residual"]))
+    launches = sum(i["cycle_launches"] for i in infos)
+    cyc_ms = sum(i["cycle_ms"] for i in infos)
+    solve_bytes = gmres_solve_bytes(n, nnz, iters)
+    cyc = None
+    if launches:
+        per_solve = launches / len(infos)
+        cyc = dict(avg_launch_us=round(cyc_ms / launches * 1e3, 3),
+                   launches_per_solve=per_solve,
+                   bytes_per_launch=round(solve_bytes / per_solve),
+                   cycle_kernel_ms_per_solve=round(cyc_ms / len(infos), 4))
+        cyc["achieved"] = round(cyc["bytes_per_launch"] / (cyc_ms / launches * 1e-3) / 1e9, 1)
+    log("fp64-mgs: %.2f solves/s, p50 %.2f ms, p99 %.2f ms, %d iters, rel %.2e, cycles %s" %
+        (solves_s, p50, p99, iters, infos[-1]["rel_residual"], cyc))
     x64 = x.cpu().numpy()
 
     # ---- roofline of the FP64 SpMV (Solver::mult_vector), right after the
@@ -509,7 +663,8 @@ def main():
     # ---- variants ----
     variants = {}
     if not args.no_variants:
-        for name, kw in (("fp64_cgs2", dict(ortho=P.CGS2)),
+        for name, kw in (("fp64_mgs_per_pass", dict(ortho=P.MGS, persistent=False)),
+                         ("fp64_cgs2", dict(ortho=P.CGS2)),
                          ("mixed_f32_cgs2", dict(ortho=P.CGS2, precision=P.F32)),
                          ("mixed_f32_mgs", dict(ortho=P.MGS, precision=P.F32)),
                          ("mixed_bf16_cgs2", dict(ortho=P.CGS2, precision=P.BF16))):
@@ -673,11 +828,17 @@ def main():
                 fwd[name] = float(np.linalg.norm(d["x"].cpu().numpy() - xt) / nt)
         if batched is not None:
             batched["fwd_err_vs_oracle_col0"] = float(np.linalg.norm(batched["x0"] - xt) / nt)
-        rate, threads, ns, dt, cinfo = cpu_baseline(rp, ci, v, bh, args.cpu_budget)
+        rate, threads, ns, dt, cinfo, rate1, n1, dt1 = cpu_baseline(rp, ci, v, bh,
+                                                                    args.cpu_budget)
+        extra = cpu_assembly_and_config1(threads)
         cpu = {"value": round(rate, 4), "unit": "solves/s", "cores": threads, "kind": "port",
                "sample": "%d oracle FP64 GMRES(30)-MGS solves of the same L7 system "
-                         "(%d iterations each) in %.1f s, OpenMP" % (ns, cinfo["iterations"], dt)}
-        log("cpu baseline: %.3f solves/s on %d threads" % (rate, threads))
+                         "(%d iterations each) in %.1f s, OpenMP on %d threads; %d in %.1f s "
+                         "on 1 thread" % (ns, cinfo["iterations"], dt, threads, n1, dt1),
+               "cpu_model": cpu_model(), "value_1_thread": round(rate1, 4)}
+        cpu.update(extra)
+        log("cpu baseline: %.3f solves/s on %d threads, %.3f on 1; %s" %
+            (rate, threads, rate1, extra))
     for d in variants.values():
         d.pop("x", None)
     if batched is not None:
@@ -711,25 +872,46 @@ def main():
         "fwd_err_vs_oracle": fwd.get("fp64_mgs"),
         "assembly_ms": round(t_asm * 1e3, 3),
         "variants": variants,
-        "roofline": {"bound": "hbm", "kernel": "ell_spmv_f64 (Solver::mult_vector)",
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles("ell_spmv_f64_L7_traffic.json"),
-                     "bytes_per_launch": bspmv, "avg_launch_us": round(spmv_ms * 1e3, 3)},
+        "roofline": None,
         "cpu_baseline": cpu,
     }
+    # the timed region's dominant kernel: the persistent MGS restart cycle
+    # (one launch per cycle), algorithmic bytes of the GMRES(30) model over
+    # its average duration from the HIP events around each launch
+    whole = dict(bytes_per_solve=solve_bytes,
+                 achieved=round(solve_bytes / (wall / args.steps) / 1e9, 1))
+    whole["frac"] = round(whole["achieved"] / HBM_PEAK_GBS, 4)
+    spmv = {"kernel": "ell_spmv_f64 (Solver::mult_vector), level 7, in the Infinity Cache",
+            "bytes_per_launch": bspmv, "avg_launch_us": round(spmv_ms * 1e3, 3),
+            "achieved": round(achieved, 1), "frac_in_cache": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles("ell_spmv_f64_L7_traffic.json")}
     l8s = variants.get("L8_spmv_fp64")
     if l8s:
-        # the same kernel family where the matrix must stream from HBM
-        out["roofline"]["level8"] = dict(
+        # the HBM figure: level 8, the matrix streamed from HBM (8 copies)
+        spmv["level8_stream"] = dict(
             kernel="ell_spmv_f64_stream", bytes_per_launch=l8s["bytes_per_launch"],
+            streamed_us=l8s["cold_us"], streamed_achieved=l8s["cold_gbs"],
+            streamed_frac=l8s["cold_frac"], streamed_copies=l8s["copies"],
             repeated_achieved=l8s["warm_gbs"], repeated_frac=l8s["warm_frac"],
-            streamed_achieved=l8s["cold_gbs"], streamed_frac=l8s["cold_frac"],
-            streamed_copies=l8s["copies"])
+            traffic=traffic_from_profiles("ell_spmv_f64_stream_L8_traffic.json"))
     if cold_ms is not None:
-        out["roofline"].update(cold_achieved=round(cold_gbs, 1),
-                               cold_frac=round(cold_gbs / HBM_PEAK_GBS, 4),
-                               cold_avg_launch_us=round(cold_ms * 1e3, 3))
+        spmv.update(cold_achieved=round(cold_gbs, 1), cold_frac=round(cold_gbs / HBM_PEAK_GBS, 4),
+                    cold_avg_launch_us=round(cold_ms * 1e3, 3))
+    if cyc is not None:
+        out["roofline"] = {
+            "bound": "hbm", "kernel": "k_mgs_cycle<double,double> (one GMRES(30) restart cycle, "
+                                      "MGS; latency-bound chain of dependent global sums)",
+            "achieved": cyc["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(cyc["achieved"] / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles("k_mgs_cycle_L7_traffic.json"),
+            "bytes_per_launch": cyc["bytes_per_launch"], "avg_launch_us": cyc["avg_launch_us"],
+            "launches_per_solve": cyc["launches_per_solve"],
+            "whole_solve": whole, "spmv": spmv}
+    else:
+        out["roofline"] = {"bound": "hbm", "kernel": spmv["kernel"], "achieved": achieved,
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": spmv["traffic"],
+                           "whole_solve": whole, "spmv": spmv}
     print(json.dumps(out))
 
 

@@ -183,6 +183,13 @@ typedef struct {
     int diag_scale;  /* 1: right-precondition with the inverse diagonal, x = D^-1 y
                         (the batched FP64 solves without another preconditioner;
                         the stopping test stays on ||b - A x|| / ||b||)    */
+    int persistent;  /* phg_gmres with PHG_MGS: 1 (default) runs each restart
+                        cycle as one cooperative kernel whose workgroups
+                        exchange the dot partials through device memory; 0
+                        launches a kernel per MGS pass (captured in a
+                        hipGraph).  Both give the same bits and, measured at
+                        level 7, the same speed; 1 falls back to 0 where
+                        the device cannot keep every workgroup resident   */
 } phg_gmres_opts;
 
 typedef struct {
@@ -191,6 +198,10 @@ typedef struct {
     int refinements;      /* outer refinement steps (mixed forms)         */
     double rel_residual;  /* true FP64 ||b - A x|| / ||b|| at exit          */
     int converged;
+    int cycle_launches;   /* persistent MGS cycles: kernels launched, and  */
+    double cycle_ms;      /* their summed duration (HIP events recorded on
+                             the solve's stream around each launch); 0
+                             for the other forms                        */
 } phg_gmres_info;
 
 void phg_gmres_default_opts(phg_gmres_opts *o);

@@ -434,6 +434,7 @@ void phg_gmres_default_opts(phg_gmres_opts* o)
     o->use_x0 = 0;
     o->graph = 1;
     o->diag_scale = 0;
+    o->persistent = 1;
 }
 
 int phg_gmres(phg_matrix_t A, const double* d_b, double* d_x, const phg_gmres_opts* opts,

@@ -51,6 +51,7 @@ struct GmresState {
     double hc[GM_MAXM + 1];             // CGS2 coefficients of the current pass
     double beta, bnorm, rel, rtol, inner_rtol;
     int m, active, k, iters, iters0, max_iters, converged, cycles;
+    int abort;                          // persistent cycle: an exchange timed out
 };
 
 struct GmresWork {
@@ -59,6 +60,9 @@ struct GmresWork {
     DevBuf<char> V, W;     // (m+1) x ldv and 2 x ldv vectors of the Krylov type
     DevBuf<double> part;   // 2 x (m+2) x G partial sums (two regions for CGS2)
     DevBuf<GmresState> st;
+    DevBuf<unsigned long long> xch;  // persistent cycle: exchange rows x G slots
+    int xch_rows = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};  // around each persistent cycle launch
     GmresState* host = nullptr;  // pinned mirror
     // captured restart cycle
     hipStream_t cap = nullptr;
@@ -69,6 +73,8 @@ struct GmresWork {
     {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (cap) (void)hipStreamDestroy(cap);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
         if (host) (void)hipHostFree(host);
     }
 };
@@ -577,6 +583,464 @@ __global__ void k_fill0(int n, double* __restrict__ x)
     if (i < n) x[i] = 0.0;
 }
 
+// ------------------------------------------------- persistent MGS cycle --
+// One cooperative launch per restart cycle: every workgroup is resident, and
+// the global sums of the MGS passes go through device memory instead of
+// kernel boundaries.  Exchange e of the cycle owns row e of xch (G 64-bit
+// slots, set to XCH_EMPTY before the launch): each workgroup stores its
+// block partial into its slot with an agent-scope relaxed atomic store, and
+// every workgroup waits until no slot of the row holds XCH_EMPTY, then adds
+// the G partials in sum_partials' fixed order.  A row is written once per
+// launch, so no slot is reused while another workgroup may still read it.
+// The vectors other workgroups gather (the next Arnoldi vector, x before the
+// residual) are written with agent-scope atomic stores that complete
+// (s_waitcnt vmcnt(0)) before the partial announcing them is stored, and are
+// gathered with agent-scope atomic loads; no cache-wide writeback or
+// invalidation is needed.  Every wait is bounded: after XCH_SPIN polls a
+// workgroup raises GmresState::abort and leaves, and the others leave at
+// their next poll; the host reports the abort as an error.
+//
+// Each workgroup keeps the cycle's Givens state (current Hessenberg column,
+// rotations, rotated right-hand side, R) in LDS and computes it from the
+// same exchanged sums as every other workgroup; workgroup 0 also records it
+// in GmresState.  The arithmetic is that of k_step_spmv, k_mgs, k_solve_y,
+// k_xupdate, k_residual and k_cycle_start, in the same order over the same
+// block partials, so the solve gives the same bits as the kernel-per-pass
+// form.
+typedef unsigned long long xch_t;
+constexpr xch_t XCH_EMPTY = 0x7FF4DEADBEEF0001ull;  // a signalling-NaN pattern no partial takes
+constexpr long XCH_SPIN = 1L << 22;                  // polls before a wait gives up
+// (PHG_XCH_SPIN overrides it: the failure-path test forces a give-up with
+// a limit of a few polls; every workgroup still exits normally)
+
+template <typename T>
+__device__ __forceinline__ T ld_agent(const T* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_agent(T* p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ell_row_f64 / ell_row_f32 with x gathered by agent-scope loads (same order
+// of operations)
+__device__ __forceinline__ double ell_row_f64_agent(int n, int width, int row,
+                                                    const int* __restrict__ col,
+                                                    const double* __restrict__ val,
+                                                    const double* x, double scale, bool scaled)
+{
+    double s = 0.0;
+    if (width <= ELL_MAXW) {
+        int c[ELL_MAXW];
+        double v[ELL_MAXW], xv[ELL_MAXW];
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k) {
+            const size_t o = (size_t)k * n + row;
+            c[k] = k < width ? col[o] : 0;
+            v[k] = k < width ? val[o] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k) xv[k] = k < width ? ld_agent(x + c[k]) : 0.0;
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k)
+            if (k < width) s += v[k] * (scaled ? xv[k] * scale : xv[k]);
+    } else {
+        for (int k = 0; k < width; ++k) {
+            const size_t o = (size_t)k * n + row;
+            const double xk = ld_agent(x + col[o]);
+            s += val[o] * (scaled ? xk * scale : xk);
+        }
+    }
+    return s;
+}
+
+template <typename MT>
+__device__ __forceinline__ float ell_row_f32_agent(int n, int width, int row,
+                                                   const int* __restrict__ col,
+                                                   const MT* __restrict__ val, const float* x,
+                                                   float scale)
+{
+    float s = 0.0f;
+    if (width <= ELL_MAXW) {
+        int c[ELL_MAXW];
+        MT v[ELL_MAXW];
+        float xv[ELL_MAXW];
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k) {
+            const size_t o = (size_t)k * n + row;
+            c[k] = k < width ? col[o] : 0;
+            v[k] = k < width ? val[o] : MT(0);
+        }
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k) xv[k] = k < width ? ld_agent(x + c[k]) : 0.0f;
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k)
+            if (k < width) s = fmaf(ell_val_f32(v[k]), xv[k] * scale, s);
+    } else {
+        for (int k = 0; k < width; ++k) {
+            const size_t o = (size_t)k * n + row;
+            s = fmaf(ell_val_f32(val[o]), ld_agent(x + col[o]) * scale, s);
+        }
+    }
+    return s;
+}
+
+template <typename VT, typename MT>
+struct RowDotAgent;
+template <>
+struct RowDotAgent<double, double> {
+    __device__ __forceinline__ static double run(int n, int width, int row, const int* col,
+                                                 const double* val, const double* x, double inv)
+    {
+        return ell_row_f64_agent(n, width, row, col, val, x, inv, true);
+    }
+};
+template <typename MT>
+struct RowDotAgent<float, MT> {
+    __device__ __forceinline__ static float run(int n, int width, int row, const int* col,
+                                                const MT* val, const float* x, float inv)
+    {
+        return ell_row_f32_agent<MT>(n, width, row, col, val, x, inv);
+    }
+};
+
+// Wave 0: once every slot of the row is written, the sum of its G partials
+// in sum_partials' order (the same value in every lane).  false when the
+// wait gives up or another workgroup gave up.
+__device__ bool xch_sum(const xch_t* row, int G, int* abort_flag, long spin, double* out)
+{
+    const int lane = threadIdx.x & 63;
+    for (long it = 0; it < spin; ++it) {
+        double v[GM_MAXG / 64];
+        bool all = true;
+#pragma unroll
+        for (int k = 0; k < GM_MAXG / 64; ++k) {
+            const int b = lane + 64 * k;
+            xch_t u = 0;  // the bits of +0.0, as sum_partials adds beyond G
+            if (b < G) u = ld_agent(row + b);
+            all = all && (u != XCH_EMPTY);
+            v[k] = __longlong_as_double((long long)u);
+        }
+        if (__all(all)) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < GM_MAXG / 64; ++k) s += v[k];
+            *out = wave_sum(s);
+            return true;
+        }
+        if ((it & 63) == 63 && ld_agent(abort_flag)) return false;
+    }
+    if (lane == 0) st_agent(abort_flag, 1);
+    return false;
+}
+
+struct XchLds {
+    double val;
+    int ok;
+};
+
+// All threads: the workgroup's partial (block_sum of acc: wave sums, then
+// the four waves in order) into its slot of row, and the row's sum in *sum.
+// With publish, every thread's earlier stores complete first (the vector
+// the partial announces); without it, loads already issued (the next
+// pass's basis rows) stay in flight while wave 0 waits.
+__device__ __forceinline__ bool xch_block(xch_t* row, int G, double acc, bool publish,
+                                          int* abort_flag, long spin, double* sum, double* sh,
+                                          XchLds* l)
+{
+    const double v = wave_sum(acc);
+    if (publish) __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this thread's stores are complete
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        if (threadIdx.x == 0) {
+            double r = 0.0;
+            for (int q = 0; q < GM_BS / 64; ++q) r += sh[q];
+            st_agent(row + lblock(), (xch_t)__double_as_longlong(r));
+        }
+        double s = 0.0;
+        const bool ok = xch_sum(row, G, abort_flag, spin, &s);
+        if (threadIdx.x == 0) {
+            l->val = s;
+            l->ok = ok;
+        }
+    }
+    __syncthreads();
+    *sum = l->val;
+    return l->ok;
+}
+
+// givens_step on the LDS copy of the cycle's state: same arithmetic; lane 0
+// stores, and in workgroup 0 also records into GmresState.
+__device__ GivensOut givens_lds(int j, double s, int m, double stop, int iters0, int max_iters,
+                                const double* sH, double* s_cs, double* s_sn, double* s_g,
+                                double* s_carry, double* s_R, GmresState* __restrict__ st,
+                                bool rec)
+{
+    const int lane = threadIdx.x & 63;
+    const double hreg = lane <= j ? sH[lane] : 0.0;
+    const double creg = lane < j ? s_cs[lane] : 0.0;
+    const double sreg = lane < j ? s_sn[lane] : 0.0;
+    const double hn = sqrt(s);
+    double cur = readlane_d(hreg, 0);
+    for (int i = 0; i < j; ++i) {
+        const double c = readlane_d(creg, i), sn = readlane_d(sreg, i);
+        const double nxt = readlane_d(hreg, i + 1);
+        const double t = c * cur + sn * nxt;
+        cur = -sn * cur + c * nxt;
+        if (lane == 0) {
+            s_R[i * m + j] = t;
+            if (rec) st->R[i * m + j] = t;
+        }
+    }
+    const double r = hypot(cur, hn);
+    const double cj = cur / r, sj = hn / r;
+    const double carry = s_carry[j];
+    const double gj = cj * carry, cn = -sj * carry;
+    const int iters = iters0 + j + 1;
+    const bool cont = !(fabs(cn) <= stop || !(hn > 0.0) || iters >= max_iters || j + 1 == m);
+    if (lane == 0) {
+        s_R[j * m + j] = r;
+        s_cs[j] = cj;
+        s_sn[j] = sj;
+        s_g[j] = gj;
+        s_carry[j + 1] = cn;
+        if (rec) {
+            st->R[j * m + j] = r;
+            st->H[(j + 1) * m + j] = hn;
+            st->cs[j] = cj;
+            st->sn[j] = sj;
+            st->g[j] = gj;
+            st->carry[j + 1] = cn;
+            st->k = j + 1;
+            st->iters = iters;
+            if (!cont) st->active = 0;
+        }
+    }
+    return GivensOut{cont, hn};
+}
+
+// One restart cycle (Arnoldi steps with MGS, y, x += V y, r = b - A x and
+// the decision of the next cycle) as one cooperative launch of G workgroups.
+template <typename VT, typename MT>
+__global__ void __launch_bounds__(GM_BS)
+k_mgs_cycle(int n, int rpt, int width, const int* __restrict__ col, const MT* __restrict__ val,
+            int width64, const int* __restrict__ col64, const double* __restrict__ val64,
+            const double* __restrict__ b, double* x, VT* V, size_t ldv, VT* W0, VT* W1,
+            xch_t* xch, int G, long spin, GmresState* __restrict__ st)
+{
+    __shared__ double sh[GM_BS / 64];
+    __shared__ XchLds xl;
+    __shared__ double sH[GM_MAXM + 1];
+    __shared__ double s_cs[GM_MAXM], s_sn[GM_MAXM], s_g[GM_MAXM], s_carry[GM_MAXM + 1];
+    __shared__ double s_R[GM_MAXM * GM_MAXM];
+    __shared__ double s_y[GM_MAXM];
+    __shared__ double s_hn;
+    __shared__ int s_cont;
+    if (!st->active) return;  // the previous launch's decision, the same for all
+    int* abort_flag = &st->abort;
+    const int m = st->m, iters0 = st->iters0, max_iters = st->max_iters;
+    const double stop = fmax(st->rtol * st->bnorm, st->inner_rtol * st->beta);
+    const bool rec = lblock() == 0;
+    if (threadIdx.x == 0) s_carry[0] = st->carry[0];
+    const int base = lblock() * GM_BS * rpt + threadIdx.x;
+    xch_t* row = xch;
+    double h = st->beta;
+    int k = 0;
+    bool ok = true;
+    for (int j = 0; j < m; ++j) {
+        VT* Wsrc = (j & 1) ? W1 : W0;
+        VT* Wdst = (j & 1) ? W0 : W1;
+        const VT inv = (VT)(1.0 / h);
+        VT* Vj = V + (size_t)j * ldv;
+        VT w[GM_RPT], vj[GM_RPT];
+#pragma unroll
+        for (int r = 0; r < GM_RPT; ++r) {
+            const int rr = base + r * GM_BS;
+            w[r] = VT(0);
+            vj[r] = VT(0);
+            if (r < rpt && rr < n) {
+                vj[r] = ld_agent(Wsrc + rr) * inv;
+                Vj[rr] = vj[r];
+                w[r] = RowDotAgent<VT, MT>::run(n, width, rr, col, val, Wsrc, inv);
+            }
+        }
+        // pv = v_{i-1}, iv = v_i of the coming pass i (own rows; v_j from
+        // registers); pass i + 1 reuses iv as its pv, so each pass loads one
+        // basis vector, issued before the exchange it does not depend on
+        VT pv[GM_RPT], iv[GM_RPT];
+        double acc = 0.0;
+#pragma unroll
+        for (int r = 0; r < GM_RPT; ++r) {
+            const int rr = base + r * GM_BS;
+            const bool in = r < rpt && rr < n;
+            pv[r] = in ? (j == 0 ? vj[r] : V[rr]) : VT(0);
+            iv[r] = (in && j >= 1) ? (j == 1 ? vj[r] : V[ldv + rr]) : VT(0);
+        }
+        // partial of <w, v_0> (block_dots with one dot)
+#pragma unroll
+        for (int r = 0; r < GM_RPT; ++r) {
+            const int rr = base + r * GM_BS;
+            if (r < rpt && rr < n) acc += to_d(w[r]) * to_d(pv[r]);
+        }
+        double hsum = 0.0;
+        ok = xch_block(row, G, acc, false, abort_flag, spin, &hsum, sh, &xl);
+        row += G;
+        if (!ok) break;
+        if (threadIdx.x == 0) {
+            sH[0] = hsum;
+            if (rec) st->H[j] = hsum;
+        }
+        // MGS passes i = 1..j+1: w -= h_{i-1} v_{i-1}, then <w, v_i> or ||w||^2
+        bool cont = false;
+        for (int i = 1; i <= j + 1 && ok; ++i) {
+            const bool last = (i == j + 1);
+            const VT hv = (VT)hsum;
+            double a2 = 0.0;
+#pragma unroll
+            for (int r = 0; r < GM_RPT; ++r) {
+                const int rr = base + r * GM_BS;
+                if (r < rpt && rr < n) {
+                    const VT wr = w[r] - hv * pv[r];
+                    w[r] = wr;
+                    const double o = last ? to_d(wr) : to_d(iv[r]);
+                    a2 += to_d(wr) * o;
+                    if (last) st_agent(Wdst + rr, wr);
+                }
+            }
+            // v_{i+1} for the next pass
+            VT nv[GM_RPT];
+#pragma unroll
+            for (int r = 0; r < GM_RPT; ++r) {
+                const int rr = base + r * GM_BS;
+                nv[r] = (r < rpt && rr < n && i + 1 <= j)
+                            ? (i + 1 == j ? vj[r] : V[(size_t)(i + 1) * ldv + rr]) : VT(0);
+            }
+            double s = 0.0;
+            ok = xch_block(row, G, a2, last, abort_flag, spin, &s, sh, &xl);
+            row += G;
+            if (!ok) break;
+            if (!last) {
+                hsum = s;
+                if (threadIdx.x == 0) {
+                    sH[i] = s;
+                    if (rec) st->H[i * m + j] = s;
+                }
+#pragma unroll
+                for (int r = 0; r < GM_RPT; ++r) {
+                    pv[r] = iv[r];
+                    iv[r] = nv[r];
+                }
+            } else {
+                if (threadIdx.x < 64) {
+                    const GivensOut g = givens_lds(j, s, m, stop, iters0, max_iters, sH, s_cs, s_sn,
+                                                   s_g, s_carry, s_R, st, rec);
+                    if (threadIdx.x == 0) {
+                        s_hn = g.hn;
+                        s_cont = g.cont;
+                    }
+                }
+                __syncthreads();
+                h = s_hn;
+                cont = s_cont;
+            }
+        }
+        if (!ok) break;
+        k = j + 1;
+        if (!cont) break;
+    }
+    if (!ok) return;
+    // y = R^-1 g (k_solve_y), in every workgroup
+    if (threadIdx.x == 0) {
+        for (int i = k - 1; i >= 0; --i) {
+            double s = s_g[i];
+            for (int l = i + 1; l < k; ++l) s -= s_R[i * m + l] * s_y[l];
+            s_y[i] = s / s_R[i * m + i];
+        }
+        if (rec) {
+            for (int i = 0; i < k; ++i) st->y[i] = s_y[i];
+            st->active = 0;
+        }
+    }
+    __syncthreads();
+    // x += V y (k_xupdate), stored for the residual's gathers
+    for (int r = 0; r < rpt; ++r) {
+        const int rr = base + r * GM_BS;
+        if (rr >= n) break;
+        double s = 0.0;
+        for (int i = 0; i < k; ++i) s += to_d(V[(size_t)i * ldv + rr]) * s_y[i];
+        st_agent(x + rr, x[rr] + s);
+    }
+    double dummy = 0.0;
+    ok = xch_block(row, G, 0.0, true, abort_flag, spin, &dummy, sh, &xl);
+    row += G;
+    if (!ok) return;
+    // r = b - A x (k_residual), W0 = r, ||r||^2
+    double acc = 0.0;
+    for (int r = 0; r < rpt; ++r) {
+        const int rr = base + r * GM_BS;
+        if (rr >= n) break;
+        const double sx = ell_row_f64_agent(n, width64, rr, col64, val64, x, 1.0, false);
+        const double res = b[rr] - sx;
+        W0[rr] = (VT)res;
+        acc += res * res;
+    }
+    double s2 = 0.0;
+    ok = xch_block(row, G, acc, false, abort_flag, spin, &s2, sh, &xl);
+    if (!ok) return;
+    // the next cycle's start (k_cycle_start), recorded by workgroup 0
+    if (rec && threadIdx.x == 0) {
+        const double beta = sqrt(s2);
+        st->beta = beta;
+        st->rel = beta / st->bnorm;
+        st->k = 0;
+        st->iters0 = st->iters;
+        if (st->rel <= st->rtol || beta == 0.0) {
+            st->converged = 1;
+            st->active = 0;
+        } else if (st->iters >= st->max_iters) {
+            st->active = 0;
+        } else {
+            st->active = 1;
+            st->carry[0] = beta;
+            st->cycles += 1;
+        }
+    }
+}
+
+// Sets the first `count` exchange slots to XCH_EMPTY and clears the abort flag.
+__global__ void k_xch_reset(xch_t* xch, size_t count, GmresState* st)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) xch[i] = XCH_EMPTY;
+    if (i == 0) st->abort = 0;
+}
+
+// Exchange rows one cycle of restart m uses: j + 2 per Arnoldi step, one
+// before and one after the residual.
+inline int xch_rows_for(int m) { return m * (m + 3) / 2 + 2; }
+
+// Whether every workgroup of the cycle kernel can be resident at once.
+template <typename VT, typename MT>
+bool coop_fits(int G)
+{
+    static int capacity = -1;  // per template instance: blocks resident at once
+    if (capacity < 0) {
+        int dev = 0, cus = 0, coop = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per, (const void*)k_mgs_cycle<VT, MT>, GM_BS, 0) != hipSuccess)
+            capacity = 0;
+        else
+            capacity = coop ? cus * per : 0;
+    }
+    return G <= capacity;
+}
+
 // ---------------------------------------------------------------- driver --
 int prepare_work(phg_matrix_s* A, int prec, int m, GmresWork** out)
 {
@@ -666,6 +1130,18 @@ int gmres_run(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double* b, 
     GmresState* st = w->st.p;
     const dim3 grid(G), block(GM_BS);
     const bool cgs = o.ortho == PHG_CGS2;
+    // MGS cycles as one cooperative kernel when every workgroup fits at once
+    const bool coop = !cgs && o.persistent && coop_fits<VT, MT>(G);
+    if (coop && w->xch_rows < xch_rows_for(m)) {
+        PHG_HIP(w->xch.alloc((size_t)xch_rows_for(m) * G));
+        w->xch_rows = xch_rows_for(m);
+    }
+    if (coop && !w->ev[0]) {
+        PHG_HIP(hipEventCreate(&w->ev[0]));
+        PHG_HIP(hipEventCreate(&w->ev[1]));
+    }
+    int launches = 0;
+    double cycle_ms = 0.0;
 
     // parameters of the solve
     GmresState init;
@@ -683,7 +1159,7 @@ int gmres_run(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double* b, 
     hipLaunchKernelGGL(k_cycle_start, dim3(1), dim3(64), 0, s, st, part, G);
     PHG_HIP(hipGetLastError());
 
-    if (o.graph) {
+    if (o.graph && !coop) {
         const void* key[4] = {b, x, E.val.p, A->ell64.val.p};
         const bool same = w->exec && w->key_m == m && w->key_ortho == o.ortho &&
                           !memcmp(key, w->key, sizeof key);
@@ -709,9 +1185,42 @@ int gmres_run(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double* b, 
     for (;;) {
         PHG_HIP(hipMemcpyAsync(w->host, st, sizeof(GmresState), hipMemcpyDeviceToHost, s));
         PHG_HIP(hipStreamSynchronize(s));
+        if (coop && launches > 0) {
+            float ms = 0.0f;
+            PHG_HIP(hipEventElapsedTime(&ms, w->ev[0], w->ev[1]));
+            cycle_ms += ms;
+        }
         const GmresState& h = *w->host;
+        if (h.abort) {
+            set_error("gmres: a workgroup of the persistent MGS cycle timed out waiting for "
+                      "the others");
+            return PHG_ERR_HIP;
+        }
         if (h.converged || !h.active) break;
-        if (o.graph) {
+        if (coop) {
+            const size_t cnt = (size_t)xch_rows_for(m) * G;
+            hipLaunchKernelGGL(k_xch_reset, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
+                               w->xch.p, cnt, st);
+            PHG_HIP(hipGetLastError());
+            int n_ = n, rpt_ = rpt, width = E.width, width64 = A->ell64.width, G_ = G;
+            const int* col = E.col.p;
+            const MT* val = E.val.p;
+            const int* col64 = A->ell64.col.p;
+            const double* val64 = A->ell64.val.p;
+            VT* V = (VT*)w->V.p;
+            VT* W1 = W0 + w->ldv;
+            size_t ldv = w->ldv;
+            xch_t* xch = w->xch.p;
+            long spin = XCH_SPIN;
+            if (const char* e = getenv("PHG_XCH_SPIN")) spin = std::max(1L, atol(e));
+            void* args[] = {&n_, &rpt_, &width, &col, &val, &width64, &col64, &val64, &b, &x,
+                            &V, &ldv, &W0, &W1, &xch, &G_, &spin, &st};
+            PHG_HIP(hipEventRecord(w->ev[0], s));
+            PHG_HIP(hipLaunchCooperativeKernel((const void*)k_mgs_cycle<VT, MT>, dim3(G),
+                                               dim3(GM_BS), args, 0, s));
+            PHG_HIP(hipEventRecord(w->ev[1], s));
+            ++launches;
+        } else if (o.graph) {
             PHG_HIP(hipGraphLaunch(w->exec, s));
         } else {
             enqueue_cycle<VT, MT>(A, E, w, b, x, m, cgs, s);
@@ -725,6 +1234,8 @@ int gmres_run(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double* b, 
         info->refinements = o.precision == PHG_F64 ? 0 : h.cycles;
         info->rel_residual = h.rel;
         info->converged = h.converged;
+        info->cycle_launches = launches;
+        info->cycle_ms = cycle_ms;
     }
     return h.converged ? PHG_OK : PHG_ERR_NOCONV;
 }

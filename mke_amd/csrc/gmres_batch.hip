@@ -971,6 +971,7 @@ int batch_run_t(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const do
         const BSum& h = w->host[q];
         all = all && h.converged;
         if (infos) {
+            memset(&infos[q], 0, sizeof infos[q]);
             infos[q].iterations = h.iters;
             infos[q].restarts = h.cycles;
             infos[q].refinements = o.precision == PHG_F64 ? 0 : h.cycles;

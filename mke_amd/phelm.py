@@ -32,12 +32,13 @@ _PI, _PD = C.POINTER(C.c_int), C.POINTER(C.c_double)
 class GmresOpts(C.Structure):
     _fields_ = [("restart", _I), ("max_iters", _I), ("rtol", _D), ("ortho", _I),
                 ("precision", _I), ("inner_rtol", _D), ("use_x0", _I), ("graph", _I),
-                ("diag_scale", _I)]
+                ("diag_scale", _I), ("persistent", _I)]
 
 
 class GmresInfo(C.Structure):
     _fields_ = [("iterations", _I), ("restarts", _I), ("refinements", _I),
-                ("rel_residual", _D), ("converged", _I)]
+                ("rel_residual", _D), ("converged", _I), ("cycle_launches", _I),
+                ("cycle_ms", _D)]
 
 
 class BlockInfo(C.Structure):
@@ -392,19 +393,23 @@ class Matrix:
               "phg_spmm_f32")
 
     def solve(self, x, b, restart=30, rtol=1e-10, max_iters=200000, ortho=MGS, precision=F64,
-              inner_rtol=1e-5, use_x0=False, graph=True, stream=None):
-        """Solves A x = b (Solver::solve) with GMRES(restart); returns the info dict."""
+              inner_rtol=1e-5, use_x0=False, graph=True, persistent=True, stream=None):
+        """Solves A x = b (Solver::solve) with GMRES(restart); returns the info dict.
+        persistent: MGS restart cycles as one cooperative kernel (same bits as
+        a kernel per MGS pass)."""
         o = GmresOpts()
         lib().phg_gmres_default_opts(C.byref(o))
         o.restart, o.max_iters, o.rtol = restart, max_iters, rtol
         o.ortho, o.precision, o.inner_rtol, o.use_x0 = ortho, precision, inner_rtol, int(use_x0)
         o.graph = int(graph)
+        o.persistent = int(persistent)
         info = GmresInfo()
         check(lib().phg_gmres(self.h, _ptr(b), _ptr(x), C.byref(o), C.byref(info),
                               _stream(stream)), "phg_gmres")
         return dict(iterations=info.iterations, restarts=info.restarts,
                     refinements=info.refinements, rel_residual=info.rel_residual,
-                    converged=bool(info.converged))
+                    converged=bool(info.converged), cycle_launches=info.cycle_launches,
+                    cycle_ms=info.cycle_ms)
 
     def solve_block(self, X, B, restart=30, rtol=1e-10, max_iters=100000, precision=F32,
                     inner_rtol=1e-5, use_x0=False, ortho=CGS2, stream=None):

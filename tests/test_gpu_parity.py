@@ -307,3 +307,53 @@ def test_dump_operator_matches_oracle(op, kind, cm, cl, tmp_path):
     rp, ci, v = oracle_matrix(3, kind, cm, cl)
     assert np.array_equal(got.indptr, rp) and np.array_equal(got.indices, ci)
     assert np.array_equal(got.data, v)
+
+
+@pytest.mark.parametrize("precision", [P.F64, P.F32])
+@pytest.mark.parametrize("level", [4, 5, 7])
+def test_gmres_persistent_cycle_same_bits(level, precision):
+    """The persistent MGS cycle (one cooperative kernel per restart cycle,
+    dot partials exchanged through device memory) gives the bits of the
+    kernel-per-pass form, FP64 and mixed FP32, with the same iteration and
+    restart counts; two runs of it give the same bits."""
+    print("gmres persistent vs per-pass level=%d precision=%d" % (level, precision), flush=True)
+    pm, _ = _mesh(level)
+    A = P.Matrix.assemble(pm, P.HELMHOLTZ, CM, CL)
+    Mm = P.Matrix.assemble(pm, P.MASS)
+    f = _rhs(pm, level, 32 if level == 7 else 8, 99)
+    b = torch.zeros(A.n_rows, dtype=torch.float64, device=DEV)
+    Mm.mult_vector(b, _dev(f))
+    xs, infos = [], []
+    for persistent in (False, True, True):
+        x = torch.zeros_like(b)
+        infos.append(A.solve(x, b, restart=30, rtol=1e-10, ortho=P.MGS, precision=precision,
+                             persistent=persistent))
+        xs.append(x)
+    assert infos[0]["converged"] and infos[0]["rel_residual"] <= 1e-10
+    keys = ("iterations", "restarts", "refinements", "rel_residual", "converged")
+    for i in (1, 2):
+        assert all(infos[i][k] == infos[0][k] for k in keys), (infos[i], infos[0])
+        assert torch.equal(xs[i], xs[0])
+    # one cooperative launch per restart cycle, timed by HIP events
+    assert infos[0]["cycle_launches"] == 0
+    assert infos[1]["cycle_launches"] == infos[1]["restarts"] and infos[1]["cycle_ms"] > 0
+
+
+def test_gmres_persistent_cycle_gives_up_cleanly(monkeypatch):
+    """A wait of the persistent cycle that gives up (forced with a limit of
+    one poll per exchange) leaves every workgroup, and the solve reports an
+    error; the next solve with the normal limit is unaffected."""
+    print("gmres persistent give-up path", flush=True)
+    pm, _ = _mesh(6)
+    A = P.Matrix.assemble(pm, P.HELMHOLTZ, CM, CL)
+    b = _dev(np.random.default_rng(3).standard_normal(A.n_rows))
+    x0 = torch.zeros_like(b)
+    ref = A.solve(x0, b, rtol=1e-10, ortho=P.MGS, persistent=False)
+    monkeypatch.setenv("PHG_XCH_SPIN", "1")
+    x = torch.zeros_like(b)
+    with pytest.raises(P.PhelmError, match="timed out"):
+        A.solve(x, b, rtol=1e-10, ortho=P.MGS)
+    monkeypatch.delenv("PHG_XCH_SPIN")
+    x = torch.zeros_like(b)
+    info = A.solve(x, b, rtol=1e-10, ortho=P.MGS)
+    assert info["iterations"] == ref["iterations"] and torch.equal(x, x0)
