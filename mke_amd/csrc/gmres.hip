@@ -27,6 +27,7 @@
 // Reductions are deterministic: every block writes one partial per dot, and
 // the consumer adds the partials in block order.  Results depend only on the
 // inputs and the launch shape, never on timing.
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -1022,6 +1023,14 @@ __global__ void k_xch_reset(xch_t* xch, size_t count, GmresState* st)
 // before and one after the residual.
 inline int xch_rows_for(int m) { return m * (m + 3) / 2 + 2; }
 
+// Set once a cooperative launch was refused by the runtime: later solves of
+// the process take the kernel-per-pass form directly.
+inline std::atomic<bool>& coop_refused()
+{
+    static std::atomic<bool> refused{false};
+    return refused;
+}
+
 // Whether every workgroup of the cycle kernel can be resident at once.
 template <typename VT, typename MT>
 bool coop_fits(int G)
@@ -1131,7 +1140,7 @@ int gmres_run(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double* b, 
     const dim3 grid(G), block(GM_BS);
     const bool cgs = o.ortho == PHG_CGS2;
     // MGS cycles as one cooperative kernel when every workgroup fits at once
-    const bool coop = !cgs && o.persistent && coop_fits<VT, MT>(G);
+    bool coop = !cgs && o.persistent && !coop_refused().load() && coop_fits<VT, MT>(G);
     if (coop && w->xch_rows < xch_rows_for(m)) {
         PHG_HIP(w->xch.alloc((size_t)xch_rows_for(m) * G));
         w->xch_rows = xch_rows_for(m);
@@ -1159,7 +1168,10 @@ int gmres_run(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double* b, 
     hipLaunchKernelGGL(k_cycle_start, dim3(1), dim3(64), 0, s, st, part, G);
     PHG_HIP(hipGetLastError());
 
+    // the captured cycle; replayed only when built (or found current) here
+    bool graph_ready = false;
     if (o.graph && !coop) {
+        graph_ready = true;
         const void* key[4] = {b, x, E.val.p, A->ell64.val.p};
         const bool same = w->exec && w->key_m == m && w->key_ortho == o.ortho &&
                           !memcmp(key, w->key, sizeof key);
@@ -1216,11 +1228,22 @@ int gmres_run(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double* b, 
             void* args[] = {&n_, &rpt_, &width, &col, &val, &width64, &col64, &val64, &b, &x,
                             &V, &ldv, &W0, &W1, &xch, &G_, &spin, &st};
             PHG_HIP(hipEventRecord(w->ev[0], s));
-            PHG_HIP(hipLaunchCooperativeKernel((const void*)k_mgs_cycle<VT, MT>, dim3(G),
-                                               dim3(GM_BS), args, 0, s));
-            PHG_HIP(hipEventRecord(w->ev[1], s));
-            ++launches;
-        } else if (o.graph) {
+            const hipError_t e = hipLaunchCooperativeKernel((const void*)k_mgs_cycle<VT, MT>,
+                                                            dim3(G), dim3(GM_BS), args, 0, s);
+            if (e == hipSuccess) {
+                PHG_HIP(hipEventRecord(w->ev[1], s));
+                ++launches;
+            } else {
+                // a runtime that refuses the cooperative launch here: the
+                // kernel-per-pass form (same bits) for this solve and, from
+                // now on, for this process
+                (void)hipGetLastError();
+                coop_refused().store(true);
+                coop = false;
+                enqueue_cycle<VT, MT>(A, E, w, b, x, m, cgs, s);
+                PHG_HIP(hipGetLastError());
+            }
+        } else if (graph_ready) {
             PHG_HIP(hipGraphLaunch(w->exec, s));
         } else {
             enqueue_cycle<VT, MT>(A, E, w, b, x, m, cgs, s);
