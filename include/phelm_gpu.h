@@ -153,6 +153,19 @@ int phg_spmm_f32(phg_matrix_t A, int prec, int nrhs, const float *d_X,
 int phg_dot(int n, const double *d_x, const double *d_y, double *result,
             phg_stream_t s);   /* result: host pointer, synchronises */
 int phg_nrm2(int n, const double *d_x, double *result, phg_stream_t s);
+/* Allocation-free dot product.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (work captured into a HIP
+ * graph, tight serving loops, harnesses that forbid hipMalloc).  *d_result
+ * (a device pointer) = x . y, ordered on the stream, with no host
+ * synchronisation and no hipMalloc/hipFree: the block partials live in a
+ * 32 KiB device array of the library (one per GPU, held until the process
+ * ends), which limits n to PHG_DOT_NOALLOC_MAX (larger n: PHG_ERR_ARG).  All
+ * calls on one GPU share that array, so calls that overlap in time on one GPU
+ * race; order them on one stream.  Deterministic; the summation order
+ * differs from phg_dot's, whose partials are added on the host. */
+#define PHG_DOT_NOALLOC_MAX 4194304
+int phg_dot_noalloc(int n, const double *d_x, const double *d_y, double *d_result,
+                    phg_stream_t s);
 /* r = k1 * a + k2 * b  (vec_sum1) */
 int phg_axpby(int n, double k1, const double *d_a, double k2,
               const double *d_b, double *d_r, phg_stream_t s);

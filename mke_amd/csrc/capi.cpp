@@ -17,6 +17,8 @@ int spmv_f64(phg_matrix_s* A, const double* x, double* y, hipStream_t s);
 int spmv_f32(phg_matrix_s* A, int prec, const float* x, float* y, hipStream_t s);
 int spmm_f32(phg_matrix_s* A, int prec, int nrhs, const float* X, float* Y, hipStream_t s);
 int dot_f64(int n, const double* x, const double* y, double* result, hipStream_t s);
+int dot_noalloc_f64(int n, const double* x, const double* y, double* result, hipStream_t s);
+int noalloc_scratch(double** out);
 int axpby_f64(int n, double k1, const double* a, double k2, const double* b, double* r,
               hipStream_t s);
 int u2p_f64(phg_mesh_s* m, const double* u, double* p, hipStream_t s);
@@ -77,7 +79,9 @@ const char* phg_last_error(void) { return g_error.c_str(); }
 int phg_init(int device)
 {
     PHG_HIP(hipSetDevice(device));
-    return PHG_OK;
+    // the device's phg_dot_noalloc scratch address, looked up here once
+    double* scratch = nullptr;
+    return noalloc_scratch(&scratch);
 }
 
 void phg_free(void* p) { free(p); }
@@ -375,6 +379,12 @@ int phg_dot(int n, const double* d_x, const double* d_y, double* result, phg_str
 {
     if (n < 0 || !result) return PHG_ERR_ARG;
     return dot_f64(n, d_x, d_y, result, (hipStream_t)s);
+}
+
+int phg_dot_noalloc(int n, const double* d_x, const double* d_y, double* d_result,
+                    phg_stream_t s)
+{
+    return dot_noalloc_f64(n, d_x, d_y, d_result, (hipStream_t)s);
 }
 
 int phg_nrm2(int n, const double* d_x, double* result, phg_stream_t s)

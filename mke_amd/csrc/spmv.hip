@@ -8,6 +8,8 @@
 // FP64 products are accumulated in CSR column order from 0.0 without fused
 // multiply-add (this file is compiled with -ffp-contract=off), so y = A x is
 // bit-identical to a plain CPU CSR loop.
+#include <atomic>
+
 #include "ell.h"
 #include "phg_internal.h"
 
@@ -364,6 +366,102 @@ int dot_f64(int n, const double* x, const double* y, double* result, hipStream_t
     double r = 0.0;
     for (double v : h) r += v;
     *result = r;
+    return PHG_OK;
+}
+
+// ---- phg_dot_noalloc: the dot product without allocating -----------------
+// Its block partials live in `scratch`, a file-scope device array: the code
+// object holds one copy per GPU from load until the process ends.  Host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// current device (kept per device in scratch_ptr); the array's name is not a
+// device address on the host.  Every call on a GPU shares the array, so
+// calls that overlap in time on one GPU race.  Each call rewrites every
+// partial it reads (one per workgroup launched), so nothing needs clearing.
+constexpr int NOALLOC_DOT_ELEMS = 1024;  // entries per workgroup: 256 threads x 4
+constexpr int NOALLOC_DOT_MAXG = PHG_DOT_NOALLOC_MAX / NOALLOC_DOT_ELEMS;  // 4096 partials
+static __device__ __align__(256) double scratch[NOALLOC_DOT_MAXG];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::atomic<double*> scratch_ptr[NOALLOC_MAX_DEVICES];
+
+__device__ __forceinline__ double dot_wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// part[g] = sum of x[i] y[i] over i = g*1024 + t + 256 r (t < 256, r < 4):
+// per thread in r order, xor-butterfly wave sums, then the four waves in order
+__global__ void __launch_bounds__(256)
+dot_partial_noalloc(int n, const double* __restrict__ x, const double* __restrict__ y,
+                    double* __restrict__ part)
+{
+    __shared__ double sh[4];
+    const int base = blockIdx.x * NOALLOC_DOT_ELEMS + threadIdx.x;
+    double s = 0.0;
+#pragma unroll
+    for (int r = 0; r < NOALLOC_DOT_ELEMS / 256; ++r) {
+        const int i = base + 256 * r;
+        if (i < n) s += x[i] * y[i];
+    }
+    s = dot_wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// result[0] = the G partials: thread t adds t, t + 256, ... in order, then
+// wave sums and the four waves in order (deterministic)
+__global__ void __launch_bounds__(256)
+dot_finish_noalloc(int G, const double* __restrict__ part, double* __restrict__ result)
+{
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int g = threadIdx.x; g < G; g += 256) s += part[g];
+    s = dot_wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) result[0] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// The scratch address on the current device, looked up once per device.
+int noalloc_scratch(double** out)
+{
+    int dev = 0;
+    PHG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+        set_error("phg_dot_noalloc: device index beyond the 64 this library tracks");
+        return PHG_ERR_ARG;
+    }
+    double* p = scratch_ptr[dev].load(std::memory_order_acquire);
+    if (!p) {
+        void* q = nullptr;
+        PHG_HIP(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+        p = (double*)q;
+        scratch_ptr[dev].store(p, std::memory_order_release);
+    }
+    *out = p;
+    return PHG_OK;
+}
+
+int dot_noalloc_f64(int n, const double* x, const double* y, double* result, hipStream_t s)
+{
+    if (n < 0 || n > PHG_DOT_NOALLOC_MAX) {
+        set_error("phg_dot_noalloc: n must be in 0..4194304 (the 4096 block partials its "
+                  "32 KiB device scratch holds)");
+        return PHG_ERR_ARG;
+    }
+    if (!result || (n > 0 && (!x || !y))) {
+        set_error("phg_dot_noalloc: null argument");
+        return PHG_ERR_ARG;
+    }
+    double* part = nullptr;
+    int rc = noalloc_scratch(&part);
+    if (rc) return rc;
+    const int G = std::max(1, (n + NOALLOC_DOT_ELEMS - 1) / NOALLOC_DOT_ELEMS);
+    hipLaunchKernelGGL(dot_partial_noalloc, dim3(G), dim3(256), 0, s, n, x, y, part);
+    hipLaunchKernelGGL(dot_finish_noalloc, dim3(1), dim3(256), 0, s, G, part, result);
+    PHG_HIP(hipGetLastError());
     return PHG_OK;
 }
 

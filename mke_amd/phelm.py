@@ -70,6 +70,7 @@ SIGNATURES = {
     "phg_spmv_f32": (_I, [_P, _I, _P, _P, _P]),
     "phg_spmm_f32": (_I, [_P, _I, _I, _P, _P, _P]),
     "phg_dot": (_I, [_I, _P, _P, _PD, _P]),
+    "phg_dot_noalloc": (_I, [_I, _P, _P, _P, _P]),
     "phg_nrm2": (_I, [_I, _P, _PD, _P]),
     "phg_axpby": (_I, [_I, _D, _P, _D, _P, _P, _P]),
     "phg_u2p": (_I, [_P, _P, _P, _P]),

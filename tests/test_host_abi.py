@@ -111,3 +111,17 @@ def test_argument_errors_without_a_gpu():
     lh = C.c_void_p(1)  # never dereferenced: the slot count is refused first
     assert L.phg_barvortex_create(m.h, C.byref(conf), lh, 3, None, C.byref(h)) == ERR_ARG
     assert "slots" in L.phg_last_error().decode()
+
+
+def test_dot_noalloc_refuses_bad_sizes_before_device_work():
+    """phg_dot_noalloc checks n against the 4096 block partials of its
+    device scratch (n <= 4,194,304) and its pointers before any device work,
+    so this runs without a GPU."""
+    import ctypes as C
+    L = P.lib()
+    dummy = C.c_void_p(8)
+    assert L.phg_dot_noalloc(4194305, dummy, dummy, dummy, None) == -1
+    assert b"4194304" in L.phg_last_error()
+    assert L.phg_dot_noalloc(-1, dummy, dummy, dummy, None) == -1
+    assert L.phg_dot_noalloc(16, dummy, dummy, None, None) == -1
+    assert b"null" in L.phg_last_error()
