@@ -116,7 +116,8 @@ def traffic_from_profiles(kernel_file):
     return d.get("traffic_bytes_per_launch")
 
 
-def serve_leg(n_requests, rate, seed=20251202, verbose=False):
+def serve_leg(n_requests, rate, seed=20251202, verbose=False, load_fracs=(0.5, 0.8, 0.95),
+              load_requests=128):
     """Ensemble serving: 256 members on levels 5/6/7/8 (40/30/20/10 %), each
     request one BarVortex implicit step (dt = 600 s, Earth-scaled units) of
     its member; Poisson arrivals at `rate` req/s.  The initial state of a
@@ -218,6 +219,25 @@ def serve_leg(n_requests, rate, seed=20251202, verbose=False):
         log("%s: %.2f steps/s, p50 %.0f ms, p99 %.0f ms, %.2f members/pass" %
             (name, st["throughput"], st["p50_latency_s"] * 1e3, st["p99_latency_s"] * 1e3,
              st["mean_occupancy"]))
+        # latency below saturation: Poisson arrivals at fractions of the
+        # throughput the burst measured (the same members and levels)
+        loads = []
+        for frac in load_fracs:
+            r_l = frac * st["throughput"]
+            mem_l = rng.integers(0, n_members, load_requests)
+            arr_l = np.cumsum(rng.exponential(1.0 / r_l, load_requests))
+            _, fin_l, inf_l, st_l = srv.run(mem_l, arr_l)
+            loads.append(dict(offered_frac=frac, offered_req_per_s=round(r_l, 3),
+                              requests=load_requests,
+                              steps_per_s=round(st_l["throughput"], 3),
+                              p50_ms=round(st_l["p50_latency_s"] * 1e3, 2),
+                              p99_ms=round(st_l["p99_latency_s"] * 1e3, 2),
+                              p50_service_ms=round(st_l["p50_service_s"] * 1e3, 2),
+                              mean_members_per_pass=round(st_l["mean_occupancy"], 3),
+                              converged=int(sum(i["converged"] for i in inf_l))))
+            log("%s at %.0f%% of capacity (%.1f req/s): p50 %.0f ms, p99 %.0f ms" %
+                (name, frac * 100, r_l, loads[-1]["p50_ms"], loads[-1]["p99_ms"]))
+        out[name]["offered_loads"] = loads
         srv.close()
     return out
 

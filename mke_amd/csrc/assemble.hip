@@ -42,11 +42,8 @@ struct AsmPlan {
 void csr_structure(const SphereMesh& m, std::vector<int>& rp, std::vector<int>& ci)
 {
     const int rs = (int)m.inner.size();
-    rp.assign(rs + 1, 0);
-    ci.clear();
-    ci.reserve((size_t)rs * 7);
-    std::vector<int> tmp;
-    for (int i = 0; i < rs; ++i) {
+    // the sorted distinct inner columns of row i (into tmp)
+    auto row_cols = [&m](int i, std::vector<int>& tmp) {
         const int p = m.inner[i];
         tmp.clear();
         for (int e = m.adj_ptr[p]; e < m.adj_ptr[p + 1]; ++e) {
@@ -58,9 +55,25 @@ void csr_structure(const SphereMesh& m, std::vector<int>& rp, std::vector<int>& 
         }
         std::sort(tmp.begin(), tmp.end());
         tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-        ci.insert(ci.end(), tmp.begin(), tmp.end());
-        rp[i + 1] = (int)ci.size();
-    }
+    };
+    // counts, prefix sum, then the columns, rows in parallel
+    rp.assign(rs + 1, 0);
+    parallel_ranges(rs, [&](int lo, int hi) {
+        std::vector<int> tmp;
+        for (int i = lo; i < hi; ++i) {
+            row_cols(i, tmp);
+            rp[i + 1] = (int)tmp.size();
+        }
+    });
+    for (int i = 0; i < rs; ++i) rp[i + 1] += rp[i];
+    ci.assign((size_t)rp[rs], 0);
+    parallel_ranges(rs, [&](int lo, int hi) {
+        std::vector<int> tmp;
+        for (int i = lo; i < hi; ++i) {
+            row_cols(i, tmp);
+            std::copy(tmp.begin(), tmp.end(), ci.begin() + rp[i]);
+        }
+    });
 }
 
 namespace {
@@ -81,28 +94,30 @@ void build_plan(const SphereMesh& m, AsmPlan& P)
     P.slot_tri.resize(S);
     P.slot_local.resize(S);
     P.pos.resize((size_t)S * 3);
-    for (int i = 0; i < rs; ++i) {
-        const int p = m.inner[i];
-        int s = P.slot_ptr[i];
-        for (int e = m.adj_ptr[p]; e < m.adj_ptr[p + 1]; ++e, ++s) {
-            const int t = m.adj[e];
-            P.slot_row[s] = i;
-            P.slot_tri[s] = t;
-            int loc = 0;
-            while (m.tri[3 * t + loc] != p) ++loc;  // Triangle::elem1(p1, zone)
-            P.slot_local[s] = loc;
-            for (int a = 0; a < 3; ++a) {
-                const int q = m.tri[3 * t + a];
-                int where = -1;
-                if (!m.is_boundary[q]) {
-                    const int j = m.p2io[q];
-                    for (int c = P.rp[i]; c < P.rp[i + 1]; ++c)
-                        if (P.ci[c] == j) where = c;
+    parallel_ranges(rs, [&](int lo, int hi) {
+        for (int i = lo; i < hi; ++i) {
+            const int p = m.inner[i];
+            int s = P.slot_ptr[i];
+            for (int e = m.adj_ptr[p]; e < m.adj_ptr[p + 1]; ++e, ++s) {
+                const int t = m.adj[e];
+                P.slot_row[s] = i;
+                P.slot_tri[s] = t;
+                int loc = 0;
+                while (m.tri[3 * t + loc] != p) ++loc;  // Triangle::elem1(p1, zone)
+                P.slot_local[s] = loc;
+                for (int a = 0; a < 3; ++a) {
+                    const int q = m.tri[3 * t + a];
+                    int where = -1;
+                    if (!m.is_boundary[q]) {
+                        const int j = m.p2io[q];
+                        for (int c = P.rp[i]; c < P.rp[i + 1]; ++c)
+                            if (P.ci[c] == j) where = c;
+                    }
+                    P.pos[(size_t)s * 3 + a] = where;
                 }
-                P.pos[(size_t)s * 3 + a] = where;
             }
         }
-    }
+    });
 }
 
 // generate_full_matrix (src/generators.h:200-234): every point is a row, in

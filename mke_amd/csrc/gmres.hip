@@ -1035,18 +1035,17 @@ inline std::atomic<bool>& coop_refused()
 template <typename VT, typename MT>
 bool coop_fits(int G)
 {
-    static int capacity = -1;  // per template instance: blocks resident at once
-    if (capacity < 0) {
+    // per template instance, computed once (thread-safe static initialisation)
+    static const int capacity = [] {
         int dev = 0, cus = 0, coop = 0, per = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
             hipOccupancyMaxActiveBlocksPerMultiprocessor(
                 &per, (const void*)k_mgs_cycle<VT, MT>, GM_BS, 0) != hipSuccess)
-            capacity = 0;
-        else
-            capacity = coop ? cus * per : 0;
-    }
+            return 0;
+        return coop ? cus * per : 0;
+    }();
     return G <= capacity;
 }
 

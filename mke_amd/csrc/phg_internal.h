@@ -2,9 +2,11 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/phelm_gpu.h"
@@ -13,6 +15,33 @@
 namespace phg {
 
 void set_error(const std::string& msg);
+
+// f(lo, hi) over [0, n) split into contiguous ranges on up to 16 host
+// threads (host-side integer work of the assembly plans; each range writes
+// only its own rows, so the result does not depend on the split).
+template <typename F>
+void parallel_ranges(int n, F f)
+{
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int T = (int)std::min(16u, std::max(1u, hw));
+    if (n < 8192 || T == 1) {
+        f(0, n);
+        return;
+    }
+    const int chunk = (n + T - 1) / T;
+    std::vector<std::thread> th;
+    th.reserve(T);
+    for (int t = 0; t < T; ++t) {
+        const int lo = t * chunk, hi = std::min(n, lo + chunk);
+        if (lo >= hi) continue;
+        try {
+            th.emplace_back(f, lo, hi);
+        } catch (...) {
+            f(lo, hi);  // no thread to be had: this range on the caller's
+        }
+    }
+    for (auto& x : th) x.join();
+}
 
 #define PHG_HIP(call)                                                          \
     do {                                                                       \

@@ -356,15 +356,28 @@ int spmm_f32(phg_matrix_s* A, int prec, int nrhs, const float* X, float* Y, hipS
 int dot_f64(int n, const double* x, const double* y, double* result, hipStream_t s)
 {
     const int G = std::max(1, std::min(256, (n + 255) / 256));
-    DevBuf<double> part;
-    PHG_HIP(part.alloc(G));
-    hipLaunchKernelGGL(dot_partial, dim3(G), dim3(256), 0, s, n, x, y, part.p);
+    // the 256 partials: a device buffer and a pinned host mirror per calling
+    // thread and device, allocated on the thread's first call on the device
+    // and kept (a call never allocates again; threads never share them)
+    constexpr int MAXDEV = 64;
+    thread_local double* part[MAXDEV] = {};
+    thread_local double* host[MAXDEV] = {};
+    int dev = 0;
+    PHG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAXDEV) {
+        set_error("phg_dot: device index beyond the 64 this library tracks");
+        return PHG_ERR_ARG;
+    }
+    if (!part[dev]) {
+        PHG_HIP(hipMalloc((void**)&part[dev], 256 * sizeof(double)));
+        PHG_HIP(hipHostMalloc((void**)&host[dev], 256 * sizeof(double), hipHostMallocDefault));
+    }
+    hipLaunchKernelGGL(dot_partial, dim3(G), dim3(256), 0, s, n, x, y, part[dev]);
     PHG_HIP(hipGetLastError());
-    std::vector<double> h(G);
-    PHG_HIP(hipMemcpyAsync(h.data(), part.p, G * sizeof(double), hipMemcpyDeviceToHost, s));
+    PHG_HIP(hipMemcpyAsync(host[dev], part[dev], G * sizeof(double), hipMemcpyDeviceToHost, s));
     PHG_HIP(hipStreamSynchronize(s));
     double r = 0.0;
-    for (double v : h) r += v;
+    for (int g = 0; g < G; ++g) r += host[dev][g];
     *result = r;
     return PHG_OK;
 }

@@ -317,6 +317,7 @@ int main(int argc, char** argv)
         std::fprintf(f, "level %d\npoints %d\ninner %d\nnnz %ld\n", level, V, n, nnz);
         std::fprintf(f, "solve_converged %d\nsolve_iterations %d\nsolve_rel %.17g\n", si.converged,
                      si.iterations, si.rel_residual);
+        std::fprintf(f, "solve_cycle_launches %d\n", si.cycle_launches);
         std::fprintf(f, "lap_solve_converged %d\nlap_solve_rel %.17g\n", li.converged,
                      li.rel_residual);
         std::fprintf(f, "norm_u %.17g\nscalar_uv %.17g\ndist_uv %.17g\n", nu, suv, duv);
