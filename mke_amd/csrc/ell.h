@@ -68,6 +68,19 @@ __device__ __forceinline__ float ell_val_f32<uint16_t>(uint16_t v)
     return __uint_as_float(((uint32_t)v) << 16);
 }
 
+// Offset of slot k of row `row` in a slot-major ELL of width 1..ELL_MAXW
+// (callers check width >= 1: an empty ELL has no slot to read), clamped to
+// the row's last slot.  The unrolled row loops below issue every
+// slot's loads without a branch (a slot beyond the width reads the last
+// slot again, in bounds) and add only the products of slots k < width, so
+// the sum is the same as a loop over the width; with no branch between the
+// loads the compiler can wait for each gather's column alone instead of
+// for every load issued before it.
+__device__ __forceinline__ size_t ell_slot(int k, int width, int n, int row)
+{
+    return (size_t)(k < width ? k : width - 1) * n + row;
+}
+
 // s = sum_k val[k] * (x[col[k]] * scale), FP64 (no fused multiply-add; the
 // including file is compiled with -ffp-contract=off).  scale = 1 multiplies
 // exactly.
@@ -78,20 +91,22 @@ __device__ __forceinline__ double ell_row_f64(int n, int width, int row,
                                               bool scaled)
 {
     double s = 0.0;
-    if (width <= ELL_MAXW) {
+    if (width >= 1 && width <= ELL_MAXW) {
         int c[ELL_MAXW];
         double v[ELL_MAXW], xv[ELL_MAXW];
 #pragma unroll
         for (int k = 0; k < ELL_MAXW; ++k) {
-            const size_t o = (size_t)k * n + row;
-            c[k] = k < width ? col[o] : 0;
-            v[k] = k < width ? val[o] : 0.0;
+            const size_t o = ell_slot(k, width, n, row);
+            c[k] = col[o];
+            v[k] = val[o];
         }
 #pragma unroll
-        for (int k = 0; k < ELL_MAXW; ++k) xv[k] = k < width ? x[c[k]] : 0.0;
+        for (int k = 0; k < ELL_MAXW; ++k) xv[k] = x[c[k]];
 #pragma unroll
-        for (int k = 0; k < ELL_MAXW; ++k)
-            if (k < width) s += v[k] * (scaled ? xv[k] * scale : xv[k]);
+        for (int k = 0; k < ELL_MAXW; ++k) {
+            const double t = s + v[k] * (scaled ? xv[k] * scale : xv[k]);
+            s = k < width ? t : s;
+        }
     } else {
         for (int k = 0; k < width; ++k) {
             const size_t o = (size_t)k * n + row;
@@ -109,21 +124,23 @@ __device__ __forceinline__ float ell_row_f32(int n, int width, int row,
                                              const float* __restrict__ x, float scale)
 {
     float s = 0.0f;
-    if (width <= ELL_MAXW) {
+    if (width >= 1 && width <= ELL_MAXW) {
         int c[ELL_MAXW];
         MT v[ELL_MAXW];
         float xv[ELL_MAXW];
 #pragma unroll
         for (int k = 0; k < ELL_MAXW; ++k) {
-            const size_t o = (size_t)k * n + row;
-            c[k] = k < width ? col[o] : 0;
-            v[k] = k < width ? val[o] : MT(0);
+            const size_t o = ell_slot(k, width, n, row);
+            c[k] = col[o];
+            v[k] = val[o];
         }
 #pragma unroll
-        for (int k = 0; k < ELL_MAXW; ++k) xv[k] = k < width ? x[c[k]] : 0.0f;
+        for (int k = 0; k < ELL_MAXW; ++k) xv[k] = x[c[k]];
 #pragma unroll
-        for (int k = 0; k < ELL_MAXW; ++k)
-            if (k < width) s = fmaf(ell_val_f32(v[k]), xv[k] * scale, s);
+        for (int k = 0; k < ELL_MAXW; ++k) {
+            const float t = fmaf(ell_val_f32(v[k]), xv[k] * scale, s);
+            s = k < width ? t : s;
+        }
     } else {
         for (int k = 0; k < width; ++k) {
             const size_t o = (size_t)k * n + row;

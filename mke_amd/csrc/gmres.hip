@@ -40,6 +40,7 @@ namespace phg {
 constexpr int GM_MAXM = 64;
 constexpr int GM_BS = 256;
 constexpr int GM_RPT = 4;     // max rows per thread
+constexpr int GM_RG = 4;      // rows whose Arnoldi SpMV loads are issued together
 constexpr int GM_MAXG = 1024; // max blocks
 
 struct GmresState {
@@ -153,6 +154,212 @@ struct RowDot<float, MT> {
         return ell_row_f32<MT>(n, width, row, col, val, x, inv);
     }
 };
+
+template <typename T>
+__device__ __forceinline__ T ld_agent(const T* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_agent(T* p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ell_row_f64 / ell_row_f32 with x gathered by agent-scope loads (same order
+// of operations)
+__device__ __forceinline__ double ell_row_f64_agent(int n, int width, int row,
+                                                    const int* __restrict__ col,
+                                                    const double* __restrict__ val,
+                                                    const double* x, double scale, bool scaled)
+{
+    double s = 0.0;
+    if (width >= 1 && width <= ELL_MAXW) {
+        int c[ELL_MAXW];
+        double v[ELL_MAXW], xv[ELL_MAXW];
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k) {
+            const size_t o = ell_slot(k, width, n, row);
+            c[k] = col[o];
+            v[k] = val[o];
+        }
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k) xv[k] = ld_agent(x + c[k]);
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k) {
+            const double t = s + v[k] * (scaled ? xv[k] * scale : xv[k]);
+            s = k < width ? t : s;
+        }
+    } else {
+        for (int k = 0; k < width; ++k) {
+            const size_t o = (size_t)k * n + row;
+            const double xk = ld_agent(x + col[o]);
+            s += val[o] * (scaled ? xk * scale : xk);
+        }
+    }
+    return s;
+}
+
+template <typename MT>
+__device__ __forceinline__ float ell_row_f32_agent(int n, int width, int row,
+                                                   const int* __restrict__ col,
+                                                   const MT* __restrict__ val, const float* x,
+                                                   float scale)
+{
+    float s = 0.0f;
+    if (width >= 1 && width <= ELL_MAXW) {
+        int c[ELL_MAXW];
+        MT v[ELL_MAXW];
+        float xv[ELL_MAXW];
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k) {
+            const size_t o = ell_slot(k, width, n, row);
+            c[k] = col[o];
+            v[k] = val[o];
+        }
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k) xv[k] = ld_agent(x + c[k]);
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k) {
+            const float t = fmaf(ell_val_f32(v[k]), xv[k] * scale, s);
+            s = k < width ? t : s;
+        }
+    } else {
+        for (int k = 0; k < width; ++k) {
+            const size_t o = (size_t)k * n + row;
+            s = fmaf(ell_val_f32(val[o]), ld_agent(x + col[o]) * scale, s);
+        }
+    }
+    return s;
+}
+
+template <typename VT, typename MT>
+struct RowDotAgent;
+template <>
+struct RowDotAgent<double, double> {
+    __device__ __forceinline__ static double run(int n, int width, int row, const int* col,
+                                                 const double* val, const double* x, double inv)
+    {
+        return ell_row_f64_agent(n, width, row, col, val, x, inv, true);
+    }
+};
+template <typename MT>
+struct RowDotAgent<float, MT> {
+    __device__ __forceinline__ static float run(int n, int width, int row, const int* col,
+                                                const MT* val, const float* x, float inv)
+    {
+        return ell_row_f32_agent<MT>(n, width, row, col, val, x, inv);
+    }
+};
+
+// One term of RowDotAgent's sum: s + val * (x * inv) in FP64 (no fused
+// multiply-add), fmaf(val, x * inv, s) in FP32, as ell_row_f64_agent and
+// ell_row_f32_agent add them.
+template <typename VT, typename MT>
+struct RowTerm;
+template <>
+struct RowTerm<double, double> {
+    __device__ __forceinline__ static double add(double s, double v, double x, double inv)
+    {
+        return s + v * (x * inv);
+    }
+};
+template <typename MT>
+struct RowTerm<float, MT> {
+    __device__ __forceinline__ static float add(float s, MT v, float x, float inv)
+    {
+        return fmaf(ell_val_f32(v), x * inv, s);
+    }
+};
+
+
+// Loads of values other workgroups of the same launch may have written
+// (agent scope: they bypass this XCD's L2 copy) and plain loads, chosen at
+// compile time.
+template <bool AGENT, typename T>
+__device__ __forceinline__ T ld_x(const T* p)
+{
+    if constexpr (AGENT) return ld_agent(p);
+    else return *p;
+}
+
+// The Arnoldi SpMV of a thread's rows (rows base + r * GM_BS, r < rpt):
+// xo[r] = x[row] and w[r] = RowDot (or RowDotAgent) of the row with x
+// scaled by inv, 0 for rows at or past n.  For rows of at most ELL_MAXW
+// entries, in groups of GM_RG rows: every row's own entry and column/value
+// slots are loaded first, then every row's gathers, then the sums in
+// RowDot's order: two memory round trips per group instead of three per
+// row.
+template <typename VT, typename MT, bool AGENT>
+__device__ __forceinline__ void arnoldi_rows(int n, int rpt, int width, int base,
+                                             const int* __restrict__ col,
+                                             const MT* __restrict__ val, const VT* x, VT inv,
+                                             VT (&xo)[GM_RPT], VT (&w)[GM_RPT])
+{
+    if (width >= 1 && width <= ELL_MAXW) {
+#pragma unroll
+        for (int g = 0; g < GM_RPT; g += GM_RG) {
+            if (g >= rpt) {
+#pragma unroll
+                for (int r = g; r < GM_RPT; ++r) {
+                    xo[r] = VT(0);
+                    w[r] = VT(0);
+                }
+                break;
+            }
+            int c[GM_RG][ELL_MAXW];
+            MT v[GM_RG][ELL_MAXW];
+            VT xv[GM_RG][ELL_MAXW];
+#pragma unroll
+            for (int q = 0; q < GM_RG; ++q) {
+                if (g + q < rpt) {
+                    const int rr = min(base + (g + q) * GM_BS, n - 1);  // in bounds; unused past n
+                    xo[g + q] = ld_x<AGENT>(x + rr);
+#pragma unroll
+                    for (int k = 0; k < ELL_MAXW; ++k) {
+                        const size_t o = ell_slot(k, width, n, rr);
+                        c[q][k] = col[o];
+                        v[q][k] = val[o];
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < GM_RG; ++q) {
+                if (g + q < rpt) {
+#pragma unroll
+                    for (int k = 0; k < ELL_MAXW; ++k) xv[q][k] = ld_x<AGENT>(x + c[q][k]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < GM_RG; ++q) {
+                const int r = g + q;
+                VT sum = VT(0);
+                if (r < rpt && base + r * GM_BS < n) {
+#pragma unroll
+                    for (int k = 0; k < ELL_MAXW; ++k) {
+                        const VT t = RowTerm<VT, MT>::add(sum, v[q][k], xv[q][k], inv);
+                        sum = k < width ? t : sum;
+                    }
+                } else {
+                    xo[r] = VT(0);
+                }
+                w[r] = sum;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < GM_RPT; ++r) {
+            const int rr = base + r * GM_BS;
+            xo[r] = VT(0);
+            w[r] = VT(0);
+            if (r < rpt && rr < n) {
+                xo[r] = ld_x<AGENT>(x + rr);
+                if constexpr (AGENT) w[r] = RowDotAgent<VT, MT>::run(n, width, rr, col, val, x, inv);
+                else w[r] = RowDot<VT, MT>::run(n, width, rr, col, val, x, inv);
+            }
+        }
+    }
+}
 
 constexpr int GM_LB = 8;  // basis vectors loaded per batch
 
@@ -379,17 +586,16 @@ k_step_spmv(int n, int rpt, int width, const int* __restrict__ col, const MT* __
     if (!sh_go) return;
     const VT inv = (VT)(1.0 / sh_h);
     VT* Vj = V + (size_t)j * ldv;
-    VT w[GM_RPT], vj[GM_RPT];
+    VT w[GM_RPT], vj[GM_RPT], xo[GM_RPT];
     const int base = lblock() * GM_BS * rpt + threadIdx.x;
+    arnoldi_rows<VT, MT, false>(n, rpt, width, base, col, val, Wsrc, inv, xo, w);
 #pragma unroll
     for (int r = 0; r < GM_RPT; ++r) {
         const int row = base + r * GM_BS;
-        w[r] = VT(0);
         vj[r] = VT(0);
         if (r < rpt && row < n) {
-            vj[r] = Wsrc[row] * inv;
+            vj[r] = xo[r] * inv;
             Vj[row] = vj[r];
-            w[r] = RowDot<VT, MT>::run(n, width, row, col, val, Wsrc, inv);
             Wdst[row] = w[r];
         }
     }
@@ -614,99 +820,6 @@ constexpr long XCH_SPIN = 1L << 22;                  // polls before a wait give
 // (PHG_XCH_SPIN overrides it: the failure-path test forces a give-up with
 // a limit of a few polls; every workgroup still exits normally)
 
-template <typename T>
-__device__ __forceinline__ T ld_agent(const T* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ void st_agent(T* p, T v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// ell_row_f64 / ell_row_f32 with x gathered by agent-scope loads (same order
-// of operations)
-__device__ __forceinline__ double ell_row_f64_agent(int n, int width, int row,
-                                                    const int* __restrict__ col,
-                                                    const double* __restrict__ val,
-                                                    const double* x, double scale, bool scaled)
-{
-    double s = 0.0;
-    if (width <= ELL_MAXW) {
-        int c[ELL_MAXW];
-        double v[ELL_MAXW], xv[ELL_MAXW];
-#pragma unroll
-        for (int k = 0; k < ELL_MAXW; ++k) {
-            const size_t o = (size_t)k * n + row;
-            c[k] = k < width ? col[o] : 0;
-            v[k] = k < width ? val[o] : 0.0;
-        }
-#pragma unroll
-        for (int k = 0; k < ELL_MAXW; ++k) xv[k] = k < width ? ld_agent(x + c[k]) : 0.0;
-#pragma unroll
-        for (int k = 0; k < ELL_MAXW; ++k)
-            if (k < width) s += v[k] * (scaled ? xv[k] * scale : xv[k]);
-    } else {
-        for (int k = 0; k < width; ++k) {
-            const size_t o = (size_t)k * n + row;
-            const double xk = ld_agent(x + col[o]);
-            s += val[o] * (scaled ? xk * scale : xk);
-        }
-    }
-    return s;
-}
-
-template <typename MT>
-__device__ __forceinline__ float ell_row_f32_agent(int n, int width, int row,
-                                                   const int* __restrict__ col,
-                                                   const MT* __restrict__ val, const float* x,
-                                                   float scale)
-{
-    float s = 0.0f;
-    if (width <= ELL_MAXW) {
-        int c[ELL_MAXW];
-        MT v[ELL_MAXW];
-        float xv[ELL_MAXW];
-#pragma unroll
-        for (int k = 0; k < ELL_MAXW; ++k) {
-            const size_t o = (size_t)k * n + row;
-            c[k] = k < width ? col[o] : 0;
-            v[k] = k < width ? val[o] : MT(0);
-        }
-#pragma unroll
-        for (int k = 0; k < ELL_MAXW; ++k) xv[k] = k < width ? ld_agent(x + c[k]) : 0.0f;
-#pragma unroll
-        for (int k = 0; k < ELL_MAXW; ++k)
-            if (k < width) s = fmaf(ell_val_f32(v[k]), xv[k] * scale, s);
-    } else {
-        for (int k = 0; k < width; ++k) {
-            const size_t o = (size_t)k * n + row;
-            s = fmaf(ell_val_f32(val[o]), ld_agent(x + col[o]) * scale, s);
-        }
-    }
-    return s;
-}
-
-template <typename VT, typename MT>
-struct RowDotAgent;
-template <>
-struct RowDotAgent<double, double> {
-    __device__ __forceinline__ static double run(int n, int width, int row, const int* col,
-                                                 const double* val, const double* x, double inv)
-    {
-        return ell_row_f64_agent(n, width, row, col, val, x, inv, true);
-    }
-};
-template <typename MT>
-struct RowDotAgent<float, MT> {
-    __device__ __forceinline__ static float run(int n, int width, int row, const int* col,
-                                                const MT* val, const float* x, float inv)
-    {
-        return ell_row_f32_agent<MT>(n, width, row, col, val, x, inv);
-    }
-};
-
 // Wave 0: once every slot of the row is written, the sum of its G partials
 // in sum_partials' order (the same value in every lane).  false when the
 // wait gives up or another workgroup gave up.
@@ -856,16 +969,15 @@ k_mgs_cycle(int n, int rpt, int width, const int* __restrict__ col, const MT* __
         VT* Wdst = (j & 1) ? W0 : W1;
         const VT inv = (VT)(1.0 / h);
         VT* Vj = V + (size_t)j * ldv;
-        VT w[GM_RPT], vj[GM_RPT];
+        VT w[GM_RPT], vj[GM_RPT], xo[GM_RPT];
+        arnoldi_rows<VT, MT, true>(n, rpt, width, base, col, val, Wsrc, inv, xo, w);
 #pragma unroll
         for (int r = 0; r < GM_RPT; ++r) {
             const int rr = base + r * GM_BS;
-            w[r] = VT(0);
             vj[r] = VT(0);
             if (r < rpt && rr < n) {
-                vj[r] = ld_agent(Wsrc + rr) * inv;
+                vj[r] = xo[r] * inv;
                 Vj[rr] = vj[r];
-                w[r] = RowDotAgent<VT, MT>::run(n, width, rr, col, val, Wsrc, inv);
             }
         }
         // pv = v_{i-1}, iv = v_i of the coming pass i (own rows; v_j from
@@ -1050,19 +1162,26 @@ bool coop_fits(int G)
 }
 
 // ---------------------------------------------------------------- driver --
-int prepare_work(phg_matrix_s* A, int prec, int m, GmresWork** out)
+int prepare_work(phg_matrix_s* A, int prec, int m, int ortho, GmresWork** out)
 {
-    auto& slot = A->work[prec];
+    const bool mgs = ortho == PHG_MGS;
+    auto& slot = A->work[2 * prec + (mgs ? 0 : 1)];
     const int n = A->n_rows;
     if (!slot || slot->m < m) {
         std::unique_ptr<GmresWork> w(new GmresWork());
         w->n = n;
         w->m = m;
         w->prec = prec;
-        // target block count: 256 (one per CU: the shortest reduction chain
-        // of the MGS passes at levels 6-8), or PHG_GMRES_BLOCKS (launch-shape
-        // experiments; it changes the grouping of the dot partials)
-        int target = 256;
+        // target block count.  MGS: the fewest blocks that hold the rows at
+        // GM_RPT rows per thread (160 at level 7): each of an Arnoldi step's
+        // j + 2 dependent sums then gathers fewer partials, and the
+        // persistent cycle ran 35.3 instead of 33.3 solves/s at level 7
+        // (the kernel-per-pass form, which shares the shape so that both
+        // give the same bits, 32.2 instead of 33.6).  CGS2: 256, one per
+        // CU, since its passes stream the basis.  PHG_GMRES_BLOCKS overrides
+        // both (launch-shape experiments; it changes the grouping of the
+        // dot partials).
+        int target = mgs ? std::max(1, (n + GM_BS * GM_RPT - 1) / (GM_BS * GM_RPT)) : 256;
         if (const char* e = getenv("PHG_GMRES_BLOCKS")) target = std::max(1, std::min(atoi(e), GM_MAXG));
         int G = std::max(1, std::min((n + GM_BS - 1) / GM_BS, target));
         int rpt = (n + G * GM_BS - 1) / (G * GM_BS);
@@ -1287,7 +1406,7 @@ int gmres_solve(phg_matrix_s* A, const double* b, double* x, const phg_gmres_opt
     int rc = ensure_ell64(A, s);
     if (rc) return rc;
     GmresWork* w = nullptr;
-    rc = prepare_work(A, o.precision, o.restart, &w);
+    rc = prepare_work(A, o.precision, o.restart, o.ortho, &w);
     if (rc) return rc;
     if (o.precision == PHG_F64) return gmres_run<double, double>(A, A->ell64, w, b, x, o, info, s);
     if (o.precision == PHG_F32) {

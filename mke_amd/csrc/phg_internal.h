@@ -125,7 +125,8 @@ struct phg_matrix_s {
     phg::Ell<uint16_t> ell16;      // bfloat16 bit patterns
     phg::Ell<double> ell64d;       // columns scaled by dinv (diagonal right preconditioning)
     phg::DevBuf<double> dinv;      // 1 / diagonal
-    std::unique_ptr<phg::GmresWork, phg::GmresWorkDeleter> work[3];  // per precision
+    // per precision (PHG_F64, PHG_F32, PHG_BF16) and orthogonalisation (MGS, CGS2)
+    std::unique_ptr<phg::GmresWork, phg::GmresWorkDeleter> work[6];
     phg::BlockWork* bwork = nullptr;  // block GMRES workspace
     phg::BatchWork* batch[2] = {nullptr, nullptr};  // batched GMRES: FP64, mixed
     ~phg_matrix_s();
