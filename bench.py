@@ -19,6 +19,7 @@ import argparse
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -240,6 +241,21 @@ def serve_leg(n_requests, rate, seed=20251202, verbose=False, load_fracs=(0.5, 0
         out[name]["offered_loads"] = loads
         srv.close()
     return out
+
+
+def serve_leg_child(n_requests, rate):
+    """serve_leg in a child process of its own, as a server runs.  Each
+    engine has one HIP stream, and HIP maps a process's streams onto a few
+    hardware queues (GPU_MAX_HW_QUEUES, 4 by default): after this process's
+    other legs, whose solver workspaces keep streams of their own, two
+    engines can land on one queue and run one after the other (measured:
+    24.6 instead of 29.5 steps/s for the FP64 form, 29.5 with 8 queues)."""
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--serve-only",
+           "--serve-requests", str(n_requests), "--serve-rate", str(rate)]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, cwd=ROOT)
+    if r.returncode != 0:
+        raise RuntimeError("serving leg failed (exit %d)" % r.returncode)
+    return json.loads(r.stdout.strip().splitlines()[-1])["serve_only"]
 
 
 def l8_leg(steps, oracle_check, seed=20251203):
@@ -633,6 +649,13 @@ def main():
     A = P.Matrix.assemble(mesh, P.HELMHOLTZ, cm, cl, stream=stream)
     torch.cuda.synchronize()
     t_asm = time.perf_counter() - t
+    # the same operator again: the first call also pays the process's first
+    # use of the assembly kernels and the mesh's device upload
+    t = time.perf_counter()
+    A_again = P.Matrix.assemble(mesh, P.HELMHOLTZ, cm, cl, stream=stream)
+    torch.cuda.synchronize()
+    t_asm2 = time.perf_counter() - t
+    del A_again
     Mm = P.Matrix.assemble(mesh, P.MASS, stream=stream)
     n, nnz = A.n_rows, A.nnz
     arrays = mesh.arrays()
@@ -641,8 +664,8 @@ def main():
     Mm.mult_vector(b, torch.tensor(f, dtype=torch.float64, device=dev), stream=stream)
     x = torch.zeros_like(b)
     torch.cuda.synchronize()
-    log("assembly (Helmholtz, GPU): %.1f ms incl. host structure; n=%d nnz=%d" %
-        (t_asm * 1e3, n, nnz))
+    log("assembly (Helmholtz, GPU): %.1f ms incl. host structure (first call), %.1f ms again; "
+        "n=%d nnz=%d" % (t_asm * 1e3, t_asm2 * 1e3, n, nnz))
 
     # ---- timed region: plain FP64 GMRES(30), MGS (the reference form), each
     #      restart cycle one cooperative kernel (HIP events around each) ----
@@ -811,7 +834,7 @@ def main():
 
     # ---- serving: ensemble BarVortex steps on mixed levels ----
     if not args.no_variants and args.serve_requests > 0:
-        variants.update(serve_leg(args.serve_requests, args.serve_rate))
+        variants.update(serve_leg_child(args.serve_requests, args.serve_rate))
 
     cold_ms = cold_gbs = None
     if args.cold_spmv:
@@ -891,6 +914,7 @@ def main():
         "converged_all": ok,
         "fwd_err_vs_oracle": fwd.get("fp64_mgs"),
         "assembly_ms": round(t_asm * 1e3, 3),
+        "assembly_again_ms": round(t_asm2 * 1e3, 3),
         "variants": variants,
         "roofline": None,
         "cpu_baseline": cpu,

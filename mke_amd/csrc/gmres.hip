@@ -286,11 +286,11 @@ __device__ __forceinline__ T ld_x(const T* p)
 // The Arnoldi SpMV of a thread's rows (rows base + r * GM_BS, r < rpt):
 // xo[r] = x[row] and w[r] = RowDot (or RowDotAgent) of the row with x
 // scaled by inv, 0 for rows at or past n.  For rows of at most ELL_MAXW
-// entries, in groups of GM_RG rows: every row's own entry and column/value
+// entries, in groups of RG rows: every row's own entry and column/value
 // slots are loaded first, then every row's gathers, then the sums in
 // RowDot's order: two memory round trips per group instead of three per
 // row.
-template <typename VT, typename MT, bool AGENT>
+template <typename VT, typename MT, bool AGENT, int RG = GM_RG>
 __device__ __forceinline__ void arnoldi_rows(int n, int rpt, int width, int base,
                                              const int* __restrict__ col,
                                              const MT* __restrict__ val, const VT* x, VT inv,
@@ -298,7 +298,7 @@ __device__ __forceinline__ void arnoldi_rows(int n, int rpt, int width, int base
 {
     if (width >= 1 && width <= ELL_MAXW) {
 #pragma unroll
-        for (int g = 0; g < GM_RPT; g += GM_RG) {
+        for (int g = 0; g < GM_RPT; g += RG) {
             if (g >= rpt) {
 #pragma unroll
                 for (int r = g; r < GM_RPT; ++r) {
@@ -307,11 +307,11 @@ __device__ __forceinline__ void arnoldi_rows(int n, int rpt, int width, int base
                 }
                 break;
             }
-            int c[GM_RG][ELL_MAXW];
-            MT v[GM_RG][ELL_MAXW];
-            VT xv[GM_RG][ELL_MAXW];
+            int c[RG][ELL_MAXW];
+            MT v[RG][ELL_MAXW];
+            VT xv[RG][ELL_MAXW];
 #pragma unroll
-            for (int q = 0; q < GM_RG; ++q) {
+            for (int q = 0; q < RG; ++q) {
                 if (g + q < rpt) {
                     const int rr = min(base + (g + q) * GM_BS, n - 1);  // in bounds; unused past n
                     xo[g + q] = ld_x<AGENT>(x + rr);
@@ -324,14 +324,14 @@ __device__ __forceinline__ void arnoldi_rows(int n, int rpt, int width, int base
                 }
             }
 #pragma unroll
-            for (int q = 0; q < GM_RG; ++q) {
+            for (int q = 0; q < RG; ++q) {
                 if (g + q < rpt) {
 #pragma unroll
                     for (int k = 0; k < ELL_MAXW; ++k) xv[q][k] = ld_x<AGENT>(x + c[q][k]);
                 }
             }
 #pragma unroll
-            for (int q = 0; q < GM_RG; ++q) {
+            for (int q = 0; q < RG; ++q) {
                 const int r = g + q;
                 VT sum = VT(0);
                 if (r < rpt && base + r * GM_BS < n) {
@@ -559,12 +559,18 @@ __global__ void k_cycle_start(GmresState* st, const double* __restrict__ part, i
 // Arnoldi step j.  For j >= 1 first the Givens rotation of step j-1 (and
 // the decision to go on); then v_j = W_src / h (stored), W_dst = A v_j, and
 // the partial of <W_dst, v_0> (MGS, ndots = 1) or of <W_dst, v_i> for all
-// i <= j (CGS2 first pass, ndots = j + 1).
+// i <= j (CGS2 first pass, ndots = j + 1).  The norm partials of step j - 1
+// are read from nrm slot j: for MGS that is part itself (this kernel writes
+// only slot 0 there), for CGS2 a region of its own, since this kernel writes
+// part slots 0..j while slower blocks may still be reading the norm (with
+// more blocks than fit on the device at once, a late block would otherwise
+// read early blocks' new dots in place of the norm).
 template <typename VT, typename MT>
 __global__ void __launch_bounds__(GM_BS)
 k_step_spmv(int n, int rpt, int width, const int* __restrict__ col, const MT* __restrict__ val,
             const VT* __restrict__ Wsrc, VT* __restrict__ V, size_t ldv, VT* __restrict__ Wdst,
-            int j, int ndots, double* __restrict__ part, int G, GmresState* __restrict__ st)
+            int j, int ndots, double* __restrict__ part, const double* nrm, int G,
+            GmresState* __restrict__ st)
 {
     __shared__ double sh[GM_BS / 64];
     __shared__ double sh_h;
@@ -576,7 +582,7 @@ k_step_spmv(int n, int rpt, int width, const int* __restrict__ col, const MT* __
             sh_go = 1;
         }
     } else if (threadIdx.x < 64) {
-        const GivensOut g = givens_step(st, j - 1, part, G, lblock() == 0);
+        const GivensOut g = givens_step(st, j - 1, nrm, G, lblock() == 0);
         if (threadIdx.x == 0) {
             sh_h = g.hn;
             sh_go = g.cont;
@@ -588,7 +594,9 @@ k_step_spmv(int n, int rpt, int width, const int* __restrict__ col, const MT* __
     VT* Vj = V + (size_t)j * ldv;
     VT w[GM_RPT], vj[GM_RPT], xo[GM_RPT];
     const int base = lblock() * GM_BS * rpt + threadIdx.x;
-    arnoldi_rows<VT, MT, false>(n, rpt, width, base, col, val, Wsrc, inv, xo, w);
+    // two rows per load group: 133 registers, so three workgroups fit a CU
+    // and level 8's 641 workgroups are resident at once (four rows: 174)
+    arnoldi_rows<VT, MT, false, 2>(n, rpt, width, base, col, val, Wsrc, inv, xo, w);
 #pragma unroll
     for (int r = 0; r < GM_RPT; ++r) {
         const int row = base + r * GM_BS;
@@ -739,12 +747,12 @@ k_cgs_update(int n, int rpt, VT* __restrict__ w, const VT* __restrict__ V, size_
 // End of a cycle: the pending Givens rotation (if the cycle ran to its last
 // step) and y = R^{-1} g, one block.
 __global__ void __launch_bounds__(GM_BS)
-k_solve_y(const double* __restrict__ part, int G, GmresState* __restrict__ st)
+k_solve_y(const double* __restrict__ nrm, int G, GmresState* __restrict__ st)
 {
     __shared__ double Rs[GM_MAXM * GM_MAXM];
     __shared__ double gs[GM_MAXM];
     if (st->converged) return;
-    if (st->active && threadIdx.x < 64) givens_step(st, st->k, part, G, true);
+    if (st->active && threadIdx.x < 64) givens_step(st, st->k, nrm, G, true);
     __syncthreads();
     const int k = st->k, m = st->m;
     // stage R and g in LDS with all threads (parallel loads), then solve
@@ -1199,7 +1207,7 @@ int prepare_work(phg_matrix_s* A, int prec, int m, int ortho, GmresWork** out)
         const size_t es = prec == PHG_F64 ? 8 : 4;
         PHG_HIP(w->V.alloc((size_t)(m + 1) * w->ldv * es));
         PHG_HIP(w->W.alloc(2 * w->ldv * es));
-        PHG_HIP(w->part.alloc((size_t)2 * (m + 2) * G));  // two regions (CGS2 passes)
+        PHG_HIP(w->part.alloc((size_t)3 * (m + 2) * G));  // regions A, B, C (CGS2 passes)
         PHG_HIP(w->st.alloc(1));
         PHG_HIP(hipHostMalloc((void**)&w->host, sizeof(GmresState), hipHostMallocDefault));
         slot.reset(w.release());
@@ -1218,6 +1226,8 @@ void enqueue_cycle(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double
     VT* W0 = (VT*)w->W.p;
     VT* W1 = W0 + ldv;
     double* part = w->part.p;
+    // ||w||^2 partials: MGS leaves them in part (slot j + 1), CGS2 in region C
+    double* nrm = cgs ? part + (size_t)2 * (w->m + 2) * G : part;
     GmresState* st = w->st.p;
     const dim3 grid(G), block(GM_BS);
     for (int j = 0; j < m; ++j) {
@@ -1225,22 +1235,22 @@ void enqueue_cycle(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double
         VT* dst = (j & 1) ? W0 : W1;
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_step_spmv<VT, MT>), grid, block, 0, s, n, rpt,
                            E.width, E.col.p, E.val.p, src, V, ldv, dst, j, cgs ? j + 1 : 1,
-                           part, G, st);
+                           part, nrm, G, st);
         if (!cgs) {
             for (int i = 1; i <= j + 1; ++i)
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mgs<VT>), grid, block, 0, s, n, rpt, dst, V,
                                    ldv, i, j, part, G, st);
         } else {
             // first pass: dots from the step kernel (region A) -> second-pass
-            // dots (region B); second pass: region B -> ||w||^2 (region A)
+            // dots (region B); second pass: region B -> ||w||^2 (region C)
             double* partB = part + (size_t)(w->m + 2) * G;
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cgs_update<VT>), grid, block, 0, s, n, rpt, dst,
                                V, ldv, j, 0, part, partB, G, st);
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cgs_update<VT>), grid, block, 0, s, n, rpt, dst,
-                               V, ldv, j, 1, partB, part, G, st);
+                               V, ldv, j, 1, partB, nrm, G, st);
         }
     }
-    hipLaunchKernelGGL(k_solve_y, dim3(1), dim3(GM_BS), 0, s, part, G, st);
+    hipLaunchKernelGGL(k_solve_y, dim3(1), dim3(GM_BS), 0, s, nrm, G, st);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_xupdate<VT>), grid, block, 0, s, n, rpt, V, ldv, x, st);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_residual<VT>), grid, block, 0, s, n, rpt,
                        A->ell64.width, A->ell64.col.p, A->ell64.val.p, b, x, W0, part, st);
@@ -1298,11 +1308,19 @@ int gmres_run(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double* b, 
                 PHG_HIP(hipGraphExecDestroy(w->exec));
                 w->exec = nullptr;
             }
-            if (!w->cap) PHG_HIP(hipStreamCreateWithFlags(&w->cap, hipStreamNonBlocking));
+            // recorded on the caller's stream; the null stream cannot be
+            // captured, so for it on a stream of the workspace (a stream
+            // per workspace would also take a share of the process's few
+            // hardware queues from the caller's own streams)
+            hipStream_t cs = s;
+            if (!cs) {
+                if (!w->cap) PHG_HIP(hipStreamCreateWithFlags(&w->cap, hipStreamNonBlocking));
+                cs = w->cap;
+            }
             hipGraph_t g = nullptr;
-            PHG_HIP(hipStreamBeginCapture(w->cap, hipStreamCaptureModeThreadLocal));
-            enqueue_cycle<VT, MT>(A, E, w, b, x, m, cgs, w->cap);
-            PHG_HIP(hipStreamEndCapture(w->cap, &g));
+            PHG_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            enqueue_cycle<VT, MT>(A, E, w, b, x, m, cgs, cs);
+            PHG_HIP(hipStreamEndCapture(cs, &g));
             hipError_t e = hipGraphInstantiate(&w->exec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             PHG_HIP(e);

@@ -912,18 +912,24 @@ int batch_run_t(phg_matrix_s* A, const Ell<MT>& E, BatchWork* w, int K, const do
                           !memcmp(key, w->key, sizeof key);
         if (!same) {
             w->drop_graphs();
-            if (!w->cap) PHG_HIP(hipStreamCreateWithFlags(&w->cap, hipStreamNonBlocking));
+            // recorded on the caller's stream, or for the null stream (which
+            // cannot be captured) on a stream of the workspace
+            hipStream_t cs = s;
+            if (!cs) {
+                if (!w->cap) PHG_HIP(hipStreamCreateWithFlags(&w->cap, hipStreamNonBlocking));
+                cs = w->cap;
+            }
             for (int c = 0; c <= nch; ++c) {
                 hipGraph_t g = nullptr;
-                PHG_HIP(hipStreamBeginCapture(w->cap, hipStreamCaptureModeThreadLocal));
+                PHG_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
                 if (c < nch)
                     enqueue_batch_steps<VT, MT, RPT>(A, E, w, K, chunk_lo(c), chunk_hi(c), m,
                                                      passes, o.rtol, inner, o.max_iters, pc,
-                                                     w->cap);
+                                                     cs);
                 else
                     enqueue_batch_finish<VT, MT, RPT>(A, E, w, K, B, X, o.rtol, o.max_iters, pc,
-                                                      w->cap);
-                PHG_HIP(hipStreamEndCapture(w->cap, &g));
+                                                      cs);
+                PHG_HIP(hipStreamEndCapture(cs, &g));
                 hipError_t e = hipGraphInstantiate(&w->exec[c], g, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(g);
                 PHG_HIP(e);

@@ -14,7 +14,8 @@ torch = pytest.importorskip("torch")
 
 import mke_amd as P
 from oracle import oracle as O
-from tests._common import oracle_matrix, oracle_mesh, real_sph_harm_field
+from tests._common import (oracle_matrix, oracle_mesh, real_sph_harm_field,
+                           real_sph_harm_field_lean)
 
 pytestmark = pytest.mark.gpu
 
@@ -357,3 +358,32 @@ def test_gmres_persistent_cycle_gives_up_cleanly(monkeypatch):
     x = torch.zeros_like(b)
     info = A.solve(x, b, rtol=1e-10, ortho=P.MGS)
     assert info["iterations"] == ref["iterations"] and torch.equal(x, x0)
+
+
+@pytest.mark.parametrize("ortho", [P.CGS2, P.MGS])
+def test_gmres_level8_same_bits_on_fresh_workspaces(ortho):
+    """Level 8 runs the single right-hand-side kernels on 641 workgroups, more
+    than the device holds at once, so workgroups of one launch run one after
+    another.  Each solve on a freshly assembled matrix (a fresh workspace)
+    gives the same bits and iteration count: no block may read a partial
+    another block of the same launch overwrites (the CGS2 step kernel once
+    read the previous step's norm partials from the slot its first-pass dots
+    go to, and its iteration count varied from run to run)."""
+    print("gmres level 8 fresh workspaces ortho=%d" % ortho, flush=True)
+    pm = P.Mesh.sphere(8)
+    Mm = P.Matrix.assemble(pm, P.MASS)
+    a = pm.arrays()
+    f = real_sph_harm_field_lean(a["pr"], 16, 7, device="cuda")[a["inner"]]
+    b = torch.zeros(Mm.n_rows, dtype=torch.float64, device=DEV)
+    Mm.mult_vector(b, _dev(f))
+    xs, infos = [], []
+    for _ in range(3):
+        A = P.Matrix.assemble(pm, P.HELMHOLTZ, CM, CL)
+        x = torch.zeros_like(b)
+        infos.append(A.solve(x, b, restart=30, rtol=1e-10, ortho=ortho))
+        xs.append(x)
+        A.close()
+    assert infos[0]["converged"] and infos[0]["rel_residual"] <= 1e-10
+    for i in (1, 2):
+        assert infos[i]["iterations"] == infos[0]["iterations"], (infos[i], infos[0])
+        assert torch.equal(xs[i], xs[0])
