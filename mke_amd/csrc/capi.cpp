@@ -288,6 +288,10 @@ int phg_matrix_from_csr(int n_rows, int n_cols, const int* row_ptr, const int* c
         set_error("phg_matrix_from_csr: row_ptr[0] must be 0");
         return PHG_ERR_ARG;
     }
+    if (row_ptr[n_rows] > 0 && (!col_idx || !vals)) {
+        set_error("phg_matrix_from_csr: entries given without col_idx or vals");
+        return PHG_ERR_ARG;
+    }
     for (int i = 0; i < n_rows; ++i) {
         if (row_ptr[i + 1] < row_ptr[i]) {
             set_error("phg_matrix_from_csr: row_ptr not monotone");

@@ -111,6 +111,15 @@ def test_argument_errors_without_a_gpu():
     lh = C.c_void_p(1)  # never dereferenced: the slot count is refused first
     assert L.phg_barvortex_create(m.h, C.byref(conf), lh, 3, None, C.byref(h)) == ERR_ARG
     assert "slots" in L.phg_last_error().decode()
+    # a CSR with entries but no column or value array is refused before it is read
+    import numpy as np
+    rp = np.array([0, 1, 2], np.int32)
+    ci = np.array([0, 1], np.int32)
+    pi = rp.ctypes.data_as(C.POINTER(C.c_int))
+    assert L.phg_matrix_from_csr(2, 2, pi, None, None, C.byref(h)) == ERR_ARG
+    assert L.phg_matrix_from_csr(2, 2, pi, ci.ctypes.data_as(C.POINTER(C.c_int)), None,
+                                 C.byref(h)) == ERR_ARG
+    assert "col_idx or vals" in L.phg_last_error().decode()
 
 
 def test_dot_noalloc_refuses_bad_sizes_before_device_work():
