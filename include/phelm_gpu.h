@@ -434,7 +434,8 @@ int phg_barvortex_load_bnd(phg_barvortex_t B, int slot, const double *d_bnd_u,
 int phg_barvortex_iterate(phg_barvortex_t B, const phg_bv_opts *opts, int *state,
                           phg_bv_info *infos, phg_stream_t s);
 int phg_barvortex_store(phg_barvortex_t B, int slot, double *d_u1, phg_stream_t s);
-/* Linearised and adjoint steps with zero boundary values, for nmembers
+/* Linearised and adjoint steps with zero boundary values (the _bnd forms
+ * below take them), for nmembers
  * (<= slots) members at once; no step may be in flight on B.  Every solve
  * starts from zero; infos (may be NULL) report the GMRES steps by solve and
  * converged = all of the member's solves converged.
@@ -457,6 +458,26 @@ int phg_barvortex_calc_L_1(phg_barvortex_t B, int nmembers, const double *d_u,
 int phg_barvortex_calc_LT(phg_barvortex_t B, int nmembers, const double *d_v,
                           const double *d_z, double *d_v1, const phg_bv_opts *opts,
                           phg_bv_info *infos, phg_stream_t s);
+/* The same steps with the members' boundary (pole) values of psi (d_bnd_u)
+ * and of its Laplacian (d_bnd_w): outer_size values per member, one member
+ * after the other; either may be NULL for zero.  As the reference uses
+ * them (barvortex_impl.h:326-535): bnd_w gives the boundary values of
+ * L(z) and L(u), and of the A_ (Ab_) solve's result, to whose right part
+ * bnd_ bnd_w (bndb_ bnd_w for calc_L_1) is ADDED; bnd_u is the Poisson
+ * solve's boundary value; calc_LT also takes bnd_w for both L(v1) and
+ * ends with set_bnd(v1, bnd_u). */
+int phg_barvortex_calc_L_bnd(phg_barvortex_t B, int nmembers, const double *d_u,
+                             const double *d_z, const double *d_bnd_u, const double *d_bnd_w,
+                             double *d_u1, const phg_bv_opts *opts, phg_bv_info *infos,
+                             phg_stream_t s);
+int phg_barvortex_calc_L_1_bnd(phg_barvortex_t B, int nmembers, const double *d_u,
+                               const double *d_z, const double *d_bnd_u,
+                               const double *d_bnd_w, double *d_u1, const phg_bv_opts *opts,
+                               phg_bv_info *infos, phg_stream_t s);
+int phg_barvortex_calc_LT_bnd(phg_barvortex_t B, int nmembers, const double *d_v,
+                              const double *d_z, const double *d_bnd_u, const double *d_bnd_w,
+                              double *d_v1, const phg_bv_opts *opts, phg_bv_info *infos,
+                              phg_stream_t s);
 
 /* ---- Baroclin implicit step (examples/baroclin.h:60-292,
  *      examples/baroclin.cpp:69-181, 445-606) ------------------------------ */

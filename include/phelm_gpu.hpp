@@ -592,25 +592,24 @@ public:
         check(phg_barvortex_calc_bnd(h_, 1, F, rp, bnd_u, bnd_w, Ans, &opts_, &info_, nullptr),
               "BarVortex::calc");
     }
+    // bnd_u, bnd_w: device arrays of outer_size boundary values (or null)
     void calc_L(double* u1, const double* u, const double* z, const double* bnd_u,
                 const double* bnd_w, double)
     {
-        check_zero_bnd(bnd_u, bnd_w, "BarVortex::calc_L");
-        check(phg_barvortex_calc_L(h_, 1, u, z, u1, &opts_, &info_, nullptr), "BarVortex::calc_L");
+        check(phg_barvortex_calc_L_bnd(h_, 1, u, z, bnd_u, bnd_w, u1, &opts_, &info_, nullptr),
+              "BarVortex::calc_L");
     }
     // adds to u1's content, as the reference does
     void calc_L_1(double* u1, const double* u, const double* z, const double* bnd_u,
                   const double* bnd_w, double)
     {
-        check_zero_bnd(bnd_u, bnd_w, "BarVortex::calc_L_1");
-        check(phg_barvortex_calc_L_1(h_, 1, u, z, u1, &opts_, &info_, nullptr),
+        check(phg_barvortex_calc_L_1_bnd(h_, 1, u, z, bnd_u, bnd_w, u1, &opts_, &info_, nullptr),
               "BarVortex::calc_L_1");
     }
     void calc_LT(double* v1, const double* v, const double* z, const double* bnd_u,
                  const double* bnd_w, double)
     {
-        check_zero_bnd(bnd_u, bnd_w, "BarVortex::calc_LT");
-        check(phg_barvortex_calc_LT(h_, 1, v, z, v1, &opts_, &info_, nullptr),
+        check(phg_barvortex_calc_LT_bnd(h_, 1, v, z, bnd_u, bnd_w, v1, &opts_, &info_, nullptr),
               "BarVortex::calc_LT");
     }
     void S_step(double* Ans, const double* F) { calc(Ans, F, nullptr, nullptr, 0.0); }
@@ -650,7 +649,6 @@ private:
             x_[i] = coords[(size_t)i * nz * 2];
             y_[i] = coords[(size_t)i * nz * 2 + 1];
         }
-        os_ = no;
     }
     void init(const Mesh& m, const double* d_lh)
     {
@@ -659,19 +657,6 @@ private:
         check(phg_barvortex_create(m.handle(), &c, d_lh, 1, nullptr, &h_), "BarVortex");
         phg_bv_default_opts(&opts_);
         rp_.resize(x_.size());
-    }
-    void check_zero_bnd(const double* bnd_u, const double* bnd_w, const char* what) const
-    {
-        for (const double* b : {bnd_u, bnd_w}) {
-            if (!b || os_ == 0) continue;
-            std::vector<double> h((size_t)os_);
-            check_hip(hipMemcpy(h.data(), b, h.size() * sizeof(double), hipMemcpyDeviceToHost),
-                      what);
-            for (double v : h)
-                if (v != 0.0)
-                    throw Error(std::string(what) + ": non-zero boundary values are not supported",
-                                PHG_ERR_ARG);
-        }
     }
     void copy_dev(double* dst, const double* src) const
     {
@@ -684,7 +669,6 @@ private:
     phg_bv_opts opts_{};
     phg_bv_info info_{};
     std::vector<double> x_, y_;
-    int os_ = 0;
     ArrayDevice<double> rp_;
 };
 

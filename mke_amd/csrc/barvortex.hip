@@ -51,6 +51,7 @@ struct phg_barvortex_s {
     phg_matrix_s* A = nullptr;      // BarVortex::A_ (Helmholtz)
     phg_matrix_s* Ab = nullptr;     // BarVortex::Ab_ (backward; built on first calc_L_1)
     phg_matrix_s* bndA = nullptr;   // BarVortex::bnd_ (integrate_cb, boundary; first load_bnd)
+    phg_matrix_s* bndAb = nullptr;  // BarVortex::bndb_ (integrate_backward_cb, boundary)
     phg_matrix_s* bnd2 = nullptr;   // SphereLaplace::bnd2_ (laplace_bnd2_cb; first load_bnd)
     phg_matrix_s* norm = nullptr;   // SphereNorm::NORM_ (full mass, V x V)
     phg_jacobian_t J = nullptr;
@@ -78,6 +79,7 @@ struct phg_barvortex_s {
         delete A;
         delete Ab;
         delete bndA;
+        delete bndAb;
         delete bnd2;
         delete norm;
         if (J) phg_jacobian_destroy(J);
@@ -414,35 +416,45 @@ int jac_full(phg_barvortex_s* B, const double* a, const double* b, double sign, 
     return PHG_OK;
 }
 
-// out = phelm::solve(M, idt u2p(in)) with bnd = 0 (mesh.h:719-746): the
-// Helmholtz-type A_ / Ab_ solves of the linearised steps, from zero.
+// out = phelm::solve(M, rp) with rp = idt u2p(in) (+ bndM bw) and out =
+// p2u(M^-1 rp, bw) (mesh.h:719-746; barvortex_impl.h:386-397, 455-465): the
+// Helmholtz-type A_ / Ab_ solves of the linearised steps, from zero; bw
+// (outer x S boundary values of the Laplacian) null for zero.
 int solve_full(phg_barvortex_s* B, phg_matrix_s* M, const double* in, double* out,
-               const phg_bv_opts& o, hipStream_t s)
+               const phg_bv_opts& o, hipStream_t s, const phg_matrix_s* bndM = nullptr,
+               const double* bw = nullptr)
 {
     const int S = B->S, n = B->n;
     hipLaunchKernelGGL(bv_u2p, g256((size_t)n * S), dim3(BV_BS), 0, s, n, S, B->mesh->d_inner.p,
                        in, B->Y.p, B->dmask.p);
     spmm(B, B->idt, B->Y.p, B->RP.p, 0, s);
+    if (bw) spmm(B, bndM, bw, B->RP.p, 1, s);   // added, as the reference adds it
     zero_cols(B, n, B->RP.p, s);
     PHG_HIP(hipGetLastError());
     int rc = batched(B, M, B->RP.p, B->X.p, o.rtol_helmholtz, o.precision, PHG_CGS2, 0, nullptr,
                      o.max_solver_iters, B->it_helm, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(bv_p2u, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
-                       B->mesh->d_isb.p, B->mesh->d_p2io.p, B->X.p, out, B->dmask.p);
+    if (bw)
+        hipLaunchKernelGGL(bv_p2u_bnd, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
+                           B->mesh->d_isb.p, B->mesh->d_p2io.p, B->X.p, bw, out, B->dmask.p);
+    else
+        hipLaunchKernelGGL(bv_p2u, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
+                           B->mesh->d_isb.p, B->mesh->d_p2io.p, B->X.p, out, B->dmask.p);
     PHG_HIP(hipGetLastError());
     return PHG_OK;
 }
 
-// out = SphereLaplace::solve(in, bnd = 0) (src/impl/slaplace_impl.h:84-130):
-// p2u(laplace^-1 idt u2p(in)), multigrid-preconditioned, from zero.
+// out = SphereLaplace::solve(in, bu) (src/impl/slaplace_impl.h:84-130):
+// p2u(laplace^-1 (idt u2p(in) + bnd2 bu), bu), multigrid-preconditioned, from
+// zero; bu (outer x S boundary values of psi) null for zero.
 int poisson_full(phg_barvortex_s* B, const double* in, double* out, const phg_bv_opts& o,
-                 hipStream_t s)
+                 hipStream_t s, const double* bu = nullptr)
 {
     const int S = B->S, n = B->n;
     hipLaunchKernelGGL(bv_u2p, g256((size_t)n * S), dim3(BV_BS), 0, s, n, S, B->mesh->d_inner.p,
                        in, B->Y.p, B->dmask.p);
     spmm(B, B->idt, B->Y.p, B->Q.p, 0, s);
+    if (bu) spmm(B, B->bnd2, bu, B->Q.p, 1, s);
     zero_cols(B, n, B->Q.p, s);
     PHG_HIP(hipGetLastError());
     phg_matrix_t lap = nullptr;
@@ -450,9 +462,67 @@ int poisson_full(phg_barvortex_s* B, const double* in, double* out, const phg_bv
     int rc = batched(B, lap, B->Q.p, B->Z.p, o.rtol_poisson, PHG_F64, PHG_CGS2, 0, B->mg,
                      o.max_solver_iters, B->it_pois, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(bv_p2u, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
-                       B->mesh->d_isb.p, B->mesh->d_p2io.p, B->Z.p, out, B->dmask.p);
+    if (bu)
+        hipLaunchKernelGGL(bv_p2u_bnd, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
+                           B->mesh->d_isb.p, B->mesh->d_p2io.p, B->Z.p, bu, out, B->dmask.p);
+    else
+        hipLaunchKernelGGL(bv_p2u, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
+                           B->mesh->d_isb.p, B->mesh->d_p2io.p, B->Z.p, out, B->dmask.p);
     PHG_HIP(hipGetLastError());
+    return PHG_OK;
+}
+
+// The boundary matrices and outer x S arrays of boundary values, on first
+// use: bnd_ (integrate_cb) and bnd2_ (laplace_bnd2_cb = -(stiffness
+// block)), and with backward bndb_ (integrate_backward_cb).
+int ensure_bnd(phg_barvortex_s* B, bool backward, hipStream_t s)
+{
+    const phg_bv_conf& c = B->conf;
+    if (!B->BU.p) {
+        int rc = assemble_boundary_device(B->mesh, PHG_HELMHOLTZ, 1.0 / c.tau + c.sigma * c.theta,
+                                          -c.theta * c.mu, s, &B->bndA);
+        if (rc) return rc;
+        rc = assemble_boundary_device(B->mesh, PHG_HELMHOLTZ, 0.0, -1.0, s, &B->bnd2);
+        if (rc) return rc;
+        for (DevBuf<double>* b : {&B->BU, &B->BW}) {
+            PHG_HIP(b->alloc((size_t)B->os * B->S));
+            PHG_HIP(hipMemsetAsync(b->p, 0, (size_t)B->os * B->S * sizeof(double), s));
+        }
+    }
+    if (backward && !B->bndAb) {
+        int rc = assemble_boundary_device(B->mesh, PHG_HELMHOLTZ,
+                                          1.0 / c.tau - c.sigma * (1.0 - c.theta),
+                                          (1.0 - c.theta) * c.mu, s, &B->bndAb);
+        if (rc) return rc;
+    }
+    return PHG_OK;
+}
+
+// Boundary values of members 0..nm-1 for a linearised step (outer_size each;
+// a null array is zero) into the BU / BW columns of slots 0..nm-1; false
+// when neither is given (the steps then take null, as for zero values).
+int lin_bnd_in(phg_barvortex_s* B, int nm, const double* d_bnd_u, const double* d_bnd_w,
+               bool backward, bool* have, hipStream_t s)
+{
+    *have = false;
+    if (!d_bnd_u && !d_bnd_w) return PHG_OK;
+    if (B->os == 0) {
+        set_error("phg_barvortex linearised step: the mesh has no boundary points");
+        return PHG_ERR_ARG;
+    }
+    int rc = ensure_bnd(B, backward, s);
+    if (rc) return rc;
+    for (int k = 0; k < nm; ++k) {
+        const size_t ob = (size_t)k * B->os;
+        hipLaunchKernelGGL(bv_col_set, g256(B->os), dim3(BV_BS), 0, s, B->os, B->S, k,
+                           d_bnd_u ? d_bnd_u + ob : nullptr, B->BU.p);
+        hipLaunchKernelGGL(bv_col_set, g256(B->os), dim3(BV_BS), 0, s, B->os, B->S, k,
+                           d_bnd_w ? d_bnd_w + ob : nullptr, B->BW.p);
+        // a later load of the slot clears its columns (phg_barvortex_load)
+        B->has_bnd[k] = 1;
+    }
+    PHG_HIP(hipGetLastError());
+    *have = true;
     return PHG_OK;
 }
 
@@ -465,9 +535,12 @@ void lin_sum(phg_barvortex_s* B, const double* base, const double* P1, double c1
                        B->dmask.p);
 }
 
-// calc_L (which 0) or calc_L_1 (which 1) of members 0..nm-1.
+// calc_L (which 0) or calc_L_1 (which 1) of members 0..nm-1, with the
+// members' boundary values of psi and of its Laplacian (outer_size each,
+// null for zero).
 int lin_forward(phg_barvortex_s* B, int which, int nm, const double* d_u, const double* d_z,
-                double* d_u1, const phg_bv_opts& o, hipStream_t s)
+                const double* d_bnd_u, const double* d_bnd_w, double* d_u1,
+                const phg_bv_opts& o, hipStream_t s)
 {
     const phg_bv_conf& c = B->conf;
     const int S = B->S;
@@ -479,11 +552,15 @@ int lin_forward(phg_barvortex_s* B, int which, int nm, const double* d_u, const 
     }
     int rc = lin_begin(B, nm, s);
     if (rc) return rc;
+    bool lb = false;
+    if ((rc = lin_bnd_in(B, nm, d_bnd_u, d_bnd_w, which == 1, &lb, s))) return rc;
+    const double* bu = lb ? B->BU.p : nullptr;
+    const double* bw = lb ? B->BW.p : nullptr;
     cols_in(B, nm, d_u, B->U.p, s);                                   // u
     cols_in(B, nm, d_z, B->T2.p, s);                                  // z
     PHG_HIP(hipGetLastError());
-    if ((rc = calc1(B, B->T2.p, B->Wn.p, o, B->it_mass, s))) return rc;   // z_lapl = L(z)
-    if ((rc = calc1(B, B->U.p, B->W.p, o, B->it_mass, s))) return rc;    // pt1 = L(u)
+    if ((rc = calc1(B, B->T2.p, B->Wn.p, o, B->it_mass, s, bw))) return rc;   // z_lapl = L(z)
+    if ((rc = calc1(B, B->U.p, B->W.p, o, B->it_mass, s, bw))) return rc;    // pt1 = L(u)
     hipLaunchKernelGGL(bv_bcast, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S, B->lh.p,
                        B->T1.p, B->dmask.p);
     if ((rc = jac_full(B, B->U.p, B->T1.p, 1.0, B->Un.p, o, s))) return rc;    // J(u, lh)
@@ -506,28 +583,37 @@ int lin_forward(phg_barvortex_s* B, int which, int nm, const double* d_u, const 
     lin_sum(B, base, B->W.p, c1, B->T1.p, c2, B->Un.p, c.k2, B->Un1.p, c.k1, B->FC.p, c.k1, sj, 0,
             B->T2.p, s);
     PHG_HIP(hipGetLastError());
-    if ((rc = solve_full(B, which == 0 ? B->A : B->Ab, B->T2.p, B->Wn.p, o, s))) return rc;
-    if ((rc = poisson_full(B, B->Wn.p, B->Un.p, o, s))) return rc;
+    if ((rc = solve_full(B, which == 0 ? B->A : B->Ab, B->T2.p, B->Wn.p, o, s,
+                         which == 0 ? B->bndA : B->bndAb, bw)))
+        return rc;
+    if ((rc = poisson_full(B, B->Wn.p, B->Un.p, o, s, bu))) return rc;
     cols_out(B, nm, B->Un.p, d_u1, s);
     PHG_HIP(hipGetLastError());
     return PHG_OK;
 }
 
-// calc_LT of members 0..nm-1.
-int lin_adjoint(phg_barvortex_s* B, int nm, const double* d_v, const double* d_z, double* d_v1,
+// calc_LT of members 0..nm-1, with boundary values as lin_forward.
+int lin_adjoint(phg_barvortex_s* B, int nm, const double* d_v, const double* d_z,
+                const double* d_bnd_u, const double* d_bnd_w, double* d_v1,
                 const phg_bv_opts& o, hipStream_t s)
 {
     const phg_bv_conf& c = B->conf;
     const int S = B->S;
     int rc = lin_begin(B, nm, s);
     if (rc) return rc;
+    bool lb = false;
+    if ((rc = lin_bnd_in(B, nm, d_bnd_u, d_bnd_w, false, &lb, s))) return rc;
+    const double* bu = lb ? B->BU.p : nullptr;
+    const double* bw = lb ? B->BW.p : nullptr;
     cols_in(B, nm, d_v, B->U.p, s);                                   // v
     cols_in(B, nm, d_z, B->T2.p, s);                                  // z
     PHG_HIP(hipGetLastError());
-    if ((rc = calc1(B, B->T2.p, B->Wn.p, o, B->it_mass, s))) return rc;   // lz = L(z)
-    if ((rc = poisson_full(B, B->U.p, B->W.p, o, s))) return rc;         // v1 = L^-1 v
-    if ((rc = solve_full(B, B->A, B->W.p, B->W.p, o, s))) return rc;     // v1 = A^-1 idt v1
-    if ((rc = calc1(B, B->W.p, B->Un.p, o, B->it_mass, s))) return rc;   // L(v1)
+    if ((rc = calc1(B, B->T2.p, B->Wn.p, o, B->it_mass, s, bw))) return rc;   // lz = L(z)
+    if ((rc = poisson_full(B, B->U.p, B->W.p, o, s, bu))) return rc;         // v1 = L^-1 v
+    if ((rc = solve_full(B, B->A, B->W.p, B->W.p, o, s, B->bndA, bw))) return rc;  // A^-1
+    // L(v1) with bnd_w: the reference's pt1 and its first calc1 of pt2 are
+    // the same product (barvortex_impl.h:501-506)
+    if ((rc = calc1(B, B->W.p, B->Un.p, o, B->it_mass, s, bw))) return rc;
     if ((rc = calc1(B, B->Un.p, B->Un1.p, o, B->it_mass, s))) return rc; // L(L(v1))
     hipLaunchKernelGGL(bv_bcast, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S, B->lh.p,
                        B->T1.p, B->dmask.p);
@@ -538,6 +624,9 @@ int lin_adjoint(phg_barvortex_s* B, int nm, const double* d_v, const double* d_z
     lin_sum(B, nullptr, B->Un.p, 1.0 / c.tau - (1.0 - c.theta) * c.sigma, B->Un1.p,
             (1.0 - c.theta) * c.mu, B->U.p, c.k1, B->T1.p, c.k1, B->FC.p, c.k2, -1.0, 1, B->T2.p,
             s);
+    if (bu)   // set_bnd(v1, bnd_u): lin_sum left zeros there
+        hipLaunchKernelGGL(bv_set_bnd_cols, g256((size_t)B->V * S), dim3(BV_BS), 0, s, B->V, S,
+                           B->mesh->d_isb.p, B->mesh->d_p2io.p, bu, B->T2.p, B->dmask.p);
     cols_out(B, nm, B->T2.p, d_v1, s);
     PHG_HIP(hipGetLastError());
     return PHG_OK;
@@ -724,19 +813,8 @@ int phg_barvortex_load_bnd(phg_barvortex_t B, int slot, const double* d_bnd_u,
         return PHG_ERR_ARG;
     }
     hipStream_t s = (hipStream_t)stream;
-    if (!B->BU.p) {
-        const phg_bv_conf& c = B->conf;
-        // bnd_ with integrate_cb's coefficients, bnd2_ = laplace_bnd2_cb = -(stiffness block)
-        int rc = assemble_boundary_device(B->mesh, PHG_HELMHOLTZ, 1.0 / c.tau + c.sigma * c.theta,
-                                          -c.theta * c.mu, s, &B->bndA);
-        if (rc) return rc;
-        rc = assemble_boundary_device(B->mesh, PHG_HELMHOLTZ, 0.0, -1.0, s, &B->bnd2);
-        if (rc) return rc;
-        for (DevBuf<double>* b : {&B->BU, &B->BW}) {
-            PHG_HIP(b->alloc((size_t)B->os * B->S));
-            PHG_HIP(hipMemsetAsync(b->p, 0, (size_t)B->os * B->S * sizeof(double), s));
-        }
-    }
+    int rc = ensure_bnd(B, false, s);
+    if (rc) return rc;
     hipLaunchKernelGGL(bv_col_set, g256(B->os), dim3(BV_BS), 0, s, B->os, B->S, slot, d_bnd_u,
                        B->BU.p);
     hipLaunchKernelGGL(bv_col_set, g256(B->os), dim3(BV_BS), 0, s, B->os, B->S, slot, d_bnd_w,
@@ -891,52 +969,76 @@ int phg_barvortex_calc_bnd(phg_barvortex_t B, int nmembers, const double* d_u,
     return all ? PHG_OK : PHG_ERR_NOCONV;
 }
 
-int phg_barvortex_calc_L(phg_barvortex_t B, int nmembers, const double* d_u, const double* d_z,
-                         double* d_u1, const phg_bv_opts* opts, phg_bv_info* infos,
-                         phg_stream_t stream)
+// The linearised steps with boundary values (null arrays: zero); the plain
+// entry points are these with none.
+static int lin_entry(const char* name, int which, phg_barvortex_t B, int nmembers,
+                     const double* d_u, const double* d_z, const double* d_bnd_u,
+                     const double* d_bnd_w, double* d_u1, const phg_bv_opts* opts,
+                     phg_bv_info* infos, phg_stream_t stream)
 {
     if (!B || !d_u || !d_z || !d_u1 || nmembers < 1 || nmembers > B->S) {
-        set_error("phg_barvortex_calc_L: bad argument (1 <= nmembers <= slots)");
+        set_error(std::string(name) + ": bad argument (1 <= nmembers <= slots)");
         return PHG_ERR_ARG;
     }
     phg_bv_opts o;
     if (opts) o = *opts;
     else phg_bv_default_opts(&o);
     hipStream_t s = (hipStream_t)stream;
-    int rc = lin_forward(B, 0, nmembers, d_u, d_z, d_u1, o, s);
+    const int rc = which == 2 ? lin_adjoint(B, nmembers, d_u, d_z, d_bnd_u, d_bnd_w, d_u1, o, s)
+                              : lin_forward(B, which, nmembers, d_u, d_z, d_bnd_u, d_bnd_w, d_u1,
+                                            o, s);
     return rc ? rc : lin_finish(B, nmembers, infos, s);
+}
+
+int phg_barvortex_calc_L(phg_barvortex_t B, int nmembers, const double* d_u, const double* d_z,
+                         double* d_u1, const phg_bv_opts* opts, phg_bv_info* infos,
+                         phg_stream_t stream)
+{
+    return lin_entry("phg_barvortex_calc_L", 0, B, nmembers, d_u, d_z, nullptr, nullptr, d_u1,
+                     opts, infos, stream);
 }
 
 int phg_barvortex_calc_L_1(phg_barvortex_t B, int nmembers, const double* d_u, const double* d_z,
                            double* d_u1, const phg_bv_opts* opts, phg_bv_info* infos,
                            phg_stream_t stream)
 {
-    if (!B || !d_u || !d_z || !d_u1 || nmembers < 1 || nmembers > B->S) {
-        set_error("phg_barvortex_calc_L_1: bad argument (1 <= nmembers <= slots)");
-        return PHG_ERR_ARG;
-    }
-    phg_bv_opts o;
-    if (opts) o = *opts;
-    else phg_bv_default_opts(&o);
-    hipStream_t s = (hipStream_t)stream;
-    int rc = lin_forward(B, 1, nmembers, d_u, d_z, d_u1, o, s);
-    return rc ? rc : lin_finish(B, nmembers, infos, s);
+    return lin_entry("phg_barvortex_calc_L_1", 1, B, nmembers, d_u, d_z, nullptr, nullptr, d_u1,
+                     opts, infos, stream);
 }
 
 int phg_barvortex_calc_LT(phg_barvortex_t B, int nmembers, const double* d_v, const double* d_z,
                           double* d_v1, const phg_bv_opts* opts, phg_bv_info* infos,
                           phg_stream_t stream)
 {
-    if (!B || !d_v || !d_z || !d_v1 || nmembers < 1 || nmembers > B->S) {
-        set_error("phg_barvortex_calc_LT: bad argument (1 <= nmembers <= slots)");
-        return PHG_ERR_ARG;
-    }
-    phg_bv_opts o;
-    if (opts) o = *opts;
-    else phg_bv_default_opts(&o);
-    hipStream_t s = (hipStream_t)stream;
-    int rc = lin_adjoint(B, nmembers, d_v, d_z, d_v1, o, s);
-    return rc ? rc : lin_finish(B, nmembers, infos, s);
+    return lin_entry("phg_barvortex_calc_LT", 2, B, nmembers, d_v, d_z, nullptr, nullptr, d_v1,
+                     opts, infos, stream);
+}
+
+int phg_barvortex_calc_L_bnd(phg_barvortex_t B, int nmembers, const double* d_u,
+                             const double* d_z, const double* d_bnd_u, const double* d_bnd_w,
+                             double* d_u1, const phg_bv_opts* opts, phg_bv_info* infos,
+                             phg_stream_t stream)
+{
+    return lin_entry("phg_barvortex_calc_L_bnd", 0, B, nmembers, d_u, d_z, d_bnd_u, d_bnd_w,
+                     d_u1, opts, infos, stream);
+}
+
+int phg_barvortex_calc_L_1_bnd(phg_barvortex_t B, int nmembers, const double* d_u,
+                               const double* d_z, const double* d_bnd_u, const double* d_bnd_w,
+                               double* d_u1, const phg_bv_opts* opts, phg_bv_info* infos,
+                               phg_stream_t stream)
+{
+    return lin_entry("phg_barvortex_calc_L_1_bnd", 1, B, nmembers, d_u, d_z, d_bnd_u, d_bnd_w,
+                     d_u1, opts, infos, stream);
+}
+
+int phg_barvortex_calc_LT_bnd(phg_barvortex_t B, int nmembers, const double* d_v,
+                              const double* d_z, const double* d_bnd_u, const double* d_bnd_w,
+                              double* d_v1, const phg_bv_opts* opts, phg_bv_info* infos,
+                              phg_stream_t stream)
+{
+    return lin_entry("phg_barvortex_calc_LT_bnd", 2, B, nmembers, d_v, d_z, d_bnd_u, d_bnd_w,
+                     d_v1, opts, infos, stream);
 }
 
 }  // extern "C"

@@ -66,6 +66,19 @@ __global__ void bv_p2u_bnd(int V, int S, const int* __restrict__ isb, const int*
 }
 
 // column s of a rows x S array = v (or 0 when v is null)
+// u = set_bnd(u, bnd) for the masked columns (src/proj.cpp:143-160): the
+// boundary points take bnd (outer x S, indexed by outer number), the inner
+// points keep their values.
+__global__ void bv_set_bnd_cols(int V, int S, const int* __restrict__ isb,
+                                const int* __restrict__ p2io, const double* __restrict__ bnd,
+                                double* __restrict__ u, const int* __restrict__ mask)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)V * S) return;
+    const int k = (int)(e / S), s = (int)(e % S);
+    if (mask[s] && isb[k]) u[e] = bnd[(size_t)p2io[k] * S + s];
+}
+
 __global__ void bv_col_set(int rows, int S, int s, const double* __restrict__ v,
                            double* __restrict__ M)
 {

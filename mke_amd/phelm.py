@@ -111,6 +111,9 @@ SIGNATURES = {
     "phg_barvortex_calc_L": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     "phg_barvortex_calc_L_1": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     "phg_barvortex_calc_LT": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
+    "phg_barvortex_calc_L_bnd": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "phg_barvortex_calc_L_1_bnd": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "phg_barvortex_calc_LT_bnd": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     "phg_bc_default_opts": (None, [_P]),
     "phg_baroclin_create": (_I, [_P, _P, _P, _I, _P, C.POINTER(_P)]),
     "phg_baroclin_destroy": (None, [_P]),
@@ -629,29 +632,34 @@ class BarVortex:
         x, y = self._xy[:, 0], self._xy[:, 1]
         return np.asarray(fn(x, y, t, self.conf.mu, self.conf.sigma), dtype=np.float64)
 
-    def _lin(self, name, out, a, z, opts, stream):
+    def _lin(self, name, out, a, z, opts, stream, bnd_u=None, bnd_w=None):
         k = 1 if a.dim() == 1 else int(a.shape[0])
         if tuple(z.shape) != tuple(a.shape) or tuple(out.shape) != tuple(a.shape):
             raise PhelmError("BarVortex.%s: arguments must have one shape" % name)
+        self._check_bnd("BarVortex." + name, k, bnd_u, bnd_w)
         infos = (BvInfo * k)()
-        rc = getattr(lib(), "phg_barvortex_" + name)(
-            self.h, k, _ptr(a), _ptr(z), _ptr(out), C.byref(opts) if opts is not None else None,
-            infos, _stream(stream))
+        opt = lambda t: _ptr(t) if t is not None else None
+        rc = getattr(lib(), "phg_barvortex_%s_bnd" % name)(
+            self.h, k, _ptr(a), _ptr(z), opt(bnd_u), opt(bnd_w), _ptr(out),
+            C.byref(opts) if opts is not None else None, infos, _stream(stream))
         check(rc, "phg_barvortex_" + name)
         return [_bv_info(i) for i in infos]
 
-    def calc_L(self, u1, u, z, opts=None, stream=None):
-        """BarVortex::calc_L(u1, u, z, 0, 0, t) (examples/barvortex_impl.h:326-400)."""
-        return self._lin("calc_L", u1, u, z, opts, stream)
+    def calc_L(self, u1, u, z, opts=None, stream=None, bnd_u=None, bnd_w=None):
+        """BarVortex::calc_L(u1, u, z, bnd_u, bnd_w, t)
+        (examples/barvortex_impl.h:326-400); bnd_u, bnd_w as in calc (None:
+        zeros)."""
+        return self._lin("calc_L", u1, u, z, opts, stream, bnd_u, bnd_w)
 
-    def calc_L_1(self, u1, u, z, opts=None, stream=None):
+    def calc_L_1(self, u1, u, z, opts=None, stream=None, bnd_u=None, bnd_w=None):
         """BarVortex::calc_L_1 (examples/barvortex_impl.h:402-467): adds to
         u1's content, as the reference does."""
-        return self._lin("calc_L_1", u1, u, z, opts, stream)
+        return self._lin("calc_L_1", u1, u, z, opts, stream, bnd_u, bnd_w)
 
-    def calc_LT(self, v1, v, z, opts=None, stream=None):
-        """BarVortex::calc_LT(v1, v, z, 0, 0, t) (examples/barvortex_impl.h:469-535)."""
-        return self._lin("calc_LT", v1, v, z, opts, stream)
+    def calc_LT(self, v1, v, z, opts=None, stream=None, bnd_u=None, bnd_w=None):
+        """BarVortex::calc_LT(v1, v, z, bnd_u, bnd_w, t)
+        (examples/barvortex_impl.h:469-535)."""
+        return self._lin("calc_LT", v1, v, z, opts, stream, bnd_u, bnd_w)
 
     def S_step(self, ans, F, opts=None, stream=None):
         """BarVortex::S_step (examples/barvortex_impl.h:538-542): calc(ans, F, 0, 0, 0)."""

@@ -11,8 +11,9 @@ parity unpinned); here each is a sparse LU factorisation, i.e. the exact
 answer any converged solve approximates.  Vector operations follow the
 reference's order; the linal helpers it calls are read from their call
 sites: vec_sum1(r, a, b, k1, k2) = k1 a + k2 b, vec_sum2(r, a, b, k) =
-a + k b, vec_sum = a + b, vec_diff = a - b.  Boundary (pole) values are zero,
-as the reference drivers pass them (examples/test_barvortex.cpp:445).
+a + k b, vec_sum = a + b, vec_diff = a - b.  Boundary (pole) values bnd_u
+(of psi) and bnd_w (of its Laplacian) are optional everywhere (None: zero, as
+the reference drivers pass them, examples/test_barvortex.cpp:445).
 """
 import math
 
@@ -45,6 +46,9 @@ class BarVortex:
         self.Ab = mesh.assemble(O.KIND_HELMHOLTZ, 1.0 / tau - sigma * (1.0 - theta),
                                 (1.0 - theta) * mu)
         self.bndA = mesh.assemble_boundary(O.KIND_HELMHOLTZ, cm, cl)    # bnd_ (integrate_cb)
+        # bndb_: integrate_backward_cb on the boundary columns (barvortex_impl.h:141-153)
+        self.bndAb = mesh.assemble_boundary(O.KIND_HELMHOLTZ, 1.0 / tau - sigma * (1.0 - theta),
+                                            (1.0 - theta) * mu)
         self.bnd2 = mesh.assemble_boundary(O.KIND_HELMHOLTZ, 0.0, -1.0)  # laplace_bnd2_cb
         # barvortex_impl.h:287-293 ADDS bnd_ bnd_w to the right part, where the
         # Dirichlet elimination of the same operator subtracts it; +1 follows
@@ -129,7 +133,7 @@ class BarVortex:
         return u_n, it, rel
 
 
-    # ---- linearised and adjoint steps (barvortex_impl.h:326-571), bnd = 0 --
+    # ---- linearised and adjoint steps (barvortex_impl.h:326-571) ----
     def jac1(self, u, v):
         """SphereJacobian::calc1 with bnd = 0 (src/sjacobian.cpp:456-461)."""
         return self.p2u(self.J.calc2(u, v))
@@ -138,13 +142,18 @@ class BarVortex:
         """SphereJacobian::calc1t with bnd = 0 (src/sjacobian.cpp:463-468)."""
         return self.p2u(self.J.calc2t(u, v))
 
-    def _solve_full(self, lu, u1):
-        """phelm::solve of idt u2p(u1) (mesh.h:719-746) with bnd = 0."""
-        return self.p2u(lu.solve(O.spmv(*self.idt, u1[self.m.inner])))
+    def _solve_full(self, lu, u1, bndm=None, bnd_w=None):
+        """The reference's right part idt u2p(u1), plus bndm bnd_w when
+        bnd_w is given (barvortex_impl.h:389-397), then phelm::solve
+        (mesh.h:719-746): p2u(A^-1 rp, bnd_w)."""
+        rp = O.spmv(*self.idt, u1[self.m.inner])
+        if bnd_w is not None:
+            rp = rp + self._bmul(bndm, bnd_w)
+        return self.p2u(lu.solve(rp), bnd_w)
 
-    def _lin_terms(self, u, z):
-        z_lapl = self.calc1(z)
-        pt1 = self.calc1(u)
+    def _lin_terms(self, u, z, bnd_w=None):
+        z_lapl = self.calc1(z, bnd_w)
+        pt1 = self.calc1(u, bnd_w)
         jac1 = self.jac1(u, self.lh)
         jac2 = self.jac1(z, pt1)
         jac3 = self.jac1(u, z_lapl)
@@ -154,10 +163,11 @@ class BarVortex:
         pt3 = pt3 + self.k1 * jac3
         return pt1, pt3
 
-    def calc_L(self, u, z):
+    def calc_L(self, u, z, bnd_u=None, bnd_w=None):
         """BarVortex::calc_L (barvortex_impl.h:326-400)."""
         th, mu, sigma, tau = self.theta, self.mu, self.sigma, self.tau
-        pt1, pt3 = self._lin_terms(np.asarray(u, dtype=np.float64), np.asarray(z, dtype=np.float64))
+        pt1, pt3 = self._lin_terms(np.asarray(u, dtype=np.float64),
+                                   np.asarray(z, dtype=np.float64), bnd_w)
         pt3 = pt3 * -1.0
         pt2 = self.calc1(pt1)
         pt2 = pt2 * ((1.0 - th) * mu)
@@ -166,14 +176,15 @@ class BarVortex:
         u1 = u1 + pt1
         u1 = u1 + pt2
         u1 = u1 + pt3
-        u1 = self._solve_full(self.lu_A, u1)
-        return self.poisson(u1)
+        u1 = self._solve_full(self.lu_A, u1, self.bndA, bnd_w)
+        return self.poisson(u1, bnd_u)
 
-    def calc_L_1(self, u1, u, z):
+    def calc_L_1(self, u1, u, z, bnd_u=None, bnd_w=None):
         """BarVortex::calc_L_1 (barvortex_impl.h:402-467): adds to u1 as the
         reference does (its caller L_1_step passes zeros)."""
         th, mu, sigma, tau = self.theta, self.mu, self.sigma, self.tau
-        pt1, pt3 = self._lin_terms(np.asarray(u, dtype=np.float64), np.asarray(z, dtype=np.float64))
+        pt1, pt3 = self._lin_terms(np.asarray(u, dtype=np.float64),
+                                   np.asarray(z, dtype=np.float64), bnd_w)
         pt2 = self.calc1(pt1)
         pt2 = pt2 * (-th * mu)
         pt1 = pt1 * (1.0 / tau + th * sigma)
@@ -181,20 +192,20 @@ class BarVortex:
         r = r + pt1
         r = r + pt2
         r = r + pt3
-        r = self._solve_full(self.lu_Ab, r)
-        return self.poisson(r)
+        r = self._solve_full(self.lu_Ab, r, self.bndAb, bnd_w)
+        return self.poisson(r, bnd_u)
 
-    def calc_LT(self, v, z):
+    def calc_LT(self, v, z, bnd_u=None, bnd_w=None):
         """BarVortex::calc_LT (barvortex_impl.h:469-535)."""
         th, mu, sigma, tau = self.theta, self.mu, self.sigma, self.tau
         k1, k2 = self.k1, self.k2
         v = np.asarray(v, dtype=np.float64)
-        lz = self.calc1(np.asarray(z, dtype=np.float64))
-        v1 = self.poisson(v)
-        v1 = self._solve_full(self.lu_A, v1)
-        pt1 = self.calc1(v1)
+        lz = self.calc1(np.asarray(z, dtype=np.float64), bnd_w)
+        v1 = self.poisson(v, bnd_u)
+        v1 = self._solve_full(self.lu_A, v1, self.bndA, bnd_w)
+        pt1 = self.calc1(v1, bnd_w)
         pt1 = pt1 * (1.0 / tau - (1.0 - th) * sigma)
-        pt2 = self.calc1(v1)
+        pt2 = self.calc1(v1, bnd_w)
         pt2 = self.calc1(pt2)
         pt2 = pt2 * ((1.0 - th) * mu)
         tmp = self.jac1t(v1, self.lh)
@@ -208,7 +219,8 @@ class BarVortex:
         out = out + pt1
         out = out + pt2
         out = out + h1
-        out[self.m.outer] = 0.0                             # set_bnd(v1, 0)
+        # set_bnd(v1, bnd_u) (src/proj.cpp:143-160)
+        out[self.m.outer] = 0.0 if bnd_u is None else np.asarray(bnd_u, dtype=np.float64)
         return out
 
 

@@ -236,3 +236,54 @@ def test_barvortex_boundary_values_vs_oracle(level):
     bv.calc(plain, _dev(U[1:]))
     assert torch.equal(mix[0], plain[0])
     assert torch.equal(mix[1], U1[0])
+
+
+def test_barvortex_linear_steps_with_boundary_values_vs_oracle():
+    """calc_L, calc_L_1 and calc_LT with non-zero boundary (pole) values
+    (barvortex_impl.h:326-535), two members with different values, against
+    the oracle (oracle/barvortex.py); zero values give the plain steps' bits;
+    a later plain step on the same engine is unaffected; calc_LT ends with
+    set_bnd(v1, bnd_u)."""
+    print("barvortex linear steps with boundary values level=3", flush=True)
+    from oracle.barvortex import BarVortex as OB
+    level = 3
+    om, pr, Z, lh = _states(level, 2)
+    pm = P.Mesh.load(oracle_mesh(level)[0])
+    bv = P.BarVortex(pm, _dev(lh), slots=2, **CONF)
+    ob = OB(om, lh=lh, **CONF)
+    U = _lin_fields(pr, om, (1, 2)) + np.array([[0.3], [-0.2]])
+    W, Vv = _lin_fields(pr, om, (3, 4)), _lin_fields(pr, om, (5, 6))
+    bu = np.array([[0.31], [-0.19]])
+    bw = np.array([[-0.7], [0.45]])
+    opts = P.bv_opts(rtol_poisson=1e-12)
+    L = torch.zeros_like(_dev(U))
+    for i in bv.calc_L(L, _dev(U), _dev(Z), opts=opts, bnd_u=_dev(bu), bnd_w=_dev(bw)):
+        assert i["converged"], i
+    L1 = _dev(W).clone()
+    bv.calc_L_1(L1, _dev(U), _dev(Z), opts=opts, bnd_u=_dev(bu), bnd_w=_dev(bw))
+    LT = torch.zeros_like(_dev(Vv))
+    bv.calc_LT(LT, _dev(Vv), _dev(Z), opts=opts, bnd_u=_dev(bu), bnd_w=_dev(bw))
+    for q in range(2):
+        for name, got, ref in (
+                ("L", L, ob.calc_L(U[q], Z[q], bu[q], bw[q])),
+                ("L_1", L1, ob.calc_L_1(W[q], U[q], Z[q], bu[q], bw[q])),
+                ("LT", LT, ob.calc_LT(Vv[q], Z[q], bu[q], bw[q]))):
+            g = got.cpu().numpy()[q]
+            err = ob.norm(g - ref) / ob.norm(ref)
+            assert err <= 1e-9, (name, q, err)
+            if name == "L":   # the values matter
+                assert ob.norm(ob.calc_L(U[q], Z[q]) - ref) > 1e-4 * ob.norm(ref)
+        assert L.cpu().numpy()[q][om.outer[0]] == bu[q][0]
+        assert LT.cpu().numpy()[q][om.outer[0]] == bu[q][0]
+    # zero boundary values: the plain steps' bits
+    zero = torch.zeros(2, pm.outer_size, dtype=torch.float64, device=DEV)
+    Lp, Lz = torch.zeros_like(L), torch.zeros_like(L)
+    bv.calc_L(Lp, _dev(U), _dev(Z), opts=opts)
+    bv.calc_L(Lz, _dev(U), _dev(Z), opts=opts, bnd_u=zero, bnd_w=zero)
+    assert torch.equal(Lp, Lz)
+    # a plain step after them on the same engine: the bits of a fresh engine
+    S1, S2 = torch.zeros_like(L), torch.zeros_like(L)
+    bv.calc(S1, _dev(U))
+    fresh = P.BarVortex(pm, _dev(lh), slots=2, **CONF)
+    fresh.calc(S2, _dev(U))
+    assert torch.equal(S1, S2)

@@ -235,3 +235,43 @@ def test_boundary_values_of_omega_steady_state():
     assert cons[-1] < 2e-3, cons
     assert cons[0] / cons[1] > 3.0 and cons[1] / cons[2] > 3.0, cons
     assert all(0.1 < e < 0.3 for e in lit), lit     # the reference's offset
+
+
+def test_linear_steps_with_boundary_values():
+    """Boundary (pole) values in the linearised steps (barvortex_impl.h:326-535).
+    calc_L without the Jacobian terms is the step's linear part, so with
+    boundary values it equals BarVortex::calc with the same values (whose
+    boundary handling the two tests above pin); dropping the bnd_ term, the
+    p2u of bnd_w or the Poisson solve's bnd_u moves it by far more.  calc_L_1
+    and calc_LT are affine in (state, boundary values) jointly, calc_LT ends
+    with set_bnd(v1, bnd_u), and zero boundary values give the plain steps'
+    results.  Where bnd_w enters calc_L_1 (bndb_, the backward boundary
+    matrix) and calc_LT (calc1 of v1 with bnd_w, twice) follows the code:
+    pinned by reading only."""
+    M, field, lh, conf = _lin_setup(4, tau=1e-2)
+    b0 = BarVortex(M, k1=0.0, k2=0.0, lh=lh, **conf)
+    u, z, w = field(1) + 0.4, field(3), field(5)
+    bu, bw = [0.4], [-0.7]
+    Lb = b0.calc_L(u, z, bu, bw)
+    S, _, _ = b0.calc(u, max_iters=50, bnd_u=bu, bnd_w=bw)
+    nL = b0.norm(Lb)
+    assert b0.norm(Lb - S) <= 1e-12 * nL
+    assert b0.norm(b0.calc_L(u, z) - S) > 1e-3 * nL                 # the values matter
+    assert Lb[M.outer[0]] == 0.4                                       # p2u(., bnd_u)
+    # planted mistakes, each far outside the agreement above
+    for bad in (dict(bnd_u=bu, bnd_w=None), dict(bnd_u=None, bnd_w=bw)):
+        assert b0.norm(b0.calc_L(u, z, **bad) - S) > 1e-6 * nL, bad
+    zero = np.zeros(1)
+    assert np.array_equal(b0.calc_L(u, z, zero, zero), b0.calc_L(u, z))
+    # calc_L_1 and calc_LT: affine in (state, boundary values) jointly
+    o = np.zeros(len(u))
+    for f in (lambda s, a, c: b0.calc_L_1(s, a, z, c[0], c[1]),
+              lambda s, a, c: b0.calc_LT(a, z, c[0], c[1]) + 0.0 * s):
+        full = f(w, u, (bu, bw))
+        parts = f(w, u, (zero, zero)) + f(o, o, (bu, bw))
+        assert b0.norm(full - parts) <= 1e-12 * b0.norm(full)
+        assert b0.norm(f(o, o, (bu, bw))) > 1e-6 * b0.norm(full)
+    LT = b0.calc_LT(u, z, bu, bw)
+    assert LT[M.outer[0]] == 0.4                                       # set_bnd(v1, bnd_u)
+    assert np.array_equal(b0.calc_L_1(w, u, z, zero, zero), b0.calc_L_1(w, u, z))
+    assert np.array_equal(b0.calc_LT(u, z, zero, zero), b0.calc_LT(u, z))
