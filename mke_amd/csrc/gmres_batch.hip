@@ -216,6 +216,86 @@ struct BRow<float, MT> {
     }
 };
 
+// One term of BRow's sum: r + val * (x * s) in FP64 (no fused multiply-add),
+// fmaf(val, x * s, r) in FP32.
+template <typename VT, typename MT>
+struct BTerm;
+template <>
+struct BTerm<double, double> {
+    __device__ __forceinline__ static double add(double r, double v, double x, double s)
+    {
+        return r + v * (x * s);
+    }
+};
+template <typename MT>
+struct BTerm<float, MT> {
+    __device__ __forceinline__ static float add(float r, MT v, float x, float s)
+    {
+        return fmaf(ell_val_f32(v), x * s, r);
+    }
+};
+
+// BRow of this thread's rows base + t * RB (t < RPT) of column q, and xo[t]
+// = X(row, q); w[t] = xo[t] = 0 for rows at or past n and inactive columns.
+// For rows of at most ELL_MAXW entries, RG rows at a time: every row's own
+// entry and column/value slots (ell_slot: past the width the last slot
+// again, in bounds) are loaded first, then every row's gathers, then the
+// sums in BRow's order; two memory round trips per group instead of three
+// per row.
+template <typename VT, typename MT, int RPT, int RG>
+__device__ __forceinline__ void brows(int n, int width, int KP, int q, int base, int RB, bool act,
+                                      const int* __restrict__ col, const MT* __restrict__ val,
+                                      const VT* __restrict__ X, VT s, VT (&xo)[RPT],
+                                      VT (&w)[RPT])
+{
+    if (width >= 1 && width <= ELL_MAXW) {
+#pragma unroll
+        for (int g = 0; g < RPT; g += RG) {
+            int c[RG][ELL_MAXW];
+            MT v[RG][ELL_MAXW];
+            VT xv[RG][ELL_MAXW];
+#pragma unroll
+            for (int u = 0; u < RG; ++u) {
+                const int row = min(base + (g + u) * RB, n - 1);  // in bounds; unused past n
+                xo[g + u] = X[(size_t)row * KP + q];
+#pragma unroll
+                for (int k = 0; k < ELL_MAXW; ++k) {
+                    const size_t o = ell_slot(k, width, n, row);
+                    c[u][k] = col[o];
+                    v[u][k] = val[o];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < RG; ++u)
+#pragma unroll
+                for (int k = 0; k < ELL_MAXW; ++k) xv[u][k] = X[(size_t)c[u][k] * KP + q];
+#pragma unroll
+            for (int u = 0; u < RG; ++u) {
+                VT r = VT(0);
+#pragma unroll
+                for (int k = 0; k < ELL_MAXW; ++k) {
+                    const VT t = BTerm<VT, MT>::add(r, v[u][k], xv[u][k], s);
+                    r = k < width ? t : r;
+                }
+                const bool in = act && base + (g + u) * RB < n;
+                w[g + u] = in ? r : VT(0);
+                xo[g + u] = in ? xo[g + u] : VT(0);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < RPT; ++t) {
+            const int row = base + t * RB;
+            w[t] = VT(0);
+            xo[t] = VT(0);
+            if (act && row < n) {
+                xo[t] = X[(size_t)row * KP + q];
+                w[t] = BRow<VT, MT>::run(n, width, row, col, val, X, KP, q, s);
+            }
+        }
+    }
+}
+
 // Block partials of <V_i, w> for i < nd (i == own_i taken from `own`), or
 // of ||w||^2 when nd == 0, for this thread's rows; written to
 // part[((out0 + i) * KP + q) * G + block].  All threads of the block call it.
@@ -301,17 +381,17 @@ bk_step_spmm(int n, int KP, int K, int width, const int* __restrict__ col,
     if (act) h = j == 0 ? cols[q].beta : cols[q].hn;
     const VT inv = (VT)(1.0 / h);
     VT* Vj = V + (size_t)j * bs;
-    VT w[RPT], vj[RPT];
+    VT w[RPT], vj[RPT], xo[RPT];
+    brows<VT, MT, RPT, (RPT < 4 ? RPT : 4)>(n, width, KP, q, base, RB, act, col, val, Wsrc, inv,
+                                            xo, w);
 #pragma unroll
     for (int t = 0; t < RPT; ++t) {
         const int row = base + t * RB;
-        w[t] = VT(0);
         vj[t] = VT(0);
         if (act && row < n) {
             const size_t o = (size_t)row * KP + q;
-            vj[t] = Wsrc[o] * inv;
+            vj[t] = xo[t] * inv;
             Vj[o] = vj[t];
-            w[t] = BRow<VT, MT>::run(n, width, row, col, val, Wsrc, KP, q, inv);
             Wdst[o] = w[t];
         }
     }
@@ -345,15 +425,13 @@ bk_step_spmm_pc(int n, int KP, int K, int width, const int* __restrict__ col,
     const int RB = BB_BS / KP;
     const int base = lblock() * RB * RPT + (int)threadIdx.x / KP;
     const bool act = q < K && cols[q].active;
-    VT w[RPT];
+    VT w[RPT], xo[RPT];
+    brows<VT, MT, RPT, (RPT < 4 ? RPT : 4)>(n, width, KP, q, base, RB, act, col, val, Zj, VT(1),
+                                            xo, w);
 #pragma unroll
     for (int t = 0; t < RPT; ++t) {
         const int row = base + t * RB;
-        w[t] = VT(0);
-        if (act && row < n) {
-            w[t] = BRow<VT, MT>::run(n, width, row, col, val, Zj, KP, q, VT(1));
-            Wdst[(size_t)row * KP + q] = w[t];
-        }
+        if (act && row < n) Wdst[(size_t)row * KP + q] = w[t];
     }
     bdots<VT, RPT>(j + 1, -1, nullptr, V, bs, n, KP, q, base, RB, act, w, part, G, 0);
 }
