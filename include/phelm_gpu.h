@@ -561,6 +561,23 @@ int phg_baroclin_calc_L_1(phg_baroclin_t B, int nmembers, const double *d_u1,
                           const double *d_u2, const double *d_z1, const double *d_z2,
                           double *d_u11, double *d_u21, const phg_bc_opts *opts,
                           phg_bc_info *infos, phg_stream_t s);
+/* calc_L and calc_L_1 with the boundary (pole) values (Baroclin::calc_L /
+ * calc_L_1 with a non-NULL bnd, examples/baroclin.cpp:620-802, 804-986): d_bnd
+ * holds, member after member, n_outer values (outer order) used as the
+ * reference uses its one bnd array: the boundary values of every Laplacian
+ * (u, w and z: :675-682, :859-866), of the right part's boundary branch for
+ * w1, w2, u1 and u2 (A_'s elements for calc_L, :708-711 with right_part_cb;
+ * Ab_'s for calc_L_1, :892-895 with right_part_backward_cb, :341-436) and of
+ * the four p2u (:783-786, :967-970).  NULL = zeros, the plain entries'
+ * results. */
+int phg_baroclin_calc_L_bnd(phg_baroclin_t B, int nmembers, const double *d_u1,
+                            const double *d_u2, const double *d_z1, const double *d_z2,
+                            const double *d_bnd, double *d_u11, double *d_u21,
+                            const phg_bc_opts *opts, phg_bc_info *infos, phg_stream_t s);
+int phg_baroclin_calc_L_1_bnd(phg_baroclin_t B, int nmembers, const double *d_u1,
+                              const double *d_u2, const double *d_z1, const double *d_z2,
+                              const double *d_bnd, double *d_u11, double *d_u21,
+                              const phg_bc_opts *opts, phg_bc_info *infos, phg_stream_t s);
 
 /* ---- serving: implicit-step requests of ensemble members on several mesh
  *      levels, continuously batched (no counterpart in the reference, which

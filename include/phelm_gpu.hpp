@@ -550,10 +550,8 @@ struct BarVortexConf {
 // (examples/barvortex_impl.h:171-310) on the device, calc_L / calc_L_1 /
 // calc_LT the linearised, inverse-linearised and adjoint steps
 // (:326-535) and S_step / L_step / L_1_step / LT_step their wrappers
-// (:537-571).  calc takes the boundary (pole) values bnd_u / bnd_w (device
-// pointers, outer-size each, null = zeros); the linearised steps take zero
-// boundary values, as the reference drivers pass them (null or zeros),
-// and reject non-zero ones.
+// (:537-571).  calc, calc_L, calc_L_1 and calc_LT take the boundary (pole)
+// values bnd_u / bnd_w (device pointers, outer-size each, null = zeros).
 class BarVortex {
 public:
     // the Coriolis term as a device mesh function (the callbacks of conf are
@@ -732,23 +730,24 @@ public:
         fv_.upload(hf.data());
         gv_.upload(hg.data());
         check(phg_baroclin_calc_bnd(h_, 1, u1, u2, f_ ? fv_.data() : nullptr,
-                                    g_ ? gv_.data() : nullptr, bnd, u11, u21, &opts_, &info_,
-                                    nullptr),
+                                    g_ ? gv_.data() : nullptr, os_ ? bnd : nullptr, u11, u21,
+                                    &opts_, &info_, nullptr),
               "Baroclin::calc");
     }
     // the linearised steps about (z1, z2) (baroclin.cpp:607-986); bnd as calc
+    // (device, n_outer values, or null for zeros)
     void calc_L(double* u11, double* u21, const double* u1, const double* u2, const double* z1,
                 const double* z2, const double* bnd, double)
     {
-        check_zero_bnd(bnd, "Baroclin::calc_L");
-        check(phg_baroclin_calc_L(h_, 1, u1, u2, z1, z2, u11, u21, &opts_, &info_, nullptr),
+        check(phg_baroclin_calc_L_bnd(h_, 1, u1, u2, z1, z2, os_ ? bnd : nullptr, u11, u21,
+                                      &opts_, &info_, nullptr),
               "Baroclin::calc_L");
     }
     void calc_L_1(double* u11, double* u21, const double* u1, const double* u2,
                   const double* z1, const double* z2, const double* bnd, double)
     {
-        check_zero_bnd(bnd, "Baroclin::calc_L_1");
-        check(phg_baroclin_calc_L_1(h_, 1, u1, u2, z1, z2, u11, u21, &opts_, &info_, nullptr),
+        check(phg_baroclin_calc_L_1_bnd(h_, 1, u1, u2, z1, z2, os_ ? bnd : nullptr, u11, u21,
+                                        &opts_, &info_, nullptr),
               "Baroclin::calc_L_1");
     }
     // the reference's calc_LT has an empty body (baroclin.cpp:988-993)
@@ -779,19 +778,6 @@ private:
         double sigma, mu, sigma1, mu1, alpha;
     };
     Config conf_c() const { return {conf_.sigma, conf_.mu, conf_.sigma1, conf_.mu1, conf_.alpha}; }
-    // the linearised steps run with zero boundary values: a bnd array that
-    // holds anything else is refused rather than ignored
-    void check_zero_bnd(const double* bnd, const char* what) const
-    {
-        if (!bnd || os_ == 0) return;
-        std::vector<double> h((size_t)os_);
-        check_hip(hipMemcpy(h.data(), bnd, h.size() * sizeof(double), hipMemcpyDeviceToHost),
-                  what);
-        for (double v : h)
-            if (v != 0.0)
-                throw Error(std::string(what) + ": non-zero boundary values are not supported",
-                            PHG_ERR_ARG);
-    }
     const Mesh& m_;
     rp_t f_, g_;
     phg_bc_conf conf_;

@@ -313,6 +313,7 @@ struct phg_baroclin_s {
     DevBuf<double> Ub;                                      // outer x S
     DevBuf<double> BND;                                     // outer x S boundary values
     phg_matrix_s* bndr[4] = {};     // -(sum of block row r's boundary blocks), n x outer
+    phg_matrix_s* bndrb[4] = {};    // the same for Ab_'s blocks (built with Ab_)
     DevBuf<double> RP, ANS;                                 // 4n x S
     DevBuf<double> Z1, Z2, DZ1, DZ2, JAC2;  // linearised steps: z, L(z) (V x S), n x 5S
     DevBuf<double> part, nrm;
@@ -326,6 +327,7 @@ struct phg_baroclin_s {
         if (Ab4) phg_matrix_destroy(Ab4);
         delete bnd3;
         for (auto* b : bndr) delete b;
+        for (auto* b : bndrb) delete b;
         delete norm;
         if (J) phg_jacobian_destroy(J);
         if (mg) phg_mg_destroy(mg);
@@ -414,6 +416,26 @@ int dists(phg_baroclin_s* B, const double* a, const double* b, std::vector<doubl
 }
 
 // The 4n x 4n CSR of the nine blocks (all on the inner stencil).
+// The right part's boundary branch for one set of the nine element formulas
+// (c_mass, c_lap): block row r's boundary elements, every block column
+// multiplied by the one bnd array (BW1 = BW2 = BU1 = BU2 = bnd), so per block
+// row the boundary block of the row's summed coefficients, negated.
+int boundary_rows(phg_baroclin_s* B, const double* cm, const double* cl, hipStream_t s,
+                  phg_matrix_s** out)
+{
+    const int row[9] = {0, 0, 1, 1, 1, 2, 2, 3, 3};
+    double sm[4] = {}, sl[4] = {};
+    for (int k = 0; k < 9; ++k) {
+        sm[row[k]] += cm[k];
+        sl[row[k]] += cl[k];
+    }
+    for (int r = 0; r < 4; ++r) {
+        int rc = assemble_boundary_device(B->mesh, PHG_HELMHOLTZ, -sm[r], -sl[r], s, &out[r]);
+        if (rc) return rc;
+    }
+    return PHG_OK;
+}
+
 int stack_blocks(phg_baroclin_s* B, phg_matrix_s* const* blk, phg_matrix_t* out)
 {
     const int n = B->n;
@@ -477,6 +499,9 @@ int ensure_backward(phg_baroclin_s* B, hipStream_t s)
     }
     int rc = stack_blocks(B, B->blkb, &B->Ab4);
     if (rc) return rc;
+    // right_part_backward_cb's boundary branch (baroclin.cpp:341-436) on
+    // integrate_backward_cb's elements
+    if (B->os && (rc = boundary_rows(B, cm, cl, s, B->bndrb))) return rc;
     BcPrecond& P = B->pcb;
     const int n = B->n;
     P.n = n;
@@ -502,10 +527,14 @@ int ensure_backward(phg_baroclin_s* B, hipStream_t s)
 
 // Baroclin::calc_L (which 0, baroclin.cpp:620-802: one pass with A_) and
 // calc_L_1 (which 1, :804-986: up to 10 passes with Ab_, stopping when the
-// distance is below 1e-8), with zero boundary values, for members 0..nm-1.
+// distance is below 1e-8) for members 0..nm-1.  d_bnd (outer x nm, member k
+// at k * outer, or null for zero): the one boundary array the reference
+// passes for every Laplacian (u, w and z: :675-682, :859-866), for the right
+// part's boundary branch of A_ or Ab_ (BW1 = BW2 = BU1 = BU2 = bnd,
+// :708-711, :892-895) and for the four p2u (:783-786, :967-970).
 int bc_linear(phg_baroclin_s* B, int which, int nm, const double* d_u1, const double* d_u2,
-              const double* d_z1, const double* d_z2, double* d_u11, double* d_u21,
-              const phg_bc_opts& o, phg_bc_info* infos, hipStream_t s)
+              const double* d_z1, const double* d_z2, const double* d_bnd, double* d_u11,
+              double* d_u21, const phg_bc_opts& o, phg_bc_info* infos, hipStream_t s)
 {
     const int S = B->S, n = B->n;
     const size_t V = B->V;
@@ -531,14 +560,28 @@ int bc_linear(phg_baroclin_s* B, int which, int nm, const double* d_u1, const do
             hipLaunchKernelGGL(bv_col_in, g256(V), bb, 0, s, (int)V, S, k, src[q] + k * V, dst[q]);
     }
     PHG_HIP(hipGetLastError());
+    // boundary values only when given: without them the step is the plain one
+    // (zero values, p2u without a boundary array) whatever an earlier call took
+    const double* bnd = nullptr;
+    if (d_bnd) {
+        if (!B->BND.p) {
+            PHG_HIP(B->BND.alloc((size_t)B->os * S));
+            PHG_HIP(hipMemsetAsync(B->BND.p, 0, (size_t)B->os * S * sizeof(double), s));
+        }
+        for (int k = 0; k < nm; ++k)
+            hipLaunchKernelGGL(bv_col_set, g256(B->os), bb, 0, s, B->os, S, k,
+                               d_bnd + (size_t)k * B->os, B->BND.p);
+        PHG_HIP(hipGetLastError());
+        bnd = B->BND.p;
+    }
     std::vector<int> it_mass(S, 0), it_jac(S, 0), it_sys(S, 0), iters(S, 0), failed(S, 0);
     std::vector<double> dist(S, 0.0);
-    if ((rc = calc1(B, B->U1.p, B->W1.p, o, it_mass, s))) return rc;    // w1
-    if ((rc = calc1(B, B->U2.p, B->W2.p, o, it_mass, s))) return rc;    // w2
-    if ((rc = calc1(B, B->W1.p, B->U1n1.p, o, it_mass, s))) return rc;  // dw1 (scratch)
-    if ((rc = calc1(B, B->W2.p, B->U2n1.p, o, it_mass, s))) return rc;  // dw2 (scratch)
-    if ((rc = calc1(B, B->Z1.p, B->DZ1.p, o, it_mass, s))) return rc;   // dz1
-    if ((rc = calc1(B, B->Z2.p, B->DZ2.p, o, it_mass, s))) return rc;   // dz2
+    if ((rc = calc1(B, B->U1.p, B->W1.p, o, it_mass, s, bnd))) return rc;    // w1
+    if ((rc = calc1(B, B->U2.p, B->W2.p, o, it_mass, s, bnd))) return rc;    // w2
+    if ((rc = calc1(B, B->W1.p, B->U1n1.p, o, it_mass, s, bnd))) return rc;  // dw1 (scratch)
+    if ((rc = calc1(B, B->W2.p, B->U2n1.p, o, it_mass, s, bnd))) return rc;  // dw2 (scratch)
+    if ((rc = calc1(B, B->Z1.p, B->DZ1.p, o, it_mass, s, bnd))) return rc;   // dz1
+    if ((rc = calc1(B, B->Z2.p, B->DZ2.p, o, it_mass, s, bnd))) return rc;   // dz2
     auto lin = [&](double k1, const double* a, double k2, const double* b, double* r) {
         hipLaunchKernelGGL(bv_axpby, gv, bb, 0, s, V, S, k1, a, k2, b, r, B->dmask.p, 1);
     };
@@ -605,11 +648,19 @@ int bc_linear(phg_baroclin_s* B, int which, int nm, const double* d_u1, const do
                            B->G.p, B->dmask.p);
         spmm(B, B->blk[5], B->F.p, B->RP.p, 0, s);
         spmm(B, B->blk[5], B->G.p, B->RP.p + (size_t)n * S, 0, s);
-        // the u rows of the right part are zero here (zero boundary values);
-        // a step with boundary values on this object wrote them, so clear
-        // them for every column
-        hipLaunchKernelGGL(bv_fill0, g256((size_t)2 * n * S), bb, 0, s, (size_t)2 * n * S,
-                           B->RP.p + (size_t)2 * n * S);
+        if (bnd) {
+            // the boundary branch of A_'s or Ab_'s elements: added to the w
+            // rows, written into the u rows
+            phg_matrix_s* const* br = which == 0 ? B->bndr : B->bndrb;
+            for (int r = 0; r < 4; ++r)
+                spmm(B, br[r], bnd, B->RP.p + (size_t)r * n * S, r < 2, s);
+        } else {
+            // the u rows of the right part are zero here (zero boundary
+            // values); a step with boundary values on this object wrote
+            // them, so clear them for every column
+            hipLaunchKernelGGL(bv_fill0, g256((size_t)2 * n * S), bb, 0, s, (size_t)2 * n * S,
+                               B->RP.p + (size_t)2 * n * S);
+        }
         hipLaunchKernelGGL(bv_zero_cols, g256((size_t)4 * n * S), bb, 0, s, (size_t)4 * n, S,
                            B->RP.p, B->dmask.p);
         hipLaunchKernelGGL(bv_zero_cols, g256((size_t)4 * n * S), bb, 0, s, (size_t)4 * n, S,
@@ -625,9 +676,14 @@ int bc_linear(phg_baroclin_s* B, int which, int nm, const double* d_u1, const do
         const double* ans = B->ANS.p;
         const size_t nb = (size_t)n * S;
         double* outs[4] = {B->W1n.p, B->W2n.p, B->U1n1.p, B->U2n1.p};
-        for (int q = 0; q < 4; ++q)
-            hipLaunchKernelGGL(bv_p2u, gv, bb, 0, s, (int)V, S, B->mesh->d_isb.p,
-                               B->mesh->d_p2io.p, ans + q * nb, outs[q], B->dmask.p);
+        for (int q = 0; q < 4; ++q) {
+            if (bnd)
+                hipLaunchKernelGGL(bv_p2u_bnd, gv, bb, 0, s, (int)V, S, B->mesh->d_isb.p,
+                                   B->mesh->d_p2io.p, ans + q * nb, bnd, outs[q], B->dmask.p);
+            else
+                hipLaunchKernelGGL(bv_p2u, gv, bb, 0, s, (int)V, S, B->mesh->d_isb.p,
+                                   B->mesh->d_p2io.p, ans + q * nb, outs[q], B->dmask.p);
+        }
         PHG_HIP(hipGetLastError());
         std::vector<double> d1, d2;
         if ((rc = dists(B, B->U1n1.p, B->U1n.p, d1, s))) return rc;
@@ -718,21 +774,10 @@ int phg_baroclin_create(phg_mesh_t m, const phg_bc_conf* conf, const double* d_l
     }
     // right_part_cb's boundary branch (baroclin.cpp:259-323) subtracts every
     // block's boundary element times the one bnd array (BW1 = BW2 = BU1 =
-    // BU2 = bnd, :541-544): per block row, the boundary block of the summed
-    // coefficients, negated.
-    if (B->os) {
-        const int row[9] = {0, 0, 1, 1, 1, 2, 2, 3, 3};
-        double sm[4] = {}, sl[4] = {};
-        for (int k = 0; k < 9; ++k) {
-            sm[row[k]] += cm[k];
-            sl[row[k]] += cl[k];
-        }
-        for (int r = 0; r < 4; ++r) {
-            int rc = assemble_boundary_device(m, PHG_HELMHOLTZ, -sm[r], -sl[r], s, &B->bndr[r]);
-            if (rc) return rc;
-        }
-    }
-    int rc = stack_blocks(B.get(), B->blk, &B->A4);
+    // BU2 = bnd, :541-544)
+    int rc = B->os ? boundary_rows(B.get(), cm, cl, s, B->bndr) : PHG_OK;
+    if (rc) return rc;
+    rc = stack_blocks(B.get(), B->blk, &B->A4);
     if (rc) return rc;
     rc = assemble_boundary_device(m, PHG_LAPLACE, 1.0, 0.0, s, &B->bnd3);
     if (rc) return rc;
@@ -991,18 +1036,23 @@ int phg_baroclin_calc_bnd(phg_baroclin_t B, int nmembers, const double* d_u1,
 
 static int bc_linear_entry(phg_baroclin_t B, int which, int nmembers, const double* d_u1,
                            const double* d_u2, const double* d_z1, const double* d_z2,
-                           double* d_u11, double* d_u21, const phg_bc_opts* opts,
-                           phg_bc_info* infos, phg_stream_t stream, const char* name)
+                           const double* d_bnd, double* d_u11, double* d_u21,
+                           const phg_bc_opts* opts, phg_bc_info* infos, phg_stream_t stream,
+                           const char* name)
 {
     if (!B || !d_u1 || !d_u2 || !d_z1 || !d_z2 || !d_u11 || !d_u21 || nmembers < 1 ||
         nmembers > B->S) {
         set_error(std::string(name) + ": bad argument (1 <= nmembers <= slots)");
         return PHG_ERR_ARG;
     }
+    if (d_bnd && B->os == 0) {
+        set_error(std::string(name) + ": the mesh has no boundary points");
+        return PHG_ERR_ARG;
+    }
     phg_bc_opts o;
     if (opts) o = *opts;
     else phg_bc_default_opts(&o);
-    return bc_linear(B, which, nmembers, d_u1, d_u2, d_z1, d_z2, d_u11, d_u21, o, infos,
+    return bc_linear(B, which, nmembers, d_u1, d_u2, d_z1, d_z2, d_bnd, d_u11, d_u21, o, infos,
                      (hipStream_t)stream);
 }
 
@@ -1010,16 +1060,34 @@ int phg_baroclin_calc_L(phg_baroclin_t B, int nmembers, const double* d_u1, cons
                         const double* d_z1, const double* d_z2, double* d_u11, double* d_u21,
                         const phg_bc_opts* opts, phg_bc_info* infos, phg_stream_t stream)
 {
-    return bc_linear_entry(B, 0, nmembers, d_u1, d_u2, d_z1, d_z2, d_u11, d_u21, opts, infos,
-                           stream, "phg_baroclin_calc_L");
+    return bc_linear_entry(B, 0, nmembers, d_u1, d_u2, d_z1, d_z2, nullptr, d_u11, d_u21, opts,
+                           infos, stream, "phg_baroclin_calc_L");
 }
 
 int phg_baroclin_calc_L_1(phg_baroclin_t B, int nmembers, const double* d_u1, const double* d_u2,
                           const double* d_z1, const double* d_z2, double* d_u11, double* d_u21,
                           const phg_bc_opts* opts, phg_bc_info* infos, phg_stream_t stream)
 {
-    return bc_linear_entry(B, 1, nmembers, d_u1, d_u2, d_z1, d_z2, d_u11, d_u21, opts, infos,
-                           stream, "phg_baroclin_calc_L_1");
+    return bc_linear_entry(B, 1, nmembers, d_u1, d_u2, d_z1, d_z2, nullptr, d_u11, d_u21, opts,
+                           infos, stream, "phg_baroclin_calc_L_1");
+}
+
+int phg_baroclin_calc_L_bnd(phg_baroclin_t B, int nmembers, const double* d_u1,
+                            const double* d_u2, const double* d_z1, const double* d_z2,
+                            const double* d_bnd, double* d_u11, double* d_u21,
+                            const phg_bc_opts* opts, phg_bc_info* infos, phg_stream_t stream)
+{
+    return bc_linear_entry(B, 0, nmembers, d_u1, d_u2, d_z1, d_z2, d_bnd, d_u11, d_u21, opts,
+                           infos, stream, "phg_baroclin_calc_L_bnd");
+}
+
+int phg_baroclin_calc_L_1_bnd(phg_baroclin_t B, int nmembers, const double* d_u1,
+                              const double* d_u2, const double* d_z1, const double* d_z2,
+                              const double* d_bnd, double* d_u11, double* d_u21,
+                              const phg_bc_opts* opts, phg_bc_info* infos, phg_stream_t stream)
+{
+    return bc_linear_entry(B, 1, nmembers, d_u1, d_u2, d_z1, d_z2, d_bnd, d_u11, d_u21, opts,
+                           infos, stream, "phg_baroclin_calc_L_1_bnd");
 }
 
 }  // extern "C"

@@ -122,6 +122,8 @@ SIGNATURES = {
     "phg_baroclin_calc": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "phg_baroclin_calc_L": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "phg_baroclin_calc_L_1": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "phg_baroclin_calc_L_bnd": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "phg_baroclin_calc_L_1_bnd": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "phg_serve_default_config": (None, [_P]),
     "phg_serve_create": (_I, [_P, C.POINTER(_P)]),
     "phg_serve_destroy": (None, [_P]),
@@ -769,10 +771,7 @@ class Baroclin:
         of w1, w2, u1, u2, (nmembers, outer_size) (or (outer_size,)) FP64
         device tensor, or None for zeros.  Returns one info dict per member."""
         k = 1 if u1.dim() == 1 else int(u1.shape[0])
-        if bnd is not None and (bnd.numel() != k * self.m.outer_size or
-                                str(bnd.dtype) != "torch.float64" or not bnd.is_contiguous()):
-            raise PhelmError("Baroclin.calc: bnd must be contiguous FP64 with outer_size "
-                             "values per member")
+        self._check_bnd("Baroclin.calc", k, bnd)
         infos = (BcInfo * k)()
         rc = lib().phg_baroclin_calc_bnd(self.h, k, _ptr(u1), _ptr(u2),
                                          _ptr(f) if f is not None else None,
@@ -785,29 +784,40 @@ class Baroclin:
                      it_mass=i.it_mass, it_jacobian=i.it_jacobian, it_system=i.it_system)
                 for i in infos]
 
-    def _lin(self, name, u11, u21, u1, u2, z1, z2, opts, stream):
+    def _check_bnd(self, what, k, bnd):
+        if bnd is not None and (bnd.numel() != k * self.m.outer_size or
+                                str(bnd.dtype) != "torch.float64" or not bnd.is_contiguous()):
+            raise PhelmError("%s: bnd must be contiguous FP64 with outer_size values per "
+                             "member" % what)
+
+    def _lin(self, name, u11, u21, u1, u2, z1, z2, opts, stream, bnd):
         k = 1 if u1.dim() == 1 else int(u1.shape[0])
         for t in (u2, z1, z2, u11, u21):
             if tuple(t.shape) != tuple(u1.shape):
                 raise PhelmError("Baroclin.%s: arguments must have one shape" % name)
+        self._check_bnd("Baroclin." + name, k, bnd)
         infos = (BcInfo * k)()
-        rc = getattr(lib(), "phg_baroclin_" + name)(
-            self.h, k, _ptr(u1), _ptr(u2), _ptr(z1), _ptr(z2), _ptr(u11), _ptr(u21),
+        rc = getattr(lib(), "phg_baroclin_" + name + "_bnd")(
+            self.h, k, _ptr(u1), _ptr(u2), _ptr(z1), _ptr(z2),
+            _ptr(bnd) if bnd is not None else None, _ptr(u11), _ptr(u21),
             C.byref(opts) if opts is not None else None, infos, _stream(stream))
         check(rc, "phg_baroclin_" + name)
         return [dict(iterations=i.iterations, distance=i.distance, converged=bool(i.converged),
                      it_mass=i.it_mass, it_jacobian=i.it_jacobian, it_system=i.it_system)
                 for i in infos]
 
-    def calc_L(self, u11, u21, u1, u2, z1, z2, opts=None, stream=None):
+    def calc_L(self, u11, u21, u1, u2, z1, z2, opts=None, stream=None, bnd=None):
         """Baroclin::calc_L (examples/baroclin.cpp:607-802): the step
-        linearised about (z1, z2), one pass with A_."""
-        return self._lin("calc_L", u11, u21, u1, u2, z1, z2, opts, stream)
+        linearised about (z1, z2), one pass with A_; bnd as in calc (the
+        boundary values of every Laplacian, of the right part and of the
+        results), or None for zeros."""
+        return self._lin("calc_L", u11, u21, u1, u2, z1, z2, opts, stream, bnd)
 
-    def calc_L_1(self, u11, u21, u1, u2, z1, z2, opts=None, stream=None):
+    def calc_L_1(self, u11, u21, u1, u2, z1, z2, opts=None, stream=None, bnd=None):
         """Baroclin::calc_L_1 (examples/baroclin.cpp:804-986): up to 10 passes
-        with Ab_ until the distance is below 1e-8."""
-        return self._lin("calc_L_1", u11, u21, u1, u2, z1, z2, opts, stream)
+        with Ab_ until the distance is below 1e-8; bnd as in calc_L (the
+        right part's boundary branch from Ab_'s elements)."""
+        return self._lin("calc_L_1", u11, u21, u1, u2, z1, z2, opts, stream, bnd)
 
     def L_step(self, u11, u21, u1, u2, z1, z2, opts=None, stream=None):
         return self.calc_L(u11, u21, u1, u2, z1, z2, opts, stream)

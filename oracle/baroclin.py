@@ -11,7 +11,10 @@ with right_part_cb's interior branch (or_right_part_mass) and its boundary
 branch (baroclin.cpp:259-323): for a boundary column j0 every element a of
 integrate_cb's four block columns adds -bnd[j0] a to its block row, one bnd
 for w1, w2, u1 and u2 (BW1 = BW2 = BU1 = BU2 = bnd, :541-544); here per block
-with the block's boundary matrix (block_kinds).  The drivers pass zero
+with the block's boundary matrix (block_kinds).  The linearised steps calc_L
+and calc_L_1 (:607-986) use the same bnd for every Laplacian (z's included),
+the right part (A_'s elements, or Ab_'s through right_part_backward_cb,
+:341-436; block_kinds_backward) and the four p2u.  The drivers pass zero
 boundary values (examples/test_baroclin.cpp:215).  A_.solve of the reference goes through its
 external solver library (GMRES, parity unpinned); here it is a sparse LU
 factorisation of the 4n x 4n matrix.  SphereLaplace::calc1, SphereJacobian and
@@ -46,6 +49,23 @@ def block_kinds(tau, sigma, mu, sigma1, mu1, alpha, theta):
             (H, 0.0, -1.0)]
 
 
+def block_kinds_backward(tau, sigma, mu, sigma1, mu1, alpha, theta):
+    """integrate_backward_cb's nine element formulas (baroclin.cpp:184-244)
+    as (kind, c_mass, c_lap), in the order of oracle.BAROCLIN_BLOCKS."""
+    H, M = O.KIND_HELMHOLTZ, O.KIND_MASS
+    a2 = alpha * alpha
+    return [(H, 1.0 / tau - 0.5 * (1.0 - theta) * sigma, (1.0 - theta) * mu),
+            (H, (0.5 * sigma) * (1.0 - theta), 0.0),
+            (H, -0.5 * (1.0 - theta) * sigma, 0.0),
+            (H, 1.0 / tau - 0.5 * (1.0 - theta) * sigma - a2 * (1.0 - theta) * mu1,
+             (1.0 - theta) * mu),
+            (H, -a2 / tau + a2 * (1.0 - theta) * sigma1, 0.0),
+            (M, 1.0, 0.0),
+            (H, 0.0, -1.0),
+            (M, 1.0, 0.0),
+            (H, 0.0, -1.0)]
+
+
 class Baroclin:
     def __init__(self, mesh, tau, sigma, mu, sigma1, mu1, alpha, lh):
         self.m = mesh
@@ -63,6 +83,8 @@ class Baroclin:
         self._lu_Ab = None
         self.kinds = block_kinds(tau, sigma, mu, sigma1, mu1, alpha, self.theta)
         self.bnd_blocks = [mesh.assemble_boundary(kd, cm, cl) for kd, cm, cl in self.kinds]
+        self.kinds_b = block_kinds_backward(tau, sigma, mu, sigma1, mu1, alpha, self.theta)
+        self._bnd_blocks_b = None
         # SphereLaplace::calc1, SphereJacobian and SphereNorm of the same mesh
         self.bv = BarVortex(mesh, tau=1.0, sigma=0.0, mu=0.0, k1=0.0, k2=0.0, theta=0.5,
                             lh=np.zeros(len(mesh.p2io)))
@@ -71,12 +93,17 @@ class Baroclin:
     def calc1(self, u, bnd=None):
         return self.bv.calc1(u, bnd)
 
-    def bnd_rhs(self, bnd):
+    def bnd_rhs(self, bnd, backward=False):
         """right_part_cb's boundary branch: the 4n vector of -sum_c
-        (block (r, c) boundary columns) bnd per block row r."""
+        (block (r, c) boundary columns) bnd per block row r; with backward,
+        right_part_backward_cb's (baroclin.cpp:341-436), on Ab_'s elements."""
         rs = self.n
         out = np.zeros(4 * rs)
-        for (r, c), (brp, bci, bv) in zip(O.BAROCLIN_BLOCKS, self.bnd_blocks):
+        if backward and self._bnd_blocks_b is None:
+            self._bnd_blocks_b = [self.m.assemble_boundary(kd, cm, cl)
+                                  for kd, cm, cl in self.kinds_b]
+        blocks = self._bnd_blocks_b if backward else self.bnd_blocks
+        for (r, c), (brp, bci, bv) in zip(O.BAROCLIN_BLOCKS, blocks):
             if len(bci):
                 out[r * rs:(r + 1) * rs] += O.spmv(brp, bci, -bv,
                                                    np.ascontiguousarray(bnd, dtype=np.float64))
@@ -160,7 +187,7 @@ class Baroclin:
         return u1_n, u2_n, it, nr
 
 
-    # ---- linearised steps (examples/baroclin.cpp:607-986), bnd = 0 ----------
+    # ---- linearised steps (examples/baroclin.cpp:607-986) ---------------------
     @property
     def lu_Ab(self):
         """Ab_ (integrate_backward_cb, baroclin.cpp:184-244), factorised."""
@@ -174,22 +201,15 @@ class Baroclin:
             self._lu_Ab = spla.splu(sp.csr_matrix((bv, bci, brp), shape=(4 * n, 4 * n)).tocsc())
         return self._lu_Ab
 
-    def calc_lin(self, which, u1, u2, z1, z2, tol=1e-8):
-        """Baroclin::calc_L (which 0, baroclin.cpp:620-802: one pass with A_)
-        or calc_L_1 (which 1, :804-986: at most 10 passes with Ab_ until the
-        distance is below tol).  The Jacobian arguments are the code's,
-        including J(L z1, avg w2) and J(L z2, avg u1) in G where the comments
-        write J(z1, .) and J(., L z2).  Returns (u11, u21, passes, distance)."""
+    def lin_fc(self, which, u1, u2, bnd=None):
+        """The explicit parts FC, GC of calc_L (which 0, baroclin.cpp:684-700)
+        or calc_L_1 (which 1, :868-884) and the Laplacians w1, w2 they use,
+        with the boundary values bnd (or None for 0)."""
         th, tau = self.theta, self.tau
         sigma, mu, sigma1, mu1, alpha = self.sigma, self.mu, self.sigma1, self.mu1, self.alpha
-        a2 = alpha * alpha
-        inner, rs = self.m.inner, self.n
-        f64 = lambda a: np.asarray(a, dtype=np.float64)  # noqa: E731
-        u1, u2, z1, z2 = f64(u1), f64(u2), f64(z1), f64(z2)
         s1 = lambda a, b, k1, k2: k1 * a + k2 * b  # noqa: E731  vec_sum1
-        w1, w2 = self.calc1(u1), self.calc1(u2)
-        dw1, dw2 = self.calc1(w1), self.calc1(w2)
-        dz1, dz2 = self.calc1(z1), self.calc1(z2)
+        w1, w2 = self.calc1(u1, bnd), self.calc1(u2, bnd)
+        dw1, dw2 = self.calc1(w1, bnd), self.calc1(w2, bnd)
         if which == 0:
             FC = s1(w1, w2, -0.5 * (1.0 - th) * sigma, 0.5 * (1.0 - th) * sigma)
             FC = s1(FC, dw1, 1.0, mu * (1.0 - th))
@@ -200,7 +220,6 @@ class Baroclin:
             GC = s1(GC, u2, 1.0, -alpha * alpha / tau)
             GC = s1(GC, w2, 1.0, -alpha * alpha * mu1 * (1 - th))
             GC = s1(GC, u2, 1.0, alpha * alpha * sigma1 * (1 - th))
-            ka, kb, lu, passes = 1.0 - th, th, self.lu_A, 1
         else:
             FC = s1(w1, w2, 0.5 * th * sigma, -0.5 * th * sigma)
             FC = s1(FC, dw1, 1.0, -mu * th)
@@ -211,7 +230,31 @@ class Baroclin:
             GC = s1(GC, u2, 1.0, -alpha * alpha / tau)
             GC = s1(GC, w2, 1.0, alpha * alpha * mu1 * th)
             GC = s1(GC, u2, 1.0, -alpha * alpha * sigma1 * th)
+        return FC, GC, w1, w2
+
+    def calc_lin(self, which, u1, u2, z1, z2, tol=1e-8, bnd=None, full=False):
+        """Baroclin::calc_L (which 0, baroclin.cpp:620-802: one pass with A_)
+        or calc_L_1 (which 1, :804-986: at most 10 passes with Ab_ until the
+        distance is below tol).  The Jacobian arguments are the code's,
+        including J(L z1, avg w2) and J(L z2, avg u1) in G where the comments
+        write J(z1, .) and J(., L z2).  bnd: the one boundary array the code
+        passes to every calc1 (u, w and z: :675-682, :859-866), to the right
+        part's boundary branch (BW1 = BW2 = BU1 = BU2 = bnd, :708-711 with
+        A_'s elements, :892-895 with Ab_'s) and to the four p2u (:783-786,
+        :967-970); None for 0.  Returns (u11, u21, passes, distance), with
+        full also the last pass's w1_n, w2_n."""
+        th, a2 = self.theta, self.alpha * self.alpha
+        inner, rs = self.m.inner, self.n
+        f64 = lambda a: np.asarray(a, dtype=np.float64)  # noqa: E731
+        u1, u2, z1, z2 = f64(u1), f64(u2), f64(z1), f64(z2)
+        s1 = lambda a, b, k1, k2: k1 * a + k2 * b  # noqa: E731  vec_sum1
+        FC, GC, w1, w2 = self.lin_fc(which, u1, u2, bnd)
+        dz1, dz2 = self.calc1(z1, bnd), self.calc1(z2, bnd)
+        if which == 0:
+            ka, kb, lu, passes = 1.0 - th, th, self.lu_A, 1
+        else:
             ka, kb, lu, passes = th, 1.0 - th, self.lu_Ab, 10
+        brhs = self.bnd_rhs(bnd, backward=which == 1) if bnd is not None else None
         u1_n, u2_n, w1_n, w2_n = u1.copy(), u2.copy(), w1.copy(), w2.copy()
         dz1lh = dz1 + self.lh                                     # vec_sum
         J = self.jac
@@ -242,15 +285,19 @@ class Baroclin:
             rp = np.zeros(4 * rs)
             rp[:rs] = O.right_part_mass(self.m, F)
             rp[rs:2 * rs] = O.right_part_mass(self.m, G)
+            if brhs is not None:
+                rp = rp + brhs
             ans = lu.solve(rp)
-            w1_n = self.bv.p2u(ans[:rs])
-            w2_n = self.bv.p2u(ans[rs:2 * rs])
-            u1_n1 = self.bv.p2u(ans[2 * rs:3 * rs])
-            u2_n1 = self.bv.p2u(ans[3 * rs:])
+            w1_n = self.bv.p2u(ans[:rs], bnd)
+            w2_n = self.bv.p2u(ans[rs:2 * rs], bnd)
+            u1_n1 = self.bv.p2u(ans[2 * rs:3 * rs], bnd)
+            u2_n1 = self.bv.p2u(ans[3 * rs:], bnd)
             nr = max(self.dist(u1_n1, u1_n), self.dist(u2_n1, u2_n))
             u1_n, u2_n = u1_n1, u2_n1
             if nr < tol:
                 break
+        if full:
+            return u1_n, u2_n, it, nr, w1_n, w2_n
         return u1_n, u2_n, it, nr
 
 

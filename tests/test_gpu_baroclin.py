@@ -11,7 +11,8 @@ bit-identical alone and beside another member; the manufactured-solution
 tendency error falls with refinement as the oracle's does; the linearised
 steps calc_L (within 1e-9 of the oracle's) and calc_L_1 (within 1e-8: up to
 10 fixed-point passes, stopping at a distance of 1e-8), member bits
-independent of the batch, and L_1(L(u)) = u without the Jacobian terms."""
+independent of the batch, and L_1(L(u)) = u without the Jacobian terms; the
+same bars for the linearised steps with boundary (pole) values."""
 import numpy as np
 import pytest
 
@@ -238,5 +239,49 @@ def test_baroclin_linear_steps_after_boundary_values():
         B1, B2 = torch.zeros_like(L1), torch.zeros_like(L2)
         bc.calc_L_1(B1, B2, L1, L2, _dev(Z1), _dev(Z2))
         outs.append((L1, L2, B1, B2))
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
+
+
+def test_baroclin_linear_steps_with_boundary_values_vs_oracle():
+    """calc_L and calc_L_1 with boundary (pole) values (baroclin.cpp:620-986:
+    the one bnd array in every Laplacian, the right part of A_ or Ab_ and the
+    p2u), two members with different values, Jacobian terms on, against the
+    oracle; the results hold the values at the boundary points, and a call
+    without values afterwards gives a fresh object's bits."""
+    print("baroclin calc_L / calc_L_1 with boundary values level=3", flush=True)
+    pm, om, xy = _setup(3)
+    U1, lh = _lin_fields(pm, om, (1, 2))
+    U2, _ = _lin_fields(pm, om, (3, 4))
+    Z1, _ = _lin_fields(pm, om, (5, 6))
+    Z2, _ = _lin_fields(pm, om, (7, 8))
+    tau = 2.5e-4
+    bnd = np.array([[0.3], [-0.45]]) * np.ones((2, pm.outer_size))
+    bc = P.Baroclin(pm, _dev(lh), tau=tau, slots=2, **PRM)
+    ob = OB(om, tau=tau, lh=lh, **PRM)
+    L1, L2 = torch.zeros_like(_dev(U1)), torch.zeros_like(_dev(U2))
+    for i in bc.calc_L(L1, L2, _dev(U1), _dev(U2), _dev(Z1), _dev(Z2), bnd=_dev(bnd)):
+        assert i["converged"] and i["iterations"] == 1, i
+    B1, B2 = torch.zeros_like(L1), torch.zeros_like(L2)
+    inf_b = bc.calc_L_1(B1, B2, L1, L2, _dev(Z1), _dev(Z2), bnd=_dev(bnd))
+    g = [t.cpu().numpy() for t in (L1, L2, B1, B2)]
+    for q in range(2):
+        r1, r2, _, _ = ob.calc_lin(0, U1[q], U2[q], Z1[q], Z2[q], bnd=bnd[q])
+        assert _rel2(ob, g[0][q], g[1][q], r1, r2) <= 1e-9, q
+        p1, p2, _, _ = ob.calc_lin(0, U1[q], U2[q], Z1[q], Z2[q])
+        assert _rel2(ob, r1, r2, p1, p2) > 1e-4, q                  # the values matter
+        r1, r2, it, nr = ob.calc_lin(1, g[0][q], g[1][q], Z1[q], Z2[q], bnd=bnd[q])
+        assert inf_b[q]["converged"] and abs(inf_b[q]["iterations"] - it) <= 1, (inf_b[q], it)
+        assert _rel2(ob, g[2][q], g[3][q], r1, r2) <= 1e-8, q
+        for t in g:
+            assert np.all(t[q][om.outer] == bnd[q]), q
+    fresh = P.Baroclin(pm, _dev(lh), tau=tau, slots=2, **PRM)
+    outs = []
+    for b in (bc, fresh):
+        a1, a2 = torch.zeros_like(L1), torch.zeros_like(L2)
+        b.calc_L(a1, a2, _dev(U1), _dev(U2), _dev(Z1), _dev(Z2))
+        c1, c2 = torch.zeros_like(L1), torch.zeros_like(L2)
+        b.calc_L_1(c1, c2, a1, a2, _dev(Z1), _dev(Z2))
+        outs.append((a1, a2, c1, c2))
     for a, b in zip(*outs):
         assert torch.equal(a, b)

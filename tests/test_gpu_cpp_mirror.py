@@ -205,9 +205,26 @@ def test_baroclin_equals_python(run):
     assert np.array_equal(load("bc_LT1.bin"), load("bc_L1.bin"))
 
 
-def test_baroclin_linear_steps_refuse_boundary_values(run):
-    summ, _ = run
-    assert summ["bc_bnd_throws"] == 1
+def test_baroclin_linear_steps_with_boundary_values_equal_python(run):
+    """calc_L of (u, v) about (psi, u) and calc_L_1 of that with a pole value
+    of 0.5 through the C++ class and through the Python class: same bits, the
+    pole value in the results."""
+    _, load = run
+    m = P.Mesh.sphere(LEVEL)
+    bc = P.Baroclin(m, torch.as_tensor(load("bc_lh.bin"), device=DEV), tau=1e-3, sigma=1.6e-2,
+                    mu=8e-2, sigma1=1.6e-2, mu1=8e-2, alpha=1.0)
+    u1 = torch.as_tensor(load("u.bin"), device=DEV).reshape(1, -1)
+    u2 = torch.as_tensor(load("v.bin"), device=DEV).reshape(1, -1)
+    psi = torch.as_tensor(load("psi.bin"), device=DEV).reshape(1, -1)
+    bnd = torch.full((1, m.outer_size), 0.5, dtype=torch.float64, device=DEV)
+    l1, l2, k1, k2 = (torch.zeros_like(u1) for _ in range(4))
+    bc.calc_L(l1, l2, u1, u2, psi, u1, bnd=bnd)
+    bc.calc_L_1(k1, k2, l1, l2, psi, u1, bnd=bnd)
+    outer = m.arrays()["outer"]
+    for t, name in ((l1, "bc_Lb1.bin"), (l2, "bc_Lb2.bin"), (k1, "bc_Bb1.bin"),
+                    (k2, "bc_Bb2.bin")):
+        assert np.array_equal(t.cpu().numpy()[0], load(name)), name
+        assert np.all(load(name)[outer] == 0.5), name
 
 
 def test_sphere_chafe_cpp_vs_oracle(run):

@@ -242,3 +242,99 @@ def test_step_takes_the_boundary_values():
     c = bc.calc(u1, u2, max_iters=5, bnd=np.full(len(M.outer), 0.5))
     assert np.all(c[0][M.outer] == 0.5) and np.all(c[1][M.outer] == 0.5)
     assert bc.dist(c[0], a[0]) > 1e-6 * bc.dist(a[0], 0 * a[0])
+
+
+def _no_jacobian(bc):
+    bc.jac = lambda u, v: np.zeros(bc.n)
+    return bc
+
+
+def _unreduced_residual(bc, kinds, X, F, G):
+    """Inner rows of the FULL block system (generate_full_matrix per block
+    element formula) applied to the four full fields X, minus the right part
+    [M F; M G; 0; 0]; relative to the largest block product."""
+    M, inner, n = bc.m, bc.m.inner, bc.n
+    res = np.zeros(4 * n)
+    res[:n] -= O.right_part_mass(M, F)
+    res[n:2 * n] -= O.right_part_mass(M, G)
+    scale = 0.0
+    for (r, c), (kd, cm, cl) in zip(O.BAROCLIN_BLOCKS, kinds):
+        frp, fci, fv = M.assemble_full(kd, cm, cl)
+        t = O.spmv(frp, fci, fv, X[c])[inner]
+        res[r * n:(r + 1) * n] += t
+        scale = max(scale, np.abs(t).max())
+    return np.abs(res).max() / scale
+
+
+def test_linear_steps_take_the_boundary_values():
+    """Boundary (pole) values in calc_L / calc_L_1 (baroclin.cpp:620-986):
+    * with every Jacobian term removed, calc_L is the nonlinear step's linear
+      part, so with the same bnd it equals Baroclin::calc (whose boundary
+      handling the two tests above pin); dropping bnd moves it far more;
+    * with every Jacobian term removed, either step's answer, completed with
+      bnd at the boundary points, solves the inner rows of the UNREDUCED
+      system of its own element formulas (A_'s for calc_L, Ab_'s for
+      calc_L_1; generate_full_matrix, another assembly route); at sigma,
+      mu != 0 the two sets differ, so the other step's formulas, a dropped
+      boundary term or a wrong sign fail it;
+    * the results hold bnd at the boundary points, zero values give the plain
+      steps bit for bit and non-zero ones change them.
+    Where bnd enters L(z) (calc1(z, bnd), :681-682, :865-866: only the
+    Jacobian terms see it) follows the code: pinned by reading only."""
+    M, field, lh = _lin_fields(3)
+    u1, u2, z1, z2 = field(1), field(2), field(3), field(4)
+    bnd = np.full(len(M.outer), 0.37)
+    bc = _no_jacobian(Baroclin(M, tau=1e-2, lh=lh, **PRM))
+    L = bc.calc_lin(0, u1, u2, z1, z2, bnd=bnd)
+    S = bc.calc(u1, u2, max_iters=3, bnd=bnd)
+    assert _rel(bc, L[:2], S[:2], S[:2]) <= 1e-13
+    assert _rel(bc, bc.calc_lin(0, u1, u2, z1, z2)[:2], S[:2], S[:2]) > 1e-4
+    for which, kinds, other in ((0, bc.kinds, bc.kinds_b), (1, bc.kinds_b, bc.kinds)):
+        r = bc.calc_lin(which, u1, u2, z1, z2, bnd=bnd, full=True)
+        X = [r[4], r[5], r[0], r[1]]
+        for x in X:
+            assert np.all(x[M.outer] == 0.37)
+        FC, GC = bc.lin_fc(which, u1, u2, bnd)[:2]
+        F, G = FC[M.inner], GC[M.inner]
+        assert _unreduced_residual(bc, kinds, X, F, G) <= 1e-12, which
+        assert _unreduced_residual(bc, other, X, F, G) > 1e-6, which
+        Xz = [x.copy() for x in X]
+        for x in Xz:
+            x[M.outer] = 0.0                           # the answer without its boundary values
+        assert _unreduced_residual(bc, kinds, Xz, F, G) > 1e-6, which
+    # zero values: the plain steps' bits; non-zero values: other results
+    full = Baroclin(M, tau=1e-2, lh=lh, **PRM)
+    zero = np.zeros(len(M.outer))
+    for which in (0, 1):
+        a = full.calc_lin(which, u1, u2, z1, z2)
+        b = full.calc_lin(which, u1, u2, z1, z2, bnd=zero)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        c = full.calc_lin(which, u1, u2, z1, z2, bnd=bnd)
+        assert np.all(c[0][M.outer] == 0.37) and np.all(c[1][M.outer] == 0.37)
+        assert _rel(full, c[:2], a[:2], a[:2]) > 1e-4
+
+
+def test_backward_boundary_values_are_a_dirichlet_elimination():
+    """right_part_backward_cb's boundary branch (baroclin.cpp:341-436) is the
+    elimination of known boundary values for Ab_: for four full fields whose
+    boundary values all equal bnd, Ab_ times their inner values equals the
+    inner rows of the FULL products of integrate_backward_cb's formulas plus
+    bnd_rhs(bnd, backward=True)."""
+    M, _ = _mesh(3)
+    bc = Baroclin(M, TAU, lh=np.zeros(len(M.p2io)), **PRM)
+    bc.lu_Ab                                            # builds Ab_
+    rng = np.random.default_rng(6)
+    V, inner, outer, n = len(M.p2io), M.inner, M.outer, M.n_inner
+    bnd = rng.standard_normal(len(outer))
+    X = rng.standard_normal((4, V))
+    X[:, outer] = bnd
+    rhs = np.zeros(4 * n)
+    for (r, c), (kd, cm, cl) in zip(O.BAROCLIN_BLOCKS, bc.kinds_b):
+        frp, fci, fv = M.assemble_full(kd, cm, cl)
+        rhs[r * n:(r + 1) * n] += O.spmv(frp, fci, fv, X[c])[inner]
+    brp, bci, bv = bc.Ab
+    lhs = O.spmv(brp, bci, bv, np.concatenate([X[c][inner] for c in range(4)]))
+    err = np.abs(lhs - (rhs + bc.bnd_rhs(bnd, backward=True))).max()
+    assert err <= 1e-10 * np.abs(rhs).max(), err
+    err_fwd = np.abs(lhs - (rhs + bc.bnd_rhs(bnd))).max()
+    assert err_fwd > 1e-6 * np.abs(rhs).max(), err_fwd
