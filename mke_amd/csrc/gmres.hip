@@ -86,11 +86,55 @@ void destroy_gmres_work(GmresWork* w) { delete w; }
 namespace {
 
 // ---------------------------------------------------------------- helpers --
+// A double moved between lanes by a DPP pattern (both 32-bit halves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+// v of the even 16-lane row (ROWS 16) or lower half (32) plus v of the odd
+// row or upper half, in every lane of the pair (v_permlane16/32_swap).
+template <int ROWS>
+__device__ __forceinline__ double swap_add(double v)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned l = (unsigned)u, h = (unsigned)(u >> 32);
+    const auto lo = ROWS == 16 ? __builtin_amdgcn_permlane16_swap(l, l, false, false)
+                               : __builtin_amdgcn_permlane32_swap(l, l, false, false);
+    const auto hi = ROWS == 16 ? __builtin_amdgcn_permlane16_swap(h, h, false, false)
+                               : __builtin_amdgcn_permlane32_swap(h, h, false, false);
+    const double a = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+    const double b = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+    return a + b;
+}
+
+// Sum over the 64 lanes of a wavefront, all active: a butterfly of VALU
+// lane moves (no LDS round trip) in which each step adds a lane and its
+// partner (lane ^ 1, lane ^ 2, the mirror lane within 8 and within 16, then
+// the two rows of a 32-lane half and the two halves), so every lane ends
+// with the same, order-fixed bits.
 __device__ __forceinline__ double wave_sum(double v)
 {
-    // xor butterfly: every lane ends with the same, order-fixed sum
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_d<0x141>(v);  // row_half_mirror
+    v += dpp_d<0x140>(v);  // row_mirror
+    v = swap_add<16>(v);
+    return swap_add<32>(v);
+}
+
+// A lane's partials b = lane + 64 k (+0.0 past G), added in k order over the
+// groups that hold any of the G partials.
+__device__ __forceinline__ double lane_partials(const double (&v)[GM_MAXG / 64], int G)
+{
+    double s = v[0];
+#pragma unroll
+    for (int k = 1; k < GM_MAXG / 64; ++k)
+        if (64 * k < G) s += v[k];
+    return s;
 }
 
 // Sum of a block's per-thread values, identical in every thread.
@@ -119,10 +163,7 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ p, int
         const int b = lane + 64 * k;
         v[k] = (b < G) ? p[b] : 0.0;
     }
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < GM_MAXG / 64; ++k) s += v[k];
-    return wave_sum(s);
+    return wave_sum(lane_partials(v, G));
 }
 
 template <typename T>
@@ -846,10 +887,7 @@ __device__ bool xch_sum(const xch_t* row, int G, int* abort_flag, long spin, dou
             v[k] = __longlong_as_double((long long)u);
         }
         if (__all(all)) {
-            double s = 0.0;
-#pragma unroll
-            for (int k = 0; k < GM_MAXG / 64; ++k) s += v[k];
-            *out = wave_sum(s);
+            *out = wave_sum(lane_partials(v, G));
             return true;
         }
         if ((it & 63) == 63 && ld_agent(abort_flag)) return false;
