@@ -491,13 +491,15 @@ __device__ GivensOut givens_step(GmresState* __restrict__ st, int j,
     const double sreg = lane < j ? st->sn[lane] : 0.0;
     const double hn = sqrt(s);
     double cur = readlane_d(hreg, 0);
+    double tv = 0.0;  // lane i keeps R(i, j): one store per lane after the loop
     for (int i = 0; i < j; ++i) {
         const double c = readlane_d(creg, i), sn = readlane_d(sreg, i);
         const double nxt = readlane_d(hreg, i + 1);
         const double t = c * cur + sn * nxt;
         cur = -sn * cur + c * nxt;
-        if (write && lane == 0) st->R[i * m + j] = t;
+        tv = lane == i ? t : tv;
     }
+    if (write && lane < j) st->R[lane * m + j] = tv;
     const double r = hypot(cur, hn);
     const double cj = cur / r, sj = hn / r;
     const double carry = st->carry[j];
@@ -945,15 +947,17 @@ __device__ GivensOut givens_lds(int j, double s, int m, double stop, int iters0,
     const double sreg = lane < j ? s_sn[lane] : 0.0;
     const double hn = sqrt(s);
     double cur = readlane_d(hreg, 0);
+    double tv = 0.0;  // lane i keeps R(i, j): one store per lane after the loop
     for (int i = 0; i < j; ++i) {
         const double c = readlane_d(creg, i), sn = readlane_d(sreg, i);
         const double nxt = readlane_d(hreg, i + 1);
         const double t = c * cur + sn * nxt;
         cur = -sn * cur + c * nxt;
-        if (lane == 0) {
-            s_R[i * m + j] = t;
-            if (rec) st->R[i * m + j] = t;
-        }
+        tv = lane == i ? t : tv;
+    }
+    if (lane < j) {
+        s_R[lane * m + j] = tv;
+        if (rec) st->R[lane * m + j] = tv;
     }
     const double r = hypot(cur, hn);
     const double cj = cur / r, sj = hn / r;
