@@ -860,8 +860,10 @@ __global__ void k_fill0(int n, double* __restrict__ x)
 //
 // Each workgroup keeps the cycle's Givens state (current Hessenberg column,
 // rotations, rotated right-hand side, R) in LDS and computes it from the
-// same exchanged sums as every other workgroup; workgroup 0 also records it
-// in GmresState.  The arithmetic is that of k_step_spmv, k_mgs, k_solve_y,
+// same exchanged sums as every other workgroup; at the cycle's end workgroup
+// 0 records y, the step count and the next cycle's start in GmresState (no
+// global stores during the steps: a workgroup's wait for its loads would
+// also wait for them, and every exchange waits for the slowest workgroup).  The arithmetic is that of k_step_spmv, k_mgs, k_solve_y,
 // k_xupdate, k_residual and k_cycle_start, in the same order over the same
 // block partials, so the solve gives the same bits as the kernel-per-pass
 // form.
@@ -934,12 +936,12 @@ __device__ __forceinline__ bool xch_block(xch_t* row, int G, double acc, bool pu
     return l->ok;
 }
 
-// givens_step on the LDS copy of the cycle's state: same arithmetic; lane 0
-// stores, and in workgroup 0 also records into GmresState.
+// givens_step on the LDS copy of the cycle's state: same arithmetic, stored
+// in LDS only (a global store here would hold up this workgroup's next wait
+// for its loads, since the load counter also counts stores).
 __device__ GivensOut givens_lds(int j, double s, int m, double stop, int iters0, int max_iters,
                                 const double* sH, double* s_cs, double* s_sn, double* s_g,
-                                double* s_carry, double* s_R, GmresState* __restrict__ st,
-                                bool rec)
+                                double* s_carry, double* s_R)
 {
     const int lane = threadIdx.x & 63;
     const double hreg = lane <= j ? sH[lane] : 0.0;
@@ -955,10 +957,7 @@ __device__ GivensOut givens_lds(int j, double s, int m, double stop, int iters0,
         cur = -sn * cur + c * nxt;
         tv = lane == i ? t : tv;
     }
-    if (lane < j) {
-        s_R[lane * m + j] = tv;
-        if (rec) st->R[lane * m + j] = tv;
-    }
+    if (lane < j) s_R[lane * m + j] = tv;
     const double r = hypot(cur, hn);
     const double cj = cur / r, sj = hn / r;
     const double carry = s_carry[j];
@@ -971,17 +970,6 @@ __device__ GivensOut givens_lds(int j, double s, int m, double stop, int iters0,
         s_sn[j] = sj;
         s_g[j] = gj;
         s_carry[j + 1] = cn;
-        if (rec) {
-            st->R[j * m + j] = r;
-            st->H[(j + 1) * m + j] = hn;
-            st->cs[j] = cj;
-            st->sn[j] = sj;
-            st->g[j] = gj;
-            st->carry[j + 1] = cn;
-            st->k = j + 1;
-            st->iters = iters;
-            if (!cont) st->active = 0;
-        }
     }
     return GivensOut{cont, hn};
 }
@@ -1021,6 +1009,20 @@ k_mgs_cycle(int n, int rpt, int width, const int* __restrict__ col, const MT* __
         VT* Vj = V + (size_t)j * ldv;
         VT w[GM_RPT], vj[GM_RPT], xo[GM_RPT];
         arnoldi_rows<VT, MT, true>(n, rpt, width, base, col, val, Wsrc, inv, xo, w);
+        // pv = v_{i-1}, iv = v_i of the coming pass i (own rows; v_j from
+        // registers); pass i + 1 reuses iv as its pv, so each pass loads one
+        // basis vector, issued before the exchange it does not depend on.
+        // v_0 and v_1 are loaded before this step's stores of v_j: the load
+        // counter also counts stores, so waiting for loads issued after the
+        // stores would wait for the stores too.
+        VT pv[GM_RPT], iv[GM_RPT];
+#pragma unroll
+        for (int r = 0; r < GM_RPT; ++r) {
+            const int rr = base + r * GM_BS;
+            const bool in = r < rpt && rr < n;
+            pv[r] = (in && j >= 1) ? V[rr] : VT(0);
+            iv[r] = (in && j >= 2) ? V[ldv + rr] : VT(0);
+        }
 #pragma unroll
         for (int r = 0; r < GM_RPT; ++r) {
             const int rr = base + r * GM_BS;
@@ -1029,19 +1031,10 @@ k_mgs_cycle(int n, int rpt, int width, const int* __restrict__ col, const MT* __
                 vj[r] = xo[r] * inv;
                 Vj[rr] = vj[r];
             }
+            if (j == 0) pv[r] = vj[r];
+            if (j == 1) iv[r] = vj[r];
         }
-        // pv = v_{i-1}, iv = v_i of the coming pass i (own rows; v_j from
-        // registers); pass i + 1 reuses iv as its pv, so each pass loads one
-        // basis vector, issued before the exchange it does not depend on
-        VT pv[GM_RPT], iv[GM_RPT];
         double acc = 0.0;
-#pragma unroll
-        for (int r = 0; r < GM_RPT; ++r) {
-            const int rr = base + r * GM_BS;
-            const bool in = r < rpt && rr < n;
-            pv[r] = in ? (j == 0 ? vj[r] : V[rr]) : VT(0);
-            iv[r] = (in && j >= 1) ? (j == 1 ? vj[r] : V[ldv + rr]) : VT(0);
-        }
         // partial of <w, v_0> (block_dots with one dot)
 #pragma unroll
         for (int r = 0; r < GM_RPT; ++r) {
@@ -1052,10 +1045,7 @@ k_mgs_cycle(int n, int rpt, int width, const int* __restrict__ col, const MT* __
         ok = xch_block(row, G, acc, false, abort_flag, spin, &hsum, sh, &xl);
         row += G;
         if (!ok) break;
-        if (threadIdx.x == 0) {
-            sH[0] = hsum;
-            if (rec) st->H[j] = hsum;
-        }
+        if (threadIdx.x == 0) sH[0] = hsum;
         // MGS passes i = 1..j+1: w -= h_{i-1} v_{i-1}, then <w, v_i> or ||w||^2
         bool cont = false;
         for (int i = 1; i <= j + 1 && ok; ++i) {
@@ -1087,10 +1077,7 @@ k_mgs_cycle(int n, int rpt, int width, const int* __restrict__ col, const MT* __
             if (!ok) break;
             if (!last) {
                 hsum = s;
-                if (threadIdx.x == 0) {
-                    sH[i] = s;
-                    if (rec) st->H[i * m + j] = s;
-                }
+                if (threadIdx.x == 0) sH[i] = s;
 #pragma unroll
                 for (int r = 0; r < GM_RPT; ++r) {
                     pv[r] = iv[r];
@@ -1099,7 +1086,7 @@ k_mgs_cycle(int n, int rpt, int width, const int* __restrict__ col, const MT* __
             } else {
                 if (threadIdx.x < 64) {
                     const GivensOut g = givens_lds(j, s, m, stop, iters0, max_iters, sH, s_cs, s_sn,
-                                                   s_g, s_carry, s_R, st, rec);
+                                                   s_g, s_carry, s_R);
                     if (threadIdx.x == 0) {
                         s_hn = g.hn;
                         s_cont = g.cont;
@@ -1124,6 +1111,8 @@ k_mgs_cycle(int n, int rpt, int width, const int* __restrict__ col, const MT* __
         }
         if (rec) {
             for (int i = 0; i < k; ++i) st->y[i] = s_y[i];
+            st->k = k;
+            st->iters = iters0 + k;
             st->active = 0;
         }
     }
