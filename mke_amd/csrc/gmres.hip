@@ -402,6 +402,64 @@ __device__ __forceinline__ void arnoldi_rows(int n, int rpt, int width, int base
     }
 }
 
+// A thread's row slots (ELL columns and values of its rows; past a row's
+// width the last slot again, and rows past n clamped to row n - 1, so every
+// load is in bounds), loaded once per launch of the persistent cycle: the
+// matrix does not change within a cycle, so each step's SpMV is one round
+// trip (the gathers) instead of two.
+template <typename MT>
+struct RowSlots {
+    int c[GM_RPT][ELL_MAXW];
+    MT v[GM_RPT][ELL_MAXW];
+};
+template <typename MT>
+__device__ __forceinline__ void load_slots(int n, int width, int base, const int* __restrict__ col,
+                                           const MT* __restrict__ val, RowSlots<MT>& s)
+{
+#pragma unroll
+    for (int r = 0; r < GM_RPT; ++r) {
+        const int rr = min(base + r * GM_BS, n - 1);
+#pragma unroll
+        for (int k = 0; k < ELL_MAXW; ++k) {
+            const size_t o = ell_slot(k, width, n, rr);
+            s.c[r][k] = col[o];
+            s.v[r][k] = val[o];
+        }
+    }
+}
+// arnoldi_rows (agent-scope gathers, rows of 1..ELL_MAXW entries) on
+// preloaded slots: every row's own entry and gathers issued together, then
+// the sums in RowDot's order (the same bits).
+template <typename VT, typename MT>
+__device__ __forceinline__ void arnoldi_rows_slots(int n, int rpt, int width, int base,
+                                                   const RowSlots<MT>& s, const VT* x, VT inv,
+                                                   VT (&xo)[GM_RPT], VT (&w)[GM_RPT])
+{
+    VT xv[GM_RPT][ELL_MAXW];
+#pragma unroll
+    for (int r = 0; r < GM_RPT; ++r) {
+        if (r < rpt) {
+            xo[r] = ld_agent(x + min(base + r * GM_BS, n - 1));
+#pragma unroll
+            for (int k = 0; k < ELL_MAXW; ++k) xv[r][k] = ld_agent(x + s.c[r][k]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < GM_RPT; ++r) {
+        VT sum = VT(0);
+        if (r < rpt && base + r * GM_BS < n) {
+#pragma unroll
+            for (int k = 0; k < ELL_MAXW; ++k) {
+                const VT t = RowTerm<VT, MT>::add(sum, s.v[r][k], xv[r][k], inv);
+                sum = k < width ? t : sum;
+            }
+        } else {
+            xo[r] = VT(0);
+        }
+        w[r] = sum;
+    }
+}
+
 constexpr int GM_LB = 8;  // basis vectors loaded per batch
 
 // Block partials of the dots <wv, v_i>, i < nd, over this thread's rows
@@ -1002,13 +1060,17 @@ k_mgs_cycle(int n, int rpt, int width, const int* __restrict__ col, const MT* __
     double h = st->beta;
     int k = 0;
     bool ok = true;
+    const bool pre = width >= 1 && width <= ELL_MAXW;
+    RowSlots<MT> slots;
+    if (pre) load_slots<MT>(n, width, base, col, val, slots);
     for (int j = 0; j < m; ++j) {
         VT* Wsrc = (j & 1) ? W1 : W0;
         VT* Wdst = (j & 1) ? W0 : W1;
         const VT inv = (VT)(1.0 / h);
         VT* Vj = V + (size_t)j * ldv;
         VT w[GM_RPT], vj[GM_RPT], xo[GM_RPT];
-        arnoldi_rows<VT, MT, true>(n, rpt, width, base, col, val, Wsrc, inv, xo, w);
+        if (pre) arnoldi_rows_slots<VT, MT>(n, rpt, width, base, slots, Wsrc, inv, xo, w);
+        else arnoldi_rows<VT, MT, true>(n, rpt, width, base, col, val, Wsrc, inv, xo, w);
         // pv = v_{i-1}, iv = v_i of the coming pass i (own rows; v_j from
         // registers); pass i + 1 reuses iv as its pv, so each pass loads one
         // basis vector, issued before the exchange it does not depend on.
