@@ -1063,6 +1063,7 @@ k_mgs_cycle(int n, int rpt, int width, const int* __restrict__ col, const MT* __
     const bool pre = width >= 1 && width <= ELL_MAXW;
     RowSlots<MT> slots;
     if (pre) load_slots<MT>(n, width, base, col, val, slots);
+    VT v0r[GM_RPT], v1r[GM_RPT];  // this thread's rows of v_0 and v_1, kept for the cycle
     for (int j = 0; j < m; ++j) {
         VT* Wsrc = (j & 1) ? W1 : W0;
         VT* Wdst = (j & 1) ? W0 : W1;
@@ -1071,20 +1072,12 @@ k_mgs_cycle(int n, int rpt, int width, const int* __restrict__ col, const MT* __
         VT w[GM_RPT], vj[GM_RPT], xo[GM_RPT];
         if (pre) arnoldi_rows_slots<VT, MT>(n, rpt, width, base, slots, Wsrc, inv, xo, w);
         else arnoldi_rows<VT, MT, true>(n, rpt, width, base, col, val, Wsrc, inv, xo, w);
-        // pv = v_{i-1}, iv = v_i of the coming pass i (own rows; v_j from
-        // registers); pass i + 1 reuses iv as its pv, so each pass loads one
-        // basis vector, issued before the exchange it does not depend on.
-        // v_0 and v_1 are loaded before this step's stores of v_j: the load
-        // counter also counts stores, so waiting for loads issued after the
-        // stores would wait for the stores too.
+        // pv = v_{i-1}, iv = v_i of the coming pass i (own rows; v_0 and v_1
+        // from registers, kept since steps 0 and 1, so the first two passes
+        // wait for no load); pass i + 1 reuses iv as its pv, so each later
+        // pass loads one basis vector, issued before the exchange it does not
+        // depend on
         VT pv[GM_RPT], iv[GM_RPT];
-#pragma unroll
-        for (int r = 0; r < GM_RPT; ++r) {
-            const int rr = base + r * GM_BS;
-            const bool in = r < rpt && rr < n;
-            pv[r] = (in && j >= 1) ? V[rr] : VT(0);
-            iv[r] = (in && j >= 2) ? V[ldv + rr] : VT(0);
-        }
 #pragma unroll
         for (int r = 0; r < GM_RPT; ++r) {
             const int rr = base + r * GM_BS;
@@ -1093,8 +1086,10 @@ k_mgs_cycle(int n, int rpt, int width, const int* __restrict__ col, const MT* __
                 vj[r] = xo[r] * inv;
                 Vj[rr] = vj[r];
             }
-            if (j == 0) pv[r] = vj[r];
-            if (j == 1) iv[r] = vj[r];
+            if (j == 0) v0r[r] = vj[r];
+            if (j == 1) v1r[r] = vj[r];
+            pv[r] = v0r[r];
+            iv[r] = j >= 1 ? v1r[r] : VT(0);
         }
         double acc = 0.0;
         // partial of <w, v_0> (block_dots with one dot)
