@@ -268,7 +268,9 @@ int phg_gmres_batch(phg_matrix_t A, int nrhs, const double *d_B, double *d_X,
  * (n_rows, nrhs rounded up to a power of two, precision) shape and fixes the
  * grouping of every dot product's block partials, so results are bitwise
  * reproducible for a fixed value.  Untuned shapes take the largest value
- * (at most 8) that still gives >= 1024 workgroups.  phg_gmres_batch_tune times one
+ * (at most 8) that still gives >= 1024 workgroups; preconditioned solves
+ * (phg_gmres_batch_mg) always take that default, whatever is tuned or pinned
+ * for the plain form.  phg_gmres_batch_tune times one
  * restart cycle (opts->restart steps, opts->ortho and opts->precision; NULL:
  * defaults with CGS2) of nrhs hash-valued right-hand sides on A for each value,
  * keeps the fastest for the shape and returns it in *rows_per_thread (may be

@@ -1179,8 +1179,12 @@ int gmres_batch_solve_pc(phg_matrix_s* A, int K, const double* B, double* X,
     }
     BatchWork* w = nullptr;
     const int slot = o.precision == PHG_F64 ? 0 : 1;
-    rc = prepare_batch(A, slot, o.precision == PHG_F64 ? PHG_F64 : PHG_F32, o.restart, kp,
-                       shape_rpt(A->n_rows, kp, o.precision), pc != nullptr, &w);
+    // the autotuner times the plain form, so a preconditioned solve keeps the
+    // untuned default shape: its speed and bits do not depend on which
+    // tunings ran earlier in the process
+    const int rpt = pc ? default_rpt(A->n_rows, kp) : shape_rpt(A->n_rows, kp, o.precision);
+    rc = prepare_batch(A, slot, o.precision == PHG_F64 ? PHG_F64 : PHG_F32, o.restart, kp, rpt,
+                       pc != nullptr, &w);
     if (rc) return rc;
     if (o.precision == PHG_F64) {
         if (o.diag_scale && !pc) {  // right preconditioning by D^-1: solve (A D^-1) y = b, x = D^-1 y
