@@ -673,7 +673,6 @@ k_step_spmv(int n, int rpt, int width, const int* __restrict__ col, const MT* __
             int j, int ndots, double* __restrict__ part, const double* nrm, int G,
             GmresState* __restrict__ st)
 {
-    __shared__ double sh[GM_BS / 64];
     __shared__ double sh_h;
     __shared__ int sh_go;
     if (!st->active) return;
@@ -767,7 +766,6 @@ k_cgs_update(int n, int rpt, VT* __restrict__ w, const VT* __restrict__ V, size_
              int final_pass, const double* __restrict__ pin, double* __restrict__ pout, int G,
              GmresState* __restrict__ st)
 {
-    __shared__ double sh[GM_BS / 64];
     __shared__ double hs[GM_MAXM + 1];
     __shared__ double hw[GM_BS / 64][GM_MAXM + 1];
     if (!st->active) return;
