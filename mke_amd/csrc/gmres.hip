@@ -1267,7 +1267,7 @@ int prepare_work(phg_matrix_s* A, int prec, int m, int ortho, GmresWork** out)
         w->m = m;
         w->prec = prec;
         // target block count.  MGS: the fewest blocks that hold the rows at
-        // GM_RPT rows per thread (160 at level 7): each of an Arnoldi step's
+        // GM_RPT rows per thread (161 at level 7): each of an Arnoldi step's
         // j + 2 dependent sums then gathers fewer partials, and the
         // persistent cycle ran 35.3 instead of 33.3 solves/s at level 7
         // (the kernel-per-pass form, which shares the shape so that both
