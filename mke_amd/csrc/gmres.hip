@@ -31,6 +31,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <vector>
 
 #include "ell.h"
 #include "phg_internal.h"
@@ -56,13 +58,82 @@ struct GmresState {
     int abort;                          // persistent cycle: an exchange timed out
 };
 
+// The persistent cycle's exchange rows live in uncached device memory
+// (hipDeviceMallocUncached): an agent-scope store of a partial and the polls
+// that see it cost 2.3 instead of 2.8 us per sum across 161 workgroups in
+// coarse-grained memory, whose accesses go through each XCD's L2.  Uncached
+// memory that a kernel has read is never freed while the process runs (an
+// allocation made later at its address may read stale data): a workspace
+// hands its rows back to this pool when it is destroyed, after its last
+// solve has synchronised, and the next workspace on the device takes them.
+struct XchChunk {
+    unsigned long long* p = nullptr;
+    size_t n = 0;
+    int dev = -1;
+};
+inline std::mutex& xch_pool_mutex()
+{
+    static std::mutex* m = new std::mutex();  // kept until exit, like the pool
+    return *m;
+}
+inline std::vector<XchChunk>& xch_pool()
+{
+    static std::vector<XchChunk>* v = new std::vector<XchChunk>();  // kept until exit
+    return *v;
+}
+struct XchRows {
+    XchChunk c;
+    XchRows() = default;
+    XchRows(const XchRows&) = delete;
+    XchRows& operator=(const XchRows&) = delete;
+    ~XchRows() { give_back(); }
+    unsigned long long* p() const { return c.p; }
+    void give_back()
+    {
+        if (!c.p) return;
+        std::lock_guard<std::mutex> g(xch_pool_mutex());
+        xch_pool().push_back(c);
+        c = XchChunk();
+    }
+    // at least count slots on the current device: the smallest pooled chunk
+    // that holds them, else a new allocation
+    hipError_t alloc(size_t count)
+    {
+        give_back();
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) return e;
+        {
+            std::lock_guard<std::mutex> g(xch_pool_mutex());
+            std::vector<XchChunk>& pool = xch_pool();
+            size_t best = pool.size();
+            for (size_t i = 0; i < pool.size(); ++i)
+                if (pool[i].dev == dev && pool[i].n >= count &&
+                    (best == pool.size() || pool[i].n < pool[best].n))
+                    best = i;
+            if (best < pool.size()) {
+                c = pool[best];
+                pool.erase(pool.begin() + (std::ptrdiff_t)best);
+                return hipSuccess;
+            }
+        }
+        void* q = nullptr;
+        e = hipExtMallocWithFlags(&q, count * sizeof(unsigned long long), hipDeviceMallocUncached);
+        if (e != hipSuccess) return e;
+        c.p = (unsigned long long*)q;
+        c.n = count;
+        c.dev = dev;
+        return hipSuccess;
+    }
+};
+
 struct GmresWork {
     int n = 0, m = 0, G = 0, rpt = 1, prec = 0;
     size_t ldv = 0;
     DevBuf<char> V, W;     // (m+1) x ldv and 2 x ldv vectors of the Krylov type
     DevBuf<double> part;   // 2 x (m+2) x G partial sums (two regions for CGS2)
     DevBuf<GmresState> st;
-    DevBuf<unsigned long long> xch;  // persistent cycle: exchange rows x G slots
+    XchRows xch;           // persistent cycle: exchange rows x G slots
     int xch_rows = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};  // around each persistent cycle launch
     GmresState* host = nullptr;  // pinned mirror
@@ -1434,7 +1505,7 @@ int gmres_run(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double* b, 
         if (coop) {
             const size_t cnt = (size_t)xch_rows_for(m) * G;
             hipLaunchKernelGGL(k_xch_reset, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
-                               w->xch.p, cnt, st);
+                               w->xch.p(), cnt, st);
             PHG_HIP(hipGetLastError());
             int n_ = n, rpt_ = rpt, width = E.width, width64 = A->ell64.width, G_ = G;
             const int* col = E.col.p;
@@ -1444,7 +1515,7 @@ int gmres_run(phg_matrix_s* A, const Ell<MT>& E, GmresWork* w, const double* b, 
             VT* V = (VT*)w->V.p;
             VT* W1 = W0 + w->ldv;
             size_t ldv = w->ldv;
-            xch_t* xch = w->xch.p;
+            xch_t* xch = w->xch.p();
             long spin = XCH_SPIN;
             if (const char* e = getenv("PHG_XCH_SPIN")) spin = std::max(1L, atol(e));
             void* args[] = {&n_, &rpt_, &width, &col, &val, &width64, &col64, &val64, &b, &x,
